@@ -1,0 +1,139 @@
+/*
+ * esrrf.h -- C ABI of the MI355X eSRRF reconstruction engine (libesrrf.so).
+ *
+ * This is the boundary the Fiji plugin's host class LiveSRRF_CL
+ * (src/nanoj/liveSRRF/LiveSRRF_CL.java) binds through Java FFM or JNI in place of its
+ * JOCL/OpenCL calls.  Every entry point below names the LiveSRRF_CL method it replaces.
+ * Plain pointers and sizes only; the caller owns host memory, the engine owns device
+ * memory.  All calls return 0 (ESRRF_OK) or a positive ESRRF_ERR_* code; the message of
+ * the last failure of an engine is esrrf_last_error().  One engine may be driven from
+ * several threads: every call takes the engine's mutex (LiveSRRF_CL.calculateSRRF is
+ * `synchronized`, LiveSRRF_CL.java:636).
+ */
+#ifndef ESRRF_H
+#define ESRRF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESRRF_OK 0
+#define ESRRF_ERR_INVALID 1     /* bad argument or size */
+#define ESRRF_ERR_HIP 2         /* HIP runtime failure (message has the HIP error) */
+#define ESRRF_ERR_STATE 3       /* call not valid in the engine's current state */
+#define ESRRF_ERR_NOMEM 4       /* device allocation failed */
+#define ESRRF_ERR_UNSUPPORTED 5 /* configuration outside what the engine implements */
+
+#define ESRRF_U16 0 /* frames as uint16 (ImageJ 16-bit stacks) */
+#define ESRRF_F32 1 /* frames as float32 */
+
+typedef struct esrrf_engine esrrf_engine;
+
+/* Arguments of LiveSRRF_CL.initialise (LiveSRRF_CL.java:159).  The 3-D calibration table
+ * that initialise reads from calibTablePath3DSRRF (:207-227) is passed as arrays: set
+ * n_planes > 0 to select 3-D (MFM) mode, which like the reference forces BIC
+ * interpolation (:166-168). */
+typedef struct esrrf_params {
+    int width;                /* raw frame width (3-D: full 3x3 montage width)   */
+    int height;               /* raw frame height (3-D: full montage height)     */
+    int magnification;
+    float fwhm;               /* the GUI's "Radius" (gui/LiveSRRF_optimised_.java:606) */
+    float sensitivity;
+    int n_frames_on_gpu;      /* frames per calculate call (one GPU load)        */
+    int n_frame_for_srrf;     /* frames per SR frame                              */
+    int block_length;         /* accepted for API parity; the engine tiles itself */
+    int device;               /* HIP device index; -1 = max CUs x clock (:121-137) */
+    int int_weighting;
+    int do_mp_map_correction;
+    int do_fht_interpolation;
+    int n_planes;             /* 0 = 2-D; >0 = 3-D with this many planes          */
+    const double *shift_x_3d; /* [n_planes] "X-shift (pixels)"                   */
+    const double *shift_y_3d; /* [n_planes] "Y-shift (pixels)"                   */
+    const double *theta_3d;   /* [n_planes] "Theta (degrees)"                    */
+    const double *roi_3d;     /* [n_planes] "ROI #"                              */
+    const double *int_coeffs_3d; /* [n_planes] "Intensity scaling"               */
+    float background_3d;      /* "Background level (ADC)"[0]                     */
+    float axial_offset;       /* nm (:236)                                       */
+    float pixel_size;         /* um (:236)                                       */
+} esrrf_params;
+
+/* LiveSRRF_CL.checkDevices (LiveSRRF_CL.java:111-155): number of HIP devices. */
+int esrrf_device_count(int *n);
+/* CLDevice.getName() of the device list built by checkDevices (:126-153). */
+int esrrf_device_name(int device, char *buf, int len);
+/* Device with the largest compute units x max clock product (:121-137). */
+int esrrf_fastest_device(int *device);
+
+/* LiveSRRF_CL.initialise (:159-264) + prepareBuffersBICinterpolation (:266-459) /
+ * prepareBuffersFHTinterpolation (:462-579): allocate, derive constants, build tables. */
+int esrrf_create(const esrrf_params *p, esrrf_engine **out);
+/* LiveSRRF_CL.release (:939-951). */
+int esrrf_destroy(esrrf_engine *e);
+
+/* LiveSRRF_CL.loadDriftXYGPUbuffer (:583-595): drift in raw pixels, stored x M. */
+int esrrf_load_drift(esrrf_engine *e, const float *drift_x, const float *drift_y, int n);
+/* LiveSRRF_CL.resetFramePosition (:955-960): start a new SR frame. */
+int esrrf_reset_frame_position(esrrf_engine *e);
+
+/* LiveSRRF_CL.calculateSRRF (:636-746): push n_frames host frames (one GPU load) of
+ * width x height (dtype ESRRF_U16 / ESRRF_F32), run gradients + RGC + accumulation.
+ * `cancel` (may be NULL) is polled between launch chunks, like IJ.escapePressed()
+ * (:731-734); *cancelled receives the method's boolean result. */
+int esrrf_calculate(esrrf_engine *e, const void *frames, int dtype, int n_frames,
+                    const volatile int *cancel, int *cancelled);
+/* Same with frames already resident in device memory (no counterpart: the reference
+ * always uploads from the JVM heap). */
+int esrrf_calculate_device(esrrf_engine *e, const void *d_frames, int dtype, int n_frames,
+                           const volatile int *cancel, int *cancelled);
+
+/* LiveSRRF_CL.readSRRFbuffer (:749-794): VAR/TAC2 finalise, MP correction (BIC and 3-D
+ * only, :762), NaN -> 0, copy out [4][n_planes_m][hM][wM] (AVG, VAR, TAC2, INT). */
+int esrrf_read_result(esrrf_engine *e, float *out);
+int esrrf_read_result_device(esrrf_engine *e, float *d_out);
+
+/* LiveSRRF_CL.readGradientBuffers (:797-878): frame 0 of the current load; BIC/3-D: per
+ * plane GxInt, GyInt[, GzInt] of (2w x 2h); FHT: Gx, Gy of (wM x hM). */
+int esrrf_read_gradients(esrrf_engine *e, float *out);
+/* LiveSRRF_CL.readAlignedPixels (:881-905): 3-D aligned planes of frame 0. */
+int esrrf_read_aligned(esrrf_engine *e, float *out);
+/* LiveSRRF_CL.readMPmaps (:909-935): [3 (x M in 3-D)][M][M]. */
+int esrrf_read_mp_maps(esrrf_engine *e, float *out);
+/* The nCurrentFrame[2] buffer (liveSRRF.cl:245-248): [0] frame in SR frame, [1] frame in
+ * load. */
+int esrrf_read_frame_counters(esrrf_engine *e, int *out2);
+/* Public fields widthS, heightS, nPlanes, nPlanesM (:41-43) and the magnified size. */
+int esrrf_get_dims(esrrf_engine *e, int *width_m, int *height_m, int *n_planes_m,
+                   int *width_s, int *height_s, int *n_planes);
+/* LiveSRRF_CL.finishQueue (:1009-1015). */
+int esrrf_finish(esrrf_engine *e);
+
+/* Run on the caller's HIP stream (NULL restores the engine's own stream). */
+int esrrf_set_stream(esrrf_engine *e, void *hip_stream);
+/* Kernel timing with HIP events on the launch stream (NanoJProfiler equivalent,
+ * LiveSRRF_CL.java:67).  which: 0 = RGC kernel, 1 = gradient prep, 2 = upsampling. */
+int esrrf_profile_enable(esrrf_engine *e, int enable);
+int esrrf_profile_read(esrrf_engine *e, int which, double *total_ms, long *launches);
+
+/* Constants of prepareBuffersBICinterpolation (:304-305,317-320). */
+int esrrf_bic_constants(float fwhm, float *sigma, float *radius, float *tSS, float *tSO);
+/* Constants of prepareBuffersFHTinterpolation (:491-500). */
+int esrrf_fht_constants(float fwhm, int magnification, float *sigma, float *radius,
+                        float *tSS, float *tSO);
+/* Gated RGC sample count of one frame (geometry only) -- the algorithmic work unit. */
+int esrrf_active_samples(esrrf_engine *e, long long *per_frame);
+
+/* Rolling-analysis bookkeeping of gui/LiveSRRF_optimised_.java:621-647, 842-877.
+ * out6 = {nFrameForSRRF, frameGap, nSRRFframe, nFrameOnGPU, nGPUloadPerSRRFframe, 0}. */
+int esrrf_plan_rolling(int n_slices, int n_frame_for_srrf_user, int frame_gap_user,
+                       int rolling, int max_frames_on_gpu, int *out6);
+/* Load l of SR frame r (1-based) of that plan: first stack index (1-based) and size
+ * (gui/LiveSRRF_optimised_.java:281,311-321). */
+int esrrf_plan_load(const int *plan6, int r, int l, int *first_index, int *n_frames);
+
+const char *esrrf_last_error(esrrf_engine *e);
+const char *esrrf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ESRRF_H */

@@ -1,0 +1,13 @@
+"""eSRRF reconstruction on MI355X (gfx950): HIP kernels behind a C ABI.
+
+The package directory is named ``nanoj-esrrf_amd``; load it with :func:`load` from
+``__graft_entry__`` (or put the directory on ``sys.path`` and import ``_capi`` /
+``liveSRRF_CL`` directly).
+
+* ``libesrrf.so``      -- HIP kernels + engine (include/esrrf.h), built by build_lib.py
+* ``liveSRRF_CL``      -- host mirror of the reference's LiveSRRF_CL on that ABI
+* ``analysis``         -- the rolling-analysis driver of gui/LiveSRRF_optimised_.java
+* ``synth``            -- seeded synthetic blinking stacks for the workloads
+"""
+from . import _capi  # noqa: F401
+from .liveSRRF_CL import Device, Engine, LiveSRRF_CL  # noqa: F401

@@ -1,0 +1,148 @@
+"""ctypes binding of libesrrf.so (the C ABI declared in include/esrrf.h).
+
+The library must have been built (``python nanoj-esrrf_amd/build_lib.py``); there is no
+fallback path: every entry point raises if the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libesrrf.so")
+
+ESRRF_OK = 0
+ESRRF_U16 = 0
+ESRRF_F32 = 1
+_ERRNAMES = {1: "ESRRF_ERR_INVALID", 2: "ESRRF_ERR_HIP", 3: "ESRRF_ERR_STATE",
+             4: "ESRRF_ERR_NOMEM", 5: "ESRRF_ERR_UNSUPPORTED"}
+
+# Every symbol include/esrrf.h declares.
+EXPORTS = [
+    "esrrf_device_count", "esrrf_device_name", "esrrf_fastest_device", "esrrf_create",
+    "esrrf_destroy", "esrrf_load_drift", "esrrf_reset_frame_position", "esrrf_calculate",
+    "esrrf_calculate_device", "esrrf_read_result", "esrrf_read_result_device",
+    "esrrf_read_gradients", "esrrf_read_aligned", "esrrf_read_mp_maps",
+    "esrrf_read_frame_counters", "esrrf_get_dims", "esrrf_finish", "esrrf_set_stream",
+    "esrrf_profile_enable", "esrrf_profile_read", "esrrf_bic_constants",
+    "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
+    "esrrf_last_error", "esrrf_version",
+]
+
+
+class EsrrfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int), ("magnification", ctypes.c_int),
+        ("fwhm", ctypes.c_float), ("sensitivity", ctypes.c_float),
+        ("n_frames_on_gpu", ctypes.c_int), ("n_frame_for_srrf", ctypes.c_int),
+        ("block_length", ctypes.c_int), ("device", ctypes.c_int),
+        ("int_weighting", ctypes.c_int), ("do_mp_map_correction", ctypes.c_int),
+        ("do_fht_interpolation", ctypes.c_int), ("n_planes", ctypes.c_int),
+        ("shift_x_3d", ctypes.POINTER(ctypes.c_double)),
+        ("shift_y_3d", ctypes.POINTER(ctypes.c_double)),
+        ("theta_3d", ctypes.POINTER(ctypes.c_double)),
+        ("roi_3d", ctypes.POINTER(ctypes.c_double)),
+        ("int_coeffs_3d", ctypes.POINTER(ctypes.c_double)),
+        ("background_3d", ctypes.c_float), ("axial_offset", ctypes.c_float),
+        ("pixel_size", ctypes.c_float),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} "
+                           "(run `python nanoj-esrrf_amd/build_lib.py`)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+    c_int, c_float = ctypes.c_int, ctypes.c_float
+    sig = {
+        "esrrf_device_count": [ip],
+        "esrrf_device_name": [c_int, ctypes.c_char_p, c_int],
+        "esrrf_fastest_device": [ip],
+        "esrrf_create": [ctypes.POINTER(Params), ctypes.POINTER(vp)],
+        "esrrf_destroy": [vp],
+        "esrrf_load_drift": [vp, fp, fp, c_int],
+        "esrrf_reset_frame_position": [vp],
+        "esrrf_calculate": [vp, vp, c_int, c_int, ip, ip],
+        "esrrf_calculate_device": [vp, vp, c_int, c_int, ip, ip],
+        "esrrf_read_result": [vp, fp],
+        "esrrf_read_result_device": [vp, vp],
+        "esrrf_read_gradients": [vp, fp],
+        "esrrf_read_aligned": [vp, fp],
+        "esrrf_read_mp_maps": [vp, fp],
+        "esrrf_read_frame_counters": [vp, ip],
+        "esrrf_get_dims": [vp, ip, ip, ip, ip, ip, ip],
+        "esrrf_finish": [vp],
+        "esrrf_set_stream": [vp, vp],
+        "esrrf_profile_enable": [vp, c_int],
+        "esrrf_profile_read": [vp, c_int, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_long)],
+        "esrrf_bic_constants": [c_float, fp, fp, fp, fp],
+        "esrrf_fht_constants": [c_float, c_int, fp, fp, fp, fp],
+        "esrrf_active_samples": [vp, ctypes.POINTER(ctypes.c_longlong)],
+        "esrrf_plan_rolling": [c_int, c_int, c_int, c_int, c_int, ip],
+        "esrrf_plan_load": [ip, c_int, c_int, ip, ip],
+        "esrrf_last_error": [vp],
+        "esrrf_version": [],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_char_p if name in ("esrrf_last_error", "esrrf_version") else c_int
+    _LIB = L
+    return L
+
+
+def check(rc, handle=None):
+    if rc != ESRRF_OK:
+        msg = ""
+        if handle:
+            m = lib().esrrf_last_error(handle)
+            msg = m.decode() if m else ""
+        raise EsrrfError(rc, msg)
+
+
+def f32ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def bic_constants(fwhm: float):
+    out = [ctypes.c_float() for _ in range(4)]
+    check(lib().esrrf_bic_constants(fwhm, *[ctypes.byref(o) for o in out]))
+    return dict(zip(("sigma", "radius", "tSS", "tSO"), (o.value for o in out)))
+
+
+def fht_constants(fwhm: float, magnification: int):
+    out = [ctypes.c_float() for _ in range(4)]
+    check(lib().esrrf_fht_constants(fwhm, magnification, *[ctypes.byref(o) for o in out]))
+    return dict(zip(("sigma", "radius", "tSS", "tSO"), (o.value for o in out)))
+
+
+def plan_rolling(n_slices, n_frame_for_srrf_user, frame_gap_user, rolling, max_frames_on_gpu):
+    """gui/LiveSRRF_optimised_.java bookkeeping through the C ABI."""
+    out = (ctypes.c_int * 6)()
+    check(lib().esrrf_plan_rolling(n_slices, n_frame_for_srrf_user, frame_gap_user,
+                                   1 if rolling else 0, max_frames_on_gpu, out))
+    plan6 = list(out)
+    loads = []
+    for r in range(1, plan6[2] + 1):
+        for l in range(plan6[4]):
+            fi, nf = ctypes.c_int(), ctypes.c_int()
+            check(lib().esrrf_plan_load(out, r, l, ctypes.byref(fi), ctypes.byref(nf)))
+            loads.append((r, l, fi.value, nf.value))
+    return dict(nFrameForSRRF=plan6[0], gap=plan6[1], nSRRFframe=plan6[2],
+                nFrameOnGPU=plan6[3], nLoads=plan6[4], plan=loads)

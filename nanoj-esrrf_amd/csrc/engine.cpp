@@ -1,0 +1,828 @@
+// engine.cpp -- the eSRRF engine behind the C ABI of include/esrrf.h.
+//
+// It owns what LiveSRRF_CL owns in the reference (LiveSRRF_CL.java:29-102): the device
+// buffers, the derived constants, the two frame counters, and the enqueue order of
+// calculateSRRF / readSRRFbuffer.  The counters live on the host (exact integers, returned
+// by esrrf_read_frame_counters) and are passed to the kernels as arguments, instead of the
+// reference's device-side int[2] bumped by a kernel launch per frame
+// (liveSRRF.cl:439-461, LiveSRRF_CL.java:737-740).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/esrrf.h"
+#include "esrrf_internal.h"
+#include "geometry.h"
+#include "modes.h"
+
+using namespace esrrf;
+
+namespace {
+
+enum Mode { MODE_BIC = 0, MODE_FHT = 1, MODE_3D = 2 };
+
+struct ProfSlot {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0.0;
+    long launches = 0;
+};
+
+}  // namespace
+
+struct esrrf_engine {
+    std::mutex mu;
+    std::string err;
+    int dev = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    Mode mode = MODE_BIC;
+    esrrf_params p{};
+    // sizes
+    int W = 0, H = 0;            // raw frame (3-D: montage)
+    int w = 0, h = 0, nP = 1;    // per plane (widthS, heightS, nPlanes)
+    int M = 1, wM = 0, hM = 0, nPlanesM = 1;
+    long nvox = 0;               // voxels of one output channel
+    float sigma = 0, radius = 0, tSS = 0, tSO = 0;
+    int sens_int = 0;
+    // device memory
+    void *d_in = nullptr;        // staging of host frames (raw dtype)
+    size_t d_in_bytes = 0;
+    float *d_px = nullptr;       // [nF][planes][h][w] float (FHT: magnified frames)
+    float2 *d_U = nullptr;       // unit tap gradients of the load
+    float *d_out = nullptr;      // [4][nvox]
+    float *d_prev = nullptr;     // [nvox]
+    float *d_mp = nullptr;       // MP map
+    float *d_drift = nullptr;    // [2][nFrameForSRRF]
+    float *d_tmp = nullptr;      // read-back staging [4][nvox]
+    float *d_aux = nullptr;      // mode scratch (FHT spectra, 3-D gradients)
+    size_t d_aux_bytes = 0;
+    Tap *d_taps = nullptr;
+    Phase *d_phases = nullptr;
+    int *d_tapk = nullptr;
+    BicGeometry geo;
+    bool tiled = false;          // tiled BIC kernel usable
+    int ppw = 1, nw = 1;
+    std::vector<float> h_drift;  // host copy of the drift buffer (x M)
+    ModeState ms;                // FHT / 3-D state
+    // bookkeeping
+    int counters[2] = {0, 0};
+    int loaded = 0;              // frames of the current load
+    bool has_mp = false;
+    bool profiling = false;
+    ProfSlot prof[3];
+};
+
+namespace {
+
+int fail(esrrf_engine *e, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (e) e->err = buf;
+    return code;
+}
+
+#define HIPCHK(e, x)                                                                     \
+    do {                                                                                 \
+        hipError_t _st = (x);                                                            \
+        if (_st != hipSuccess)                                                           \
+            return fail((e), ESRRF_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_st)); \
+    } while (0)
+
+template <class T>
+int dalloc(esrrf_engine *e, T **ptr, size_t bytes)
+{
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 4;
+    hipError_t st = hipMalloc((void **)ptr, bytes);
+    if (st != hipSuccess) {
+        *ptr = nullptr;
+        return fail(e, ESRRF_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st));
+    }
+    return ESRRF_OK;
+}
+
+void free_all(esrrf_engine *e)
+{
+    void *ptrs[] = {e->d_in, e->d_px, e->d_U, e->d_out, e->d_prev, e->d_mp, e->d_drift,
+                    e->d_tmp, e->d_aux, e->d_taps, e->d_phases, e->d_tapk};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    free_mode_state(e->ms);
+    for (auto &s : e->prof)
+        for (auto &ev : s.pending) {
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+    if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
+}
+
+int sens_exact_int(float s)
+{
+    for (int k = 1; k <= 4; k++)
+        if (s == (float)k) return k;
+    return 0;
+}
+
+// Record an event pair around a launch when profiling (HIP events on the launch stream).
+struct ProfScope {
+    esrrf_engine *e;
+    int slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(esrrf_engine *e_, int s) : e(e_), slot(s)
+    {
+        if (!e->profiling) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, e->stream);
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        (void)hipEventRecord(b, e->stream);
+        e->prof[slot].pending.emplace_back(a, b);
+    }
+};
+
+void prof_collect(esrrf_engine *e)
+{
+    for (auto &s : e->prof) {
+        for (auto &ev : s.pending) {
+            float ms = 0.0f;
+            if (hipEventSynchronize(ev.second) == hipSuccess &&
+                hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+                s.total_ms += ms;
+                s.launches += 1;
+            }
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+        s.pending.clear();
+    }
+}
+
+bool drift_zero(const esrrf_engine *e, int c0, int n)
+{
+    const int N = e->p.n_frame_for_srrf;
+    for (int i = c0; i < c0 + n; i++)
+        if (e->h_drift[i] != 0.0f || e->h_drift[i + N] != 0.0f) return false;
+    return true;
+}
+
+// RGC + accumulation for `n` frames of the current load starting at load index f0,
+// SR-frame index c0 (BIC mode).
+int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
+{
+    const int N = e->p.n_frame_for_srrf;
+    const float rn = 1.0f / (float)N;
+    const float rn1 = 1.0f / (float)(N - 1);
+    if (e->tiled && drift_zero(e, c0, n)) {
+        BicRgcArgs a{};
+        a.U = e->d_U;
+        a.px = e->d_px;
+        a.taps = e->d_taps;
+        a.phases = e->d_phases;
+        a.tapk = e->d_tapk;
+        a.out = e->d_out;
+        a.prev = e->d_prev;
+        a.w = e->w;
+        a.h = e->h;
+        a.M = e->M;
+        a.wM = e->wM;
+        a.hM = e->hM;
+        a.frame0 = f0;
+        a.nframes = n;
+        a.first_init = (c0 == 0);
+        a.rn = rn;
+        a.rn1 = rn1;
+        a.sens = e->p.sensitivity;
+        a.sens_int = e->sens_int;
+        a.int_weighting = e->p.int_weighting ? 1 : 0;
+        a.tac_prev_scaled = 1;
+        a.G0 = e->geo.G0;
+        a.PP = e->geo.PP;
+        a.PH = e->geo.PH;
+        a.TW = e->geo.TW;
+        a.TH = e->geo.TH;
+        a.R2lo = e->geo.R2lo;
+        a.R2hi = e->geo.R2hi;
+        a.xs_log2 = e->geo.xs_log2;
+        a.ncls_x = e->geo.ncls_x;
+        a.ncls_y = e->geo.ncls_y;
+        a.tiles_x = (e->w + 7) / 8;
+        a.tiles_y = (e->h + 7) / 8;
+        a.nw = e->nw;
+        ProfScope ps(e, 0);
+        HIPCHK(e, launch_bic_rgc(a, e->ppw, e->stream));
+        return ESRRF_OK;
+    }
+    GenericRgcArgs g{};
+    g.U = e->d_U;
+    g.px = e->d_px;
+    g.drift = e->d_drift;
+    g.out = e->d_out;
+    g.prev = e->d_prev;
+    g.w = e->w;
+    g.h = e->h;
+    g.M = e->M;
+    g.wM = e->wM;
+    g.hM = e->hM;
+    g.frame0 = f0;
+    g.nframes = n;
+    g.count0 = c0;
+    g.nFrameForSRRF = N;
+    g.rn = rn;
+    g.rn1 = rn1;
+    g.sens = e->p.sensitivity;
+    g.sens_int = e->sens_int;
+    g.int_weighting = e->p.int_weighting ? 1 : 0;
+    g.tac_prev_scaled = 1;
+    g.radius = e->radius;
+    g.tSS = e->tSS;
+    g.tSO = e->tSO;
+    ProfScope ps(e, 0);
+    HIPCHK(e, launch_generic_rgc(g, e->stream));
+    return ESRRF_OK;
+}
+
+int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtype, int n,
+                   const volatile int *cancel, int *cancelled)
+{
+    if (cancelled) *cancelled = 0;
+    if (!frames) return fail(e, ESRRF_ERR_INVALID, "frames is NULL");
+    if (dtype != ESRRF_U16 && dtype != ESRRF_F32)
+        return fail(e, ESRRF_ERR_INVALID, "dtype must be ESRRF_U16 or ESRRF_F32");
+    if (n <= 0 || n > e->p.n_frames_on_gpu)
+        return fail(e, ESRRF_ERR_INVALID, "n_frames=%d outside [1, n_frames_on_gpu=%d]", n,
+                    e->p.n_frames_on_gpu);
+    if (e->counters[0] + n > e->p.n_frame_for_srrf)
+        return fail(e, ESRRF_ERR_STATE,
+                    "frame %d of the SR frame exceeds n_frame_for_srrf=%d (call "
+                    "esrrf_reset_frame_position between SR frames)",
+                    e->counters[0] + n, e->p.n_frame_for_srrf);
+    const size_t esz = dtype == ESRRF_U16 ? 2 : 4;
+    const long npx = (long)e->W * e->H;
+    const void *src = frames;
+    if (!on_device) {
+        HIPCHK(e, hipMemcpyAsync(e->d_in, frames, esz * npx * n, hipMemcpyHostToDevice,
+                                 e->stream));
+        src = e->d_in;
+    }
+    e->counters[1] = 0;   // the gradient kernel resets the load counter (liveSRRF.cl:207)
+    e->loaded = n;
+    int rc = ESRRF_OK;
+    if (e->mode == MODE_BIC) {
+        HIPCHK(e, launch_convert_frames(src, dtype, e->d_px, npx * n, e->stream));
+        {
+            ProfScope ps(e, 1);
+            HIPCHK(e, launch_bic_prep(e->d_px, e->d_U, e->w, e->h, n, e->stream));
+        }
+    } else {
+        ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,
+                     e->d_px, e->d_U, e->d_aux, e->profiling};
+        rc = mode_prepare(e->mode == MODE_FHT ? 1 : 2, c, e->ms, src, dtype, n);
+        if (rc != ESRRF_OK) return fail(e, rc, "%s", e->ms.err.c_str());
+    }
+    const int chunk = cancel ? 16 : n;
+    for (int f0 = 0; f0 < n; f0 += chunk) {
+        const int k = std::min(chunk, n - f0);
+        if (e->mode == MODE_BIC) {
+            rc = run_bic_rgc(e, f0, e->counters[0], k);
+        } else {
+            ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,
+                         e->d_px, e->d_U, e->d_aux, e->profiling};
+            ModeRgc r{};
+            r.frame0 = f0;
+            r.nframes = k;
+            r.count0 = e->counters[0];
+            r.nFrameForSRRF = e->p.n_frame_for_srrf;
+            r.sens = e->p.sensitivity;
+            r.sens_int = e->sens_int;
+            r.int_weighting = e->p.int_weighting ? 1 : 0;
+            r.radius = e->radius;
+            r.tSS = e->tSS;
+            r.tSO = e->tSO;
+            r.drift = e->d_drift;
+            r.drift_zero = drift_zero(e, e->counters[0], k);
+            r.out = e->d_out;
+            r.prev = e->d_prev;
+            ProfScope ps(e, 0);
+            rc = mode_rgc(e->mode == MODE_FHT ? 1 : 2, c, e->ms, r);
+            if (rc != ESRRF_OK) rc = fail(e, rc, "%s", e->ms.err.c_str());
+        }
+        if (rc != ESRRF_OK) return rc;
+        e->counters[0] += k;
+        e->counters[1] += k;
+        if (cancel && *cancel && f0 + k < n) {
+            if (cancelled) *cancelled = 1;
+            return ESRRF_OK;
+        }
+    }
+    return ESRRF_OK;
+}
+
+int finalize_impl(esrrf_engine *e, float *dst_dev)
+{
+    HIPCHK(e, launch_var(e->d_out, e->nvox, e->stream));
+    if (e->p.do_mp_map_correction && e->mode != MODE_FHT) {
+        const int dims = e->mode == MODE_3D ? 3 : 2;
+        HIPCHK(e, launch_mp_map(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, dims, e->stream));
+        HIPCHK(e, launch_mp_correct(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, dims, e->stream));
+        e->has_mp = true;
+    }
+    HIPCHK(e, launch_nan_to_zero_copy(e->d_out, dst_dev, 4 * e->nvox, e->stream));
+    return ESRRF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int esrrf_device_count(int *n)
+{
+    if (!n) return ESRRF_ERR_INVALID;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return ESRRF_OK;
+}
+
+int esrrf_device_name(int device, char *buf, int len)
+{
+    if (!buf || len <= 0) return ESRRF_ERR_INVALID;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ESRRF_ERR_INVALID;
+    snprintf(buf, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+    return ESRRF_OK;
+}
+
+int esrrf_fastest_device(int *device)
+{
+    if (!device) return ESRRF_ERR_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return ESRRF_ERR_HIP;
+    double best = -1;
+    int bi = 0;
+    for (int i = 0; i < n; i++) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, i) != hipSuccess) continue;
+        double fl = (double)prop.multiProcessorCount * prop.clockRate;
+        if (fl > best) {
+            best = fl;
+            bi = i;
+        }
+    }
+    *device = bi;
+    return ESRRF_OK;
+}
+
+int esrrf_bic_constants(float fwhm, float *sigma, float *radius, float *tSS, float *tSO)
+{
+    // LiveSRRF_CL.java:304-305,317-318 (float arithmetic, left to right)
+    volatile float s = fwhm / 2.354f;
+    volatile float g2s = 2.0f * 2.0f;
+    g2s = g2s * s;
+    volatile float r = (float)((int)g2s) / 2.0f;
+    r = r + 1.0f;
+    volatile float tss = 2.0f * s;
+    tss = tss * s;
+    volatile float tso = 2.0f * s;
+    tso = tso + 1.0f;
+    if (sigma) *sigma = s;
+    if (radius) *radius = r;
+    if (tSS) *tSS = tss;
+    if (tSO) *tSO = tso;
+    return ESRRF_OK;
+}
+
+int esrrf_fht_constants(float fwhm, int magnification, float *sigma, float *radius, float *tSS,
+                        float *tSO)
+{
+    // LiveSRRF_CL.java:491-500
+    const float Mf = (float)magnification;
+    volatile float s = fwhm / 2.354f;
+    volatile float t2 = 2.0f * s;
+    volatile float ar = (float)((int)t2);
+    ar = ar + 1.0f;
+    volatile float tss = 2.0f * s;
+    tss = tss * Mf;
+    tss = tss * s;
+    tss = tss * Mf;
+    volatile float tso = 2.0f * s;
+    tso = tso + 1.0f;
+    tso = tso * Mf;
+    volatile float r = Mf * ar;
+    if (sigma) *sigma = s;
+    if (radius) *radius = r;
+    if (tSS) *tSS = tss;
+    if (tSO) *tSO = tso;
+    return ESRRF_OK;
+}
+
+int esrrf_create(const esrrf_params *p, esrrf_engine **out)
+{
+    if (!p || !out) return ESRRF_ERR_INVALID;
+    *out = nullptr;
+    esrrf_engine *e = new esrrf_engine();
+    e->p = *p;
+    auto bad = [&](int code) {
+        free_all(e);
+        std::string msg = e->err;
+        delete e;
+        (void)msg;
+        return code;
+    };
+    if (p->width <= 0 || p->height <= 0 || p->magnification < 1 || p->n_frames_on_gpu < 1 ||
+        p->n_frame_for_srrf < 1 || !(p->fwhm > 0.0f) || !(p->sensitivity == p->sensitivity)) {
+        return bad(fail(e, ESRRF_ERR_INVALID, "invalid parameters"));
+    }
+    int dev = p->device;
+    if (dev < 0 && esrrf_fastest_device(&dev) != ESRRF_OK) return bad(ESRRF_ERR_HIP);
+    e->dev = dev;
+    if (hipSetDevice(dev) != hipSuccess) return bad(ESRRF_ERR_HIP);
+    if (hipStreamCreate(&e->own_stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
+    e->stream = e->own_stream;
+    const bool do3d = p->n_planes > 0;
+    e->mode = do3d ? MODE_3D : (p->do_fht_interpolation ? MODE_FHT : MODE_BIC);
+    e->M = p->magnification;
+    e->W = p->width;
+    e->H = p->height;
+    if (do3d) {
+        if (!p->shift_x_3d || !p->shift_y_3d || !p->theta_3d || !p->roi_3d || !p->int_coeffs_3d)
+            return bad(ESRRF_ERR_INVALID);
+        e->nP = p->n_planes;
+        e->w = p->width / 3;    // nSplits = 3 (LiveSRRF_CL.java:39,229-230)
+        e->h = p->height / 3;
+        e->nPlanesM = e->nP * e->M;
+    } else {
+        e->nP = 1;
+        e->w = p->width;
+        e->h = p->height;
+        e->nPlanesM = 1;
+    }
+    if (e->w < 4 || e->h < 4)
+        return bad(fail(e, ESRRF_ERR_INVALID, "frames must be at least 4x4 pixels (per plane)"));
+    e->wM = e->w * e->M;
+    e->hM = e->h * e->M;
+    e->nvox = (long)e->wM * e->hM * (do3d ? (long)e->nP * e->M : 1);
+    e->sens_int = sens_exact_int(p->sensitivity);
+    if (e->mode == MODE_FHT)
+        esrrf_fht_constants(p->fwhm, e->M, &e->sigma, &e->radius, &e->tSS, &e->tSO);
+    else
+        esrrf_bic_constants(p->fwhm, &e->sigma, &e->radius, &e->tSS, &e->tSO);
+
+    const int nF = p->n_frames_on_gpu;
+    const long npx = (long)e->W * e->H;
+    e->d_in_bytes = (size_t)npx * nF * 4;
+    int rc;
+    if ((rc = dalloc(e, &e->d_in, e->d_in_bytes))) return bad(rc);
+    if ((rc = dalloc(e, &e->d_out, sizeof(float) * 4 * e->nvox))) return bad(rc);
+    if ((rc = dalloc(e, &e->d_tmp, sizeof(float) * 4 * e->nvox))) return bad(rc);
+    if ((rc = dalloc(e, &e->d_prev, sizeof(float) * e->nvox))) return bad(rc);
+    if ((rc = dalloc(e, &e->d_drift, sizeof(float) * 2 * p->n_frame_for_srrf))) return bad(rc);
+    const int Md = do3d ? e->M * e->M * e->M : e->M * e->M;
+    if ((rc = dalloc(e, &e->d_mp, sizeof(float) * 3 * Md))) return bad(rc);
+    // defined initial state (the reference leaves fresh OpenCL buffers uninitialised)
+    if (hipMemsetAsync(e->d_out, 0, sizeof(float) * 4 * e->nvox, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->d_prev, 0, sizeof(float) * e->nvox, e->stream) != hipSuccess ||
+        hipMemsetAsync(e->d_drift, 0, sizeof(float) * 2 * p->n_frame_for_srrf, e->stream) !=
+            hipSuccess ||
+        hipMemsetAsync(e->d_mp, 0, sizeof(float) * 3 * Md, e->stream) != hipSuccess)
+        return bad(fail(e, ESRRF_ERR_HIP, "hipMemsetAsync failed"));
+    e->h_drift.assign(2 * (size_t)p->n_frame_for_srrf, 0.0f);
+
+    if (e->mode == MODE_BIC) {
+        if ((rc = dalloc(e, &e->d_px, sizeof(float) * npx * nF))) return bad(rc);
+        if ((rc = dalloc(e, &e->d_U, sizeof(float2) * 4 * npx * nF))) return bad(rc);
+        if (e->M <= 8 && build_bic_geometry(e->w, e->h, e->M, e->radius, e->tSS, e->tSO, e->geo)) {
+            // phases per wave: one block (<= 16 waves) covers every phase of its tile
+            const int MM = e->M * e->M;
+            int ppw = MM <= 4 ? 1 : 4;
+            if (const char *ov = getenv("ESRRF_BIC_PPW")) {
+                const int v = atoi(ov);
+                if (v >= 1 && v <= 8) ppw = v;
+            }
+            while ((MM + ppw - 1) / ppw > (ppw <= 4 ? 16 : 8)) ppw++;
+            e->ppw = ppw;
+            e->nw = (MM + ppw - 1) / ppw;
+            e->tiled = true;
+            const size_t ntap = std::max<size_t>(e->geo.taps.size(), 1);
+            if ((rc = dalloc(e, &e->d_taps, sizeof(Tap) * ntap))) return bad(rc);
+            if ((rc = dalloc(e, &e->d_tapk, sizeof(int) * ntap))) return bad(rc);
+            if ((rc = dalloc(e, &e->d_phases, sizeof(Phase) * e->geo.phases.size()))) return bad(rc);
+            if (!e->geo.taps.empty()) {
+                if (hipMemcpy(e->d_taps, e->geo.taps.data(), sizeof(Tap) * e->geo.taps.size(),
+                              hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(e->d_tapk, e->geo.tapk.data(), sizeof(int) * e->geo.tapk.size(),
+                              hipMemcpyHostToDevice) != hipSuccess)
+                    return bad(fail(e, ESRRF_ERR_HIP, "table upload failed"));
+            }
+            if (hipMemcpy(e->d_phases, e->geo.phases.data(), sizeof(Phase) * e->geo.phases.size(),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return bad(fail(e, ESRRF_ERR_HIP, "table upload failed"));
+        }
+    } else {
+        ModeCreate mc{};
+        mc.mode = e->mode == MODE_FHT ? 1 : 2;
+        mc.w = e->w;
+        mc.h = e->h;
+        mc.W = e->W;
+        mc.H = e->H;
+        mc.nP = e->nP;
+        mc.M = e->M;
+        mc.nF = nF;
+        mc.radius = e->radius;
+        mc.tSS = e->tSS;
+        mc.tSO = e->tSO;
+        mc.p = p;
+        rc = mode_create(mc, e->ms, &e->d_px, &e->d_U, &e->d_aux, &e->d_aux_bytes);
+        if (rc != ESRRF_OK) {
+            fail(e, rc, "%s", e->ms.err.c_str());
+            return bad(rc);
+        }
+    }
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
+    *out = e;
+    return ESRRF_OK;
+}
+
+int esrrf_destroy(esrrf_engine *e)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        (void)hipSetDevice(e->dev);
+        (void)hipStreamSynchronize(e->stream);
+        free_all(e);
+    }
+    delete e;
+    return ESRRF_OK;
+}
+
+int esrrf_load_drift(esrrf_engine *e, const float *dx, const float *dy, int n)
+{
+    if (!e || !dx || !dy) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const int N = e->p.n_frame_for_srrf;
+    if (n < N) return fail(e, ESRRF_ERR_INVALID, "drift arrays hold %d < n_frame_for_srrf=%d", n, N);
+    // driftXY[i] = magnification * driftX[i] (LiveSRRF_CL.java:586-589), in all modes
+    for (int i = 0; i < N; i++) {
+        e->h_drift[i] = (float)e->M * dx[i];
+        e->h_drift[i + N] = (float)e->M * dy[i];
+    }
+    (void)hipSetDevice(e->dev);
+    HIPCHK(e, hipMemcpyAsync(e->d_drift, e->h_drift.data(), sizeof(float) * 2 * N,
+                             hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ESRRF_OK;
+}
+
+int esrrf_reset_frame_position(esrrf_engine *e)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->counters[0] = 0;
+    return ESRRF_OK;
+}
+
+int esrrf_calculate(esrrf_engine *e, const void *frames, int dtype, int n_frames,
+                    const volatile int *cancel, int *cancelled)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    return calculate_impl(e, frames, false, dtype, n_frames, cancel, cancelled);
+}
+
+int esrrf_calculate_device(esrrf_engine *e, const void *d_frames, int dtype, int n_frames,
+                           const volatile int *cancel, int *cancelled)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    return calculate_impl(e, d_frames, true, dtype, n_frames, cancel, cancelled);
+}
+
+int esrrf_read_result(esrrf_engine *e, float *out)
+{
+    if (!e || !out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    int rc = finalize_impl(e, e->d_tmp);
+    if (rc) return rc;
+    HIPCHK(e, hipMemcpyAsync(out, e->d_tmp, sizeof(float) * 4 * e->nvox, hipMemcpyDeviceToHost,
+                             e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ESRRF_OK;
+}
+
+int esrrf_read_result_device(esrrf_engine *e, float *d_out)
+{
+    if (!e || !d_out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    return finalize_impl(e, d_out);
+}
+
+int esrrf_read_gradients(esrrf_engine *e, float *out)
+{
+    if (!e || !out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    if (e->loaded == 0) return fail(e, ESRRF_ERR_STATE, "no frames loaded");
+    if (e->mode == MODE_BIC) {
+        const long n = 4L * e->w * e->h;
+        float *tmp = nullptr;
+        int rc = dalloc(e, &tmp, sizeof(float) * 2 * n);
+        if (rc) return rc;
+        hipError_t st = launch_bic_gint(e->d_px, tmp, tmp + n, e->w, e->h, 0, e->stream);
+        if (st == hipSuccess) st = launch_nan_to_zero_copy(tmp, tmp, 2 * n, e->stream);
+        if (st == hipSuccess)
+            st = hipMemcpyAsync(out, tmp, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, e->stream);
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+        (void)hipFree(tmp);
+        if (st != hipSuccess) return fail(e, ESRRF_ERR_HIP, "gradient read-back: %s", hipGetErrorString(st));
+        return ESRRF_OK;
+    }
+    ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,
+                 e->d_px, e->d_U, e->d_aux, e->profiling};
+    int rc = mode_read_gradients(e->mode == MODE_FHT ? 1 : 2, c, e->ms, out);
+    if (rc) return fail(e, rc, "%s", e->ms.err.c_str());
+    return ESRRF_OK;
+}
+
+int esrrf_read_aligned(esrrf_engine *e, float *out)
+{
+    if (!e || !out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->mode != MODE_3D) return fail(e, ESRRF_ERR_STATE, "aligned pixels exist only in 3-D mode");
+    if (e->loaded == 0) return fail(e, ESRRF_ERR_STATE, "no frames loaded");
+    (void)hipSetDevice(e->dev);
+    const long n = (long)e->w * e->h * e->nP;
+    HIPCHK(e, hipMemcpyAsync(out, e->d_px, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (long i = 0; i < n; i++)
+        if (out[i] != out[i]) out[i] = 0.0f;
+    return ESRRF_OK;
+}
+
+int esrrf_read_mp_maps(esrrf_engine *e, float *out)
+{
+    if (!e || !out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->mode == MODE_FHT)
+        return fail(e, ESRRF_ERR_STATE, "no MP map in FHT mode (LiveSRRF_CL.java:762)");
+    (void)hipSetDevice(e->dev);
+    const int Md = e->mode == MODE_3D ? e->M * e->M * e->M : e->M * e->M;
+    HIPCHK(e, hipMemcpyAsync(out, e->d_mp, sizeof(float) * 3 * Md, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (int i = 0; i < 3 * Md; i++)
+        if (out[i] != out[i]) out[i] = 0.0f;
+    return ESRRF_OK;
+}
+
+int esrrf_read_frame_counters(esrrf_engine *e, int *out2)
+{
+    if (!e || !out2) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    out2[0] = e->counters[0];
+    out2[1] = e->counters[1];
+    return ESRRF_OK;
+}
+
+int esrrf_get_dims(esrrf_engine *e, int *width_m, int *height_m, int *n_planes_m, int *width_s,
+                   int *height_s, int *n_planes)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    if (width_m) *width_m = e->wM;
+    if (height_m) *height_m = e->hM;
+    if (n_planes_m) *n_planes_m = e->nPlanesM;
+    if (width_s) *width_s = e->w;
+    if (height_s) *height_s = e->h;
+    if (n_planes) *n_planes = e->nP;
+    return ESRRF_OK;
+}
+
+int esrrf_finish(esrrf_engine *e)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ESRRF_OK;
+}
+
+int esrrf_set_stream(esrrf_engine *e, void *hip_stream)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->stream = hip_stream ? (hipStream_t)hip_stream : e->own_stream;
+    return ESRRF_OK;
+}
+
+int esrrf_profile_enable(esrrf_engine *e, int enable)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    prof_collect(e);
+    for (auto &s : e->prof) {
+        s.total_ms = 0.0;
+        s.launches = 0;
+    }
+    e->profiling = enable != 0;
+    return ESRRF_OK;
+}
+
+int esrrf_profile_read(esrrf_engine *e, int which, double *total_ms, long *launches)
+{
+    if (!e || which < 0 || which > 2) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    prof_collect(e);
+    ProfSlot &s = e->prof[which];
+    if (which == 2) {
+        s.total_ms = e->ms.prof_up_ms;
+        s.launches = e->ms.prof_up_n;
+    }
+    if (total_ms) *total_ms = s.total_ms;
+    if (launches) *launches = s.launches;
+    return ESRRF_OK;
+}
+
+int esrrf_active_samples(esrrf_engine *e, long long *per_frame)
+{
+    if (!e || !per_frame) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->mode == MODE_BIC)
+        *per_frame = count_active_samples_bic(e->w, e->h, e->M, e->radius, e->tSO);
+    else
+        *per_frame = mode_active_samples(e->mode == MODE_FHT ? 1 : 2, e->ms);
+    return ESRRF_OK;
+}
+
+int esrrf_plan_rolling(int n_slices, int n_frame_for_srrf_user, int frame_gap_user, int rolling,
+                       int max_frames_on_gpu, int *out6)
+{
+    if (!out6 || n_slices < 1 || max_frames_on_gpu < 1 || n_frame_for_srrf_user < 0 ||
+        frame_gap_user < 0)
+        return ESRRF_ERR_INVALID;
+    // gui/LiveSRRF_optimised_.java:621-627
+    const int nF = n_frame_for_srrf_user == 0 ? n_slices : std::min(n_slices, n_frame_for_srrf_user);
+    const int gap = (frame_gap_user == 0 || !rolling) ? nF : frame_gap_user;
+    volatile float q = (float)(n_slices - nF) / (float)gap;
+    const int nSR = (int)q + 1;
+    // :856-858 frames per load from the memory-limited maximum
+    volatile float a1 = (float)nF / (float)max_frames_on_gpu;
+    int nOn = (int)std::ceil((double)a1);
+    volatile float a2 = (float)nF / (float)nOn;
+    nOn = (int)std::ceil((double)a2);
+    nOn = std::min(nF, nOn);
+    // :644 loads per SR frame
+    volatile float a3 = (float)nF / (float)nOn;
+    const int nLoads = (int)std::ceil((double)a3);
+    out6[0] = nF;
+    out6[1] = gap;
+    out6[2] = nSR;
+    out6[3] = nOn;
+    out6[4] = nLoads;
+    out6[5] = 0;
+    return ESRRF_OK;
+}
+
+int esrrf_plan_load(const int *plan6, int r, int l, int *first_index, int *n_frames)
+{
+    if (!plan6 || r < 1 || r > plan6[2] || l < 0 || l >= plan6[4]) return ESRRF_ERR_INVALID;
+    // indexStartSRRFframe = (r-1)*gap + 1; nFrameToLoad = min(nOn, nF - nOn*l)  (:281,311)
+    const int start = (r - 1) * plan6[1] + 1;
+    if (first_index) *first_index = start + plan6[3] * l;
+    if (n_frames) *n_frames = std::min(plan6[3], plan6[0] - plan6[3] * l);
+    return ESRRF_OK;
+}
+
+const char *esrrf_last_error(esrrf_engine *e)
+{
+    if (!e) return "null engine";
+    return e->err.c_str();
+}
+
+const char *esrrf_version(void)
+{
+    return "esrrf-mi355x 0.1 (gfx950)";
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// esrrf_internal.h -- shared host/device declarations of the eSRRF engine (not public ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace esrrf {
+
+// One radial-gradient-convergence sample ("tap") of a magnified pixel.
+//   off : float2 index of the tap's unit gradient inside the block's LDS tile,
+//         relative to the lane's base index (see tile layout in k_bic.hip)
+//   a,b : unit vector (dx, dy)/d from the pixel centre to the sample point
+//   W   : distance weight (d * exp(-d^2 / tSS))^4  (liveSRRF.cl:296-297)
+struct Tap {
+    int off;
+    float a;
+    float b;
+    float W;
+};
+
+// Geometry of one (x-class, y-class, sub-pixel phase) of a magnified pixel.
+struct Phase {
+    int first_tap;  // index into the Tap array
+    int ntaps;
+    float wsum;     // sum of W over all taps, in the reference's j-then-i order
+    int du, dv;     // bicubic base u0 - X, v0 - Y of the intensity sample
+    float wx[4];    // Catmull-Rom weights along x for u0-1 .. u0+2
+    float wy[4];
+    int sx, sy;     // sub-pixel phase (xM mod M, yM mod M)
+    int pad[2];
+};
+
+// Launch parameters of the BIC 2-D radial-gradient-convergence kernel.
+struct BicRgcArgs {
+    const float2 *U;       // [nF][2h][2w] unit tap gradients
+    const float *px;       // [nF][h][w] raw frames (float)
+    const Tap *taps;
+    const Phase *phases;   // [ncls_y][ncls_x][M*M]
+    const int *tapk;       // per tap: (kx_rel + 128) | (ky_rel + 128) << 8
+    float *out;            // [4][hM][wM]
+    float *prev;           // [hM][wM]
+    int w, h, M, wM, hM;
+    int frame0, nframes;   // frames [frame0, frame0 + nframes) of the load
+    int first_init;        // 1: the launch's first frame starts the SR frame
+    float rn, rn1;         // 1/nFrameForSRRF, 1/(nFrameForSRRF - 1)
+    float sens;
+    int sens_int;          // 1..4 exact integer power, 0 = general powf
+    int int_weighting;
+    int tac_prev_scaled;   // 1: Prev stores t/(n-1) (BIC quirk)
+    int G0, PP, PH, TW, TH;
+    int R2lo, R2hi;
+    int xs_log2, ncls_x, ncls_y;
+    int tiles_x, tiles_y;
+    int nw;                // waves per block
+    int zero;              // always 0 (defeats loop-invariant hoisting of phase tables)
+};
+
+struct GenericRgcArgs {
+    const float2 *U;
+    const float *px;
+    const float *drift;    // [2][nFrameForSRRF]
+    float *out;
+    float *prev;
+    int w, h, M, wM, hM;
+    int frame0, nframes, count0;
+    int nFrameForSRRF;
+    float rn, rn1;
+    float sens;
+    int sens_int;
+    int int_weighting;
+    int tac_prev_scaled;
+    float radius, tSS, tSO;
+};
+
+// Host helpers implemented in the .hip files (all launches on `s`).
+hipError_t launch_convert_frames(const void *src, int dtype, float *dst, long n, hipStream_t s);
+hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s);
+hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h, int f,
+                           hipStream_t s);
+hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s);
+hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
+hipError_t launch_var(float *out, long nvox, hipStream_t s);
+hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,
+                         hipStream_t s);
+hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int dims,
+                             hipStream_t s);
+hipError_t launch_nan_to_zero_copy(const float *src, float *dst, long n, hipStream_t s);
+
+}  // namespace esrrf

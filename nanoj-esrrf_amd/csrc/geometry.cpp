@@ -1,0 +1,203 @@
+// geometry.cpp -- host-side tap tables of the tiled BIC RGC kernel.
+//
+// For an unshifted frame the RGC geometry of magnified pixel (xM, yM) depends on the
+// sub-pixel phase (xM mod M, yM mod M) and, through the float rounding of the centre
+// xc = (float)((xM + 0.5) / M) (liveSRRF.cl:263), on the binade of the macro-pixel index
+// X = xM / M: for X >= Xs (Xs a power of two >= 1 - min window offset, so that every
+// vx - xc subtraction is exact by Sterbenz), xc = X + round_{ulp(X)}(frac) and every
+// quantity the reference derives (dx, dy, distance, gate, weight) is identical for all X of
+// one binade.  Below Xs the rounding error of xc is < 2^-21 and one representative is used.
+// Each (y-class, x-class, phase) entry lists its taps in the reference's j-then-i order
+// (liveSRRF.cl:279-284) with exactly the reference's float arithmetic.
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "esrrf_internal.h"
+#include "geometry.h"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace esrrf {
+
+static float cubic_host(float x)
+{
+    const float a = 0.5f;
+    if (x < 0.0f) x = -x;
+    float z = 0.0f;
+    if (x < 1.0f) {
+        volatile float t = x * (-a + 2.0f);
+        t = t + (a - 3.0f);
+        volatile float zz = x * x;
+        zz = zz * t;
+        z = zz + 1.0f;
+    } else if (x < 2.0f) {
+        volatile float t1 = -a * x;
+        t1 = t1 * x;
+        t1 = t1 * x;
+        volatile float t2 = 5.0f * a;
+        t2 = t2 * x;
+        t2 = t2 * x;
+        volatile float t3 = 8.0f * a;
+        t3 = t3 * x;
+        volatile float zz = t1 + t2;
+        zz = zz - t3;
+        z = zz + 4.0f * a;
+    }
+    return z;
+}
+
+static int class_of(int X, int xs_log2)
+{
+    if (X < (1 << xs_log2)) return 0;
+    int lg = 0;
+    while ((X >> (lg + 1)) > 0) lg++;
+    return lg - xs_log2 + 1;
+}
+
+static int rep_of(int cls, int xs_log2)
+{
+    return cls == 0 ? (1 << xs_log2) / 2 : (1 << (xs_log2 + cls - 1));
+}
+
+bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO, BicGeometry &g)
+{
+    g.R2lo = -(int)(2.0f * radius);
+    g.R2hi = (int)(2.0f * radius + 1.0f);
+    if (g.R2hi - g.R2lo > 120) return false;
+    g.G0 = 2 * ((-g.R2lo + 1) / 2);
+    int TW = 16 + g.R2hi + g.G0;
+    if (TW & 1) TW++;
+    g.TW = g.TH = TW;
+    g.PW = g.PH = TW / 2;
+    g.PP = 24;
+    while (g.PP < g.PW) g.PP += 32;
+    int xs = 8;
+    while (xs < 1 - g.R2lo) xs *= 2;
+    int lg = 0;
+    while ((1 << lg) < xs) lg++;
+    g.xs_log2 = lg;
+    g.ncls_x = class_of(w > 0 ? w - 1 : 0, lg) + 1;
+    g.ncls_y = class_of(h > 0 ? h - 1 : 0, lg) + 1;
+    const int MM = M * M;
+    const int planesz = g.PP * g.PH;
+    g.taps.clear();
+    g.tapk.clear();
+    g.phases.assign((size_t)g.ncls_x * g.ncls_y * MM, Phase{});
+    g.max_off = 0;
+    for (int cy = 0; cy < g.ncls_y; cy++) {
+        for (int cx = 0; cx < g.ncls_x; cx++) {
+            const int Xr = rep_of(cx, lg), Yr = rep_of(cy, lg);
+            for (int sy = 0; sy < M; sy++) {
+                for (int sx = 0; sx < M; sx++) {
+                    Phase &P = g.phases[((size_t)cy * g.ncls_x + cx) * MM + sy * M + sx];
+                    const int xM = Xr * M + sx, yM = Yr * M + sy;
+                    const float xc = (float)((xM + 0.5) / M);
+                    const float yc = (float)((yM + 0.5) / M);
+                    const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc);
+                    P.first_tap = (int)g.taps.size();
+                    float wsum = 0.0f;
+                    for (int j = g.R2lo; j <= g.R2hi; j++) {
+                        const float vy = ((float)by + (float)j) / 2.0f;
+                        for (int i = g.R2lo; i <= g.R2hi; i++) {
+                            const float vx = ((float)bx + (float)i) / 2.0f;
+                            volatile float dx = vx - xc;
+                            volatile float dy = vy - yc;
+                            volatile float d2 = dx * dx;
+                            volatile float dy2 = dy * dy;
+                            d2 = d2 + dy2;
+                            const float d = std::sqrt((float)d2);
+                            if (!(d != 0.0f && d <= tSO)) continue;
+                            // distanceWeight = d*exp(-(d*d)/tSS); then w*w*w*w, left to
+                            // right (liveSRRF.cl:296-297)
+                            volatile float e = -(d * d);
+                            e = e / tSS;
+                            volatile float w1 = d * std::exp((float)e);
+                            volatile float W = w1 * w1;
+                            W = W * w1;
+                            W = W * w1;
+                            wsum = wsum + (float)W;
+                            const int kxr = bx - 2 * Xr + i;
+                            const int kyr = by - 2 * Yr + j;
+                            const int gx = kxr + g.G0, gy = kyr + g.G0;
+                            Tap t;
+                            t.off = ((gy & 1) * 2 + (gx & 1)) * planesz + (gy >> 1) * g.PP + (gx >> 1);
+                            t.a = (float)dx / d;
+                            t.b = (float)dy / d;
+                            t.W = (float)W;
+                            if (t.off > g.max_off) g.max_off = t.off;
+                            g.taps.push_back(t);
+                            g.tapk.push_back((kxr + 128) | ((kyr + 128) << 8));
+                        }
+                    }
+                    // pad to a multiple of 4 with zero-weight taps (the kernel walks the
+                    // list in chunks of 4); they add W * h = 0 and are not part of wsum
+                    while (((int)g.taps.size() - P.first_tap) % 4 != 0) {
+                        Tap z;
+                        z.off = 0;
+                        z.a = 0.0f;
+                        z.b = 0.0f;
+                        z.W = 0.0f;
+                        g.taps.push_back(z);
+                        g.tapk.push_back(128 | (128 << 8));
+                    }
+                    P.ntaps = (int)g.taps.size() - P.first_tap;
+                    P.sx = sx;
+                    P.sy = sy;
+                    P.wsum = wsum;
+                    // intensity sample at xM/M - 0.5 (liveSRRF.cl:361)
+                    volatile float xi = (float)xM / (float)M;
+                    xi = xi - 0.5f;
+                    volatile float yi = (float)yM / (float)M;
+                    yi = yi - 0.5f;
+                    const int u0 = (int)std::floor((float)xi), v0 = (int)std::floor((float)yi);
+                    P.du = u0 - Xr;
+                    P.dv = v0 - Yr;
+                    for (int k = 0; k < 4; k++) {
+                        P.wx[k] = cubic_host((float)xi - (float)(u0 - 1 + k));
+                        P.wy[k] = cubic_host((float)yi - (float)(v0 - 1 + k));
+                    }
+                }
+            }
+        }
+    }
+    // every lane (lx, ly in [0, 8)) must stay inside the 4-plane LDS tile
+    if (g.max_off + 7 * g.PP + 7 >= 4 * g.PP * g.PH) return false;
+    if (g.PW > g.PP) return false;
+    return true;
+}
+
+// Gated sample count of one whole frame (border gates included), geometry only: the
+// algorithmic work unit of the RGC (14 FLOP per sample by the survey's convention).
+long long count_active_samples_bic(int w, int h, int M, float radius, float tSO)
+{
+    const int R2lo = -(int)(2.0f * radius), R2hi = (int)(2.0f * radius + 1.0f);
+    long long total = 0;
+    const int wM = w * M, hM = h * M;
+    std::vector<float> xcs(wM), ycs(hM);
+    for (int x = 0; x < wM; x++) xcs[x] = (float)((x + 0.5) / M);
+    for (int y = 0; y < hM; y++) ycs[y] = (float)((y + 0.5) / M);
+    for (int yM = 0; yM < hM; yM++) {
+        const float yc = ycs[yM];
+        const int by = (int)(2.0f * yc);
+        for (int xM = 0; xM < wM; xM++) {
+            const float xc = xcs[xM];
+            const int bx = (int)(2.0f * xc);
+            for (int j = R2lo; j <= R2hi; j++) {
+                const float vy = ((float)by + (float)j) / 2.0f;
+                if (!(vy > 0.0f && vy < (float)h)) continue;
+                const float dy = vy - yc;
+                for (int i = R2lo; i <= R2hi; i++) {
+                    const float vx = ((float)bx + (float)i) / 2.0f;
+                    if (!(vx > 0.0f && vx < (float)w)) continue;
+                    const float dx = vx - xc;
+                    const float d = std::sqrt(dx * dx + dy * dy);
+                    if (d != 0.0f && d <= tSO) total++;
+                }
+            }
+        }
+    }
+    return total;
+}
+
+}  // namespace esrrf

@@ -1,0 +1,25 @@
+// geometry.h -- host-side tap tables of the tiled BIC RGC kernel.
+#pragma once
+#include <vector>
+
+#include "esrrf_internal.h"
+
+namespace esrrf {
+
+struct BicGeometry {
+    int R2lo = 0, R2hi = 0;   // window offsets of liveSRRF.cl:279,283
+    int G0 = 0;               // even margin >= -R2lo of the LDS tile
+    int TW = 0, TH = 0;       // LDS tile extent on the 2x grid
+    int PW = 0, PH = 0, PP = 0;  // parity-plane width, height, pitch (float2)
+    int xs_log2 = 3;
+    int ncls_x = 1, ncls_y = 1;
+    int max_off = 0;
+    std::vector<Tap> taps;
+    std::vector<Phase> phases;
+    std::vector<int> tapk;
+};
+
+bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO, BicGeometry &g);
+long long count_active_samples_bic(int w, int h, int M, float radius, float tSO);
+
+}  // namespace esrrf

@@ -1,0 +1,163 @@
+// k_common.hip -- frame conversion and the read-out epilogues of the eSRRF engine.
+//
+//   VAR / TAC2 finalise          liveSRRF.cl:427-435, live3DSRRF.cl:524-533
+//   macro-pixel (MP) map         liveSRRF.cl:464-487, live3DSRRF.cl:561-588
+//   MP correction                liveSRRF.cl:490-505, live3DSRRF.cl:591-611
+//   NaN -> 0 on read-back        LiveSRRF_CL.java:789
+//
+// All of these are HBM-bound streaming kernels over the 4 output channels.
+#include "esrrf_internal.h"
+
+namespace esrrf {
+
+__global__ void k_convert_u16(const uint16_t *__restrict__ src, float *__restrict__ dst, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = (float)src[i];
+}
+
+__global__ void k_copy_f32(const float *__restrict__ src, float *__restrict__ dst, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+static int grid_for(long n, int block)
+{
+    long g = (n + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+hipError_t launch_convert_frames(const void *src, int dtype, float *dst, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_convert_u16, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           (const uint16_t *)src, dst, n);
+    else
+        hipLaunchKernelGGL(k_copy_f32, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           (const float *)src, dst, n);
+    return hipGetLastError();
+}
+
+// VAR = E[x^2] - AVG^2 and TAC2 -= AVG^2, in place on channels 1 and 2.
+__global__ void k_var(float *__restrict__ out, long nvox)
+{
+    long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; o < nvox; o += stride) {
+        const float a = out[o];
+        const float av2 = a * a;
+        out[o + nvox] = out[o + nvox] - av2;
+        out[o + 2 * nvox] = out[o + 2 * nvox] - av2;
+    }
+}
+
+hipError_t launch_var(float *out, long nvox, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_var, dim3(grid_for(nvox, 256)), dim3(256), 0, s, out, nvox);
+    return hipGetLastError();
+}
+
+// One block per MP-map entry (channel, [z,] y, x sub-position): mean of that sub-position
+// over every macro-pixel (macro-voxel).  The reference sums sequentially in float; this
+// tree-reduces in double, which is at least as accurate.
+__global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
+                                                float *__restrict__ mp, int w, int h, int nP,
+                                                int M, int dims)
+{
+    const int o = blockIdx.x;
+    const int MM = M * M;
+    const int Md = dims == 3 ? MM * M : MM;
+    const int frame = o / Md;
+    const int r = o - frame * Md;
+    const int zMP = dims == 3 ? r / MM : 0;
+    const int yMP = (r - zMP * MM) / M;
+    const int xMP = r - zMP * MM - yMP * M;
+    const long wM = (long)w * M;
+    const long whM = wM * h * M;
+    const long nz = dims == 3 ? nP : 1;
+    const long whdM = whM * (dims == 3 ? nP * M : 1);
+    const long count = (long)w * h * nz;
+    double acc = 0.0;
+    for (long i = threadIdx.x; i < count; i += blockDim.x) {
+        long z = i / ((long)w * h);
+        long rem = i - z * (long)w * h;
+        long y = rem / w;
+        long x = rem - y * w;
+        long off = xMP + x * M + wM * (yMP + y * M) + whM * (zMP + z * M) + frame * whdM;
+        acc += (double)out[off];
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mp[o] = (float)red[0] / (float)count;
+}
+
+__global__ void k_mp_correct(float *__restrict__ out, const float *__restrict__ mp, int w,
+                             int h, int nP, int M, int dims)
+{
+    const long wM = (long)w * M, hM = (long)h * M;
+    const long whM = wM * hM;
+    const long zMn = dims == 3 ? (long)nP * M : 1;
+    const long whdM = whM * zMn;
+    const long n = 3 * whdM;
+    long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    const int MM = M * M;
+    for (; o < n; o += stride) {
+        long frame = o / whdM;
+        long rem = o - frame * whdM;
+        long zM = rem / whM;
+        rem -= zM * whM;
+        long yM = rem / wM;
+        long xM = rem - yM * wM;
+        int sx = (int)(xM % M), sy = (int)(yM % M), sz = (int)(zM % M);
+        long om = dims == 3 ? (long)MM * M * frame + MM * sz + M * sy + sx
+                            : (long)MM * frame + M * sy + sx;
+        out[o] = out[o] / mp[om];
+    }
+}
+
+hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,
+                         hipStream_t s)
+{
+    int nentries = 3 * M * M * (dims == 3 ? M : 1);
+    hipLaunchKernelGGL(k_mp_map, dim3(nentries), dim3(256), 0, s, out, mp, w, h, nP, M, dims);
+    return hipGetLastError();
+}
+
+hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int dims,
+                             hipStream_t s)
+{
+    long n = 3L * w * M * h * M * (dims == 3 ? (long)nP * M : 1);
+    hipLaunchKernelGGL(k_mp_correct, dim3(grid_for(n, 256)), dim3(256), 0, s, out, mp, w, h,
+                       nP, M, dims);
+    return hipGetLastError();
+}
+
+__global__ void k_nan_to_zero(const float *__restrict__ src, float *__restrict__ dst, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        float v = src[i];
+        dst[i] = (v != v) ? 0.0f : v;
+    }
+}
+
+hipError_t launch_nan_to_zero_copy(const float *src, float *dst, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nan_to_zero, dim3(grid_for(n, 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace esrrf

@@ -1,0 +1,428 @@
+"""Oracle for the eSRRF path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module; the product (nanoj-esrrf_amd/) never does.
+
+* ``OracleLiveSRRF`` restates the host engine ``LiveSRRF_CL`` (LiveSRRF_CL.java:159-994):
+  buffer plan, the two frame counters, the enqueue order of calculateSRRF and the
+  read-back of readSRRFbuffer, calling the C restatement of the OpenCL kernels in
+  esrrf_oracle.c (liboracle.so).
+* ``rolling_plan`` / ``frames_on_gpu`` restate the integer bookkeeping of
+  gui/LiveSRRF_optimised_.java:621-647, 276-323, 842-877.
+* ``sweep_plan`` restates gui/ParametersSweep_.java:720-747 and its loop order :328-380.
+* ``rse`` restates the SQUIRREL error of ErrorMapLiveSRRF.java:145-246 with a declared
+  stand-in for ImageJ's blurGaussian (parity unpinned against ImageJ, see below).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import fht as _fht
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class _BicCfg(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("magnification", ctypes.c_int), ("sensitivity", ctypes.c_float),
+                ("radius", ctypes.c_float), ("tSS", ctypes.c_float), ("tSO", ctypes.c_float),
+                ("nFrameForSRRF", ctypes.c_int), ("intWeighting", ctypes.c_int)]
+
+
+class _FhtCfg(ctypes.Structure):
+    _fields_ = [("widthM", ctypes.c_int), ("heightM", ctypes.c_int),
+                ("sensitivity", ctypes.c_float),
+                ("radius", ctypes.c_float), ("tSS", ctypes.c_float), ("tSO", ctypes.c_float),
+                ("nFrameForSRRF", ctypes.c_int), ("intWeighting", ctypes.c_int)]
+
+
+class _Cfg3D(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("nPlanes", ctypes.c_int), ("magnification", ctypes.c_int),
+                ("sensitivity", ctypes.c_float),
+                ("radius", ctypes.c_float), ("tSS", ctypes.c_float), ("tSO", ctypes.c_float),
+                ("nFrameForSRRF", ctypes.c_int), ("intWeighting", ctypes.c_int)]
+
+
+def lib():
+    """Load oracle/liboracle.so (built by oracle/Makefile)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library not built: {path} (run `make -C oracle`)")
+    L = ctypes.CDLL(path)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.orc_bic_constants.argtypes = [ctypes.c_float, ctypes.c_int, fp, fp, fp, fp]
+    L.orc_fht_constants.argtypes = [ctypes.c_float, ctypes.c_int, fp, fp, fp, fp]
+    L.orc_cubic.argtypes = [ctypes.c_float]
+    L.orc_cubic.restype = ctypes.c_float
+    L.orc_interp.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+    L.orc_interp.restype = ctypes.c_float
+    L.orc_gradient_2point.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+    L.orc_gradient_interp.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+    L.orc_bic_rgc_frame.argtypes = [ctypes.POINTER(_BicCfg), _f32p, _f32p, _f32p, _f32p,
+                                    ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int]
+    L.orc_var.argtypes = [_f32p, ctypes.c_long]
+    L.orc_mp_map.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+    L.orc_mp_correct.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+    L.orc_gradient_robx.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+    L.orc_fht_rgc_frame.argtypes = [ctypes.POINTER(_FhtCfg), _f32p, _f32p, _f32p, _f32p,
+                                    ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int]
+    L.orc3d_align.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              _f32p, _f32p]
+    L.orc3d_gradient.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 _f32p, _f32p, _f32p]
+    L.orc3d_interp.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+    L.orc3d_rgc_frame.argtypes = [ctypes.POINTER(_Cfg3D), _f32p, _f32p, _f32p, _f32p, _f32p,
+                                  ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int]
+    L.orc3d_mp_map.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+    L.orc3d_mp_correct.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _f32p]
+    L.orc3d_reshape.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p,
+                                ctypes.c_float, _f32p]
+    _LIB = L
+    return L
+
+
+def _c(a, dt=np.float32):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ----------------------------------------------------------------------------------------
+# Constants
+# ----------------------------------------------------------------------------------------
+
+def bic_constants(fwhm: float, gradient_mag: int = 2):
+    out = [ctypes.c_float() for _ in range(4)]
+    lib().orc_bic_constants(ctypes.c_float(fwhm), gradient_mag, *[ctypes.byref(o) for o in out])
+    s, r, tss, tso = (o.value for o in out)
+    return dict(sigma=s, radius=r, tSS=tss, tSO=tso)
+
+
+def fht_constants(fwhm: float, magnification: int):
+    out = [ctypes.c_float() for _ in range(4)]
+    lib().orc_fht_constants(ctypes.c_float(fwhm), magnification, *[ctypes.byref(o) for o in out])
+    s, r, tss, tso = (o.value for o in out)
+    return dict(sigma=s, radius=r, tSS=tss, tSO=tso)
+
+
+def cubic(x: float) -> float:
+    return lib().orc_cubic(ctypes.c_float(x))
+
+
+def interp(a: np.ndarray, x: float, y: float) -> float:
+    a = _c(a)
+    return lib().orc_interp(a, a.shape[1], a.shape[0], ctypes.c_float(x), ctypes.c_float(y))
+
+
+def gradient_2point(px: np.ndarray):
+    px = _c(px)
+    nF, h, w = px.shape
+    gx = np.empty_like(px)
+    gy = np.empty_like(px)
+    lib().orc_gradient_2point(px, w, h, nF, gx, gy)
+    return gx, gy
+
+
+def gradient_interp(g: np.ndarray):
+    g = _c(g)
+    nF, h, w = g.shape
+    gi = np.empty((nF, 2 * h, 2 * w), np.float32)
+    lib().orc_gradient_interp(g, w, h, nF, gi)
+    return gi
+
+
+def gradient_robx(px: np.ndarray):
+    px = _c(px)
+    nF, h, w = px.shape
+    gx = np.empty_like(px)
+    gy = np.empty_like(px)
+    lib().orc_gradient_robx(px, w, h, nF, gx, gy)
+    return gx, gy
+
+
+# ----------------------------------------------------------------------------------------
+# Engine restatement (LiveSRRF_CL.java)
+# ----------------------------------------------------------------------------------------
+
+@dataclass
+class Calib3D:
+    """The six columns LiveSRRF_CL.initialise reads from the .njt table (:212-220)."""
+    shift_x: np.ndarray
+    shift_y: np.ndarray
+    theta_deg: np.ndarray
+    roi: np.ndarray
+    int_coeffs: np.ndarray
+    background: float = 0.0
+
+
+class OracleLiveSRRF:
+    """Restatement of LiveSRRF_CL's state machine on the CPU.
+
+    initialise (:159-264) -> buffers and constants; resetFramePosition (:955-960);
+    loadDriftXYGPUbuffer (:583-595); calculateSRRF (:636-746); readSRRFbuffer (:749-794);
+    readGradientBuffers (:797-878); readMPmaps (:909-935); readAlignedPixels (:881-905).
+    """
+
+    nReconstructions = 3
+    gradientMag = 2
+
+    def __init__(self, width, height, magnification, fwhm, sensitivity, nFramesOnGPU,
+                 nFrameForSRRF, intWeighting=True, doMPmapCorrection=True,
+                 doFHTinterpolation=False, calib3d: Calib3D | None = None, nthreads=0):
+        self.M = int(magnification)
+        self.nFrameForSRRF = int(nFrameForSRRF)
+        self.nFramesOnGPU = int(nFramesOnGPU)
+        self.doMP = bool(doMPmapCorrection)
+        self.do3D = calib3d is not None
+        self.doFHT = False if self.do3D else bool(doFHTinterpolation)
+        self.intWeighting = 1 if intWeighting else 0
+        self.sensitivity = float(sensitivity)
+        self.nthreads = nthreads
+        self.width, self.height = int(width), int(height)
+        if self.do3D:
+            self.calib = calib3d
+            self.nPlanes = len(calib3d.shift_x)
+            self.widthS, self.heightS = width // 3, height // 3
+            self.nPlanesM = self.nPlanes * self.M
+        else:
+            self.nPlanes, self.widthS, self.heightS, self.nPlanesM = 1, width, height, 1
+        self.wM, self.hM = self.widthS * self.M, self.heightS * self.M
+        if self.doFHT:
+            self.k = fht_constants(fwhm, self.M)
+        else:
+            self.k = bic_constants(fwhm, self.gradientMag)
+        nvox = self.wM * self.hM * (self.nPlanes * self.M if self.do3D else 1)
+        self.nvox = nvox
+        self.out = np.zeros(4 * nvox, np.float32)
+        self.prev = np.zeros(nvox, np.float32)
+        self.mp = None
+        self.drift = np.zeros(2 * self.nFrameForSRRF, np.float32)
+        self.counters = [0, 0]
+        self.px = None
+        self.gx = self.gy = self.gz = None
+        if self.do3D:
+            st = np.zeros(3 * self.nPlanes, np.float32)
+            for i in range(3 * self.nPlanes):   # LiveSRRF_CL.java:440-445
+                if i // self.nPlanes == 0:
+                    st[i] = np.float32(-float(calib3d.theta_deg[i]) * math.pi / 180.0)
+                if i // self.nPlanes == 1:
+                    st[i] = np.float32(calib3d.shift_x[i - self.nPlanes])
+                if i // self.nPlanes == 2:
+                    st[i] = np.float32(calib3d.shift_y[i - 2 * self.nPlanes])
+            self.st = st
+
+    # -- :583-595
+    def loadDriftXYGPUbuffer(self, dx, dy):
+        n = self.nFrameForSRRF
+        d = np.zeros(2 * n, np.float32)
+        for i in range(n):
+            d[i] = np.float32(self.M) * np.float32(dx[i])
+            d[i + n] = np.float32(self.M) * np.float32(dy[i])
+        self.drift = d
+
+    # -- :955-960
+    def resetFramePosition(self):
+        self.counters[0] = 0
+
+    def _cfg(self):
+        k = self.k
+        if self.doFHT:
+            return _FhtCfg(self.wM, self.hM, self.sensitivity, k["radius"], k["tSS"], k["tSO"],
+                           self.nFrameForSRRF, self.intWeighting)
+        if self.do3D:
+            return _Cfg3D(self.widthS, self.heightS, self.nPlanes, self.M, self.sensitivity,
+                          k["radius"], k["tSS"], k["tSO"], self.nFrameForSRRF, self.intWeighting)
+        return _BicCfg(self.widthS, self.heightS, self.M, self.sensitivity, k["radius"],
+                       k["tSS"], k["tSO"], self.nFrameForSRRF, self.intWeighting)
+
+    def reshape3D(self, frames):
+        """reshapeImageStack3D (LiveSRRF_CL.java:963-994)."""
+        c = self.calib
+        n = frames.shape[0]
+        out = np.empty((n, self.nPlanes, self.heightS, self.widthS), np.float32)
+        roi = _c(c.roi, np.float64)
+        ic = _c(c.int_coeffs, np.float64)
+        for f in range(n):
+            tmp = np.empty(self.nPlanes * self.heightS * self.widthS, np.float32)
+            lib().orc3d_reshape(_c(frames[f]), self.width, self.height, self.nPlanes, roi, ic,
+                                ctypes.c_float(c.background), tmp)
+            out[f] = tmp.reshape(self.nPlanes, self.heightS, self.widthS)
+        return out
+
+    # -- :636-746
+    def calculateSRRF(self, frames):
+        frames = np.asarray(frames)
+        n = frames.shape[0]
+        L = lib()
+        cfg = self._cfg()
+        if self.doFHT:
+            if self.M > 1:
+                px = _fht.fht_stack(frames.astype(np.float32), self.M, True)
+            else:
+                px = _c(frames)
+            self.px = _c(px)
+            self.gx, self.gy = gradient_robx(self.px)
+            self.counters[1] = 0
+            for _ in range(n):
+                L.orc_fht_rgc_frame(ctypes.byref(cfg), self.px, self.gx, self.gy, self.drift,
+                                    self.counters[0], self.counters[1], self.out, self.prev,
+                                    self.nthreads)
+                self.counters[0] += 1
+                self.counters[1] += 1
+            return False
+        if self.do3D:
+            px = self.reshape3D(frames)
+            w, h, nP = self.widthS, self.heightS, self.nPlanes
+            al = np.empty(px.size, np.float32)
+            L.orc3d_align(_c(px).ravel(), w, h, nP, n, self.st, al)
+            self.px = al
+            gx = np.empty_like(al)
+            gy = np.empty_like(al)
+            gz = np.empty_like(al)
+            L.orc3d_gradient(al, w, h, nP, n, gx, gy, gz)
+            self.counters[1] = 0
+            gis = []
+            for g in (gx, gy, gz):
+                gi = np.empty(4 * g.size, np.float32)
+                L.orc3d_interp(g, w, h, nP, n, gi)
+                gis.append(gi)
+            self.gx, self.gy, self.gz = gis
+            for _ in range(n):
+                L.orc3d_rgc_frame(ctypes.byref(cfg), self.px, self.gx, self.gy, self.gz,
+                                  self.drift, self.counters[0], self.counters[1], self.out,
+                                  self.prev, self.nthreads)
+                self.counters[0] += 1
+                self.counters[1] += 1
+            return False
+        self.px = _c(frames)
+        gx, gy = gradient_2point(self.px)
+        self.counters[1] = 0
+        self.gx, self.gy = gradient_interp(gx), gradient_interp(gy)
+        for _ in range(n):
+            L.orc_bic_rgc_frame(ctypes.byref(cfg), self.px, self.gx, self.gy, self.drift,
+                                self.counters[0], self.counters[1], self.out, self.prev,
+                                self.nthreads)
+            self.counters[0] += 1
+            self.counters[1] += 1
+        return False
+
+    # -- :749-794
+    def readSRRFbuffer(self):
+        L = lib()
+        L.orc_var(self.out, self.nvox)
+        if self.doMP and not self.doFHT:
+            if self.do3D:
+                self.mp = np.empty(3 * self.M ** 3, np.float32)
+                L.orc3d_mp_map(self.out, self.widthS, self.heightS, self.nPlanes, self.M, self.mp)
+                L.orc3d_mp_correct(self.out, self.widthS, self.heightS, self.nPlanes, self.M,
+                                   self.mp)
+            else:
+                self.mp = np.empty(3 * self.M ** 2, np.float32)
+                L.orc_mp_map(self.out, self.widthS, self.heightS, self.M, self.mp)
+                L.orc_mp_correct(self.out, self.widthS, self.heightS, self.M, self.mp)
+        res = self.out.copy()
+        res[np.isnan(res)] = 0
+        return res.reshape(4 * self.nPlanesM, self.hM, self.wM)
+
+    # -- :797-878 (frame 0 of the current load, every plane)
+    def readGradientBuffers(self):
+        if self.doFHT:
+            n = self.wM * self.hM
+            g = [self.gx.reshape(-1)[:n].reshape(self.hM, self.wM),
+                 self.gy.reshape(-1)[:n].reshape(self.hM, self.wM)]
+            return np.stack(g)
+        wi, hi = 2 * self.widthS, 2 * self.heightS
+        sl = []
+        for p in range(self.nPlanes):
+            for g in ((self.gx, self.gy, self.gz) if self.do3D else (self.gx, self.gy)):
+                sl.append(g.reshape(-1)[p * wi * hi:(p + 1) * wi * hi].reshape(hi, wi))
+        out = np.stack(sl)
+        out[np.isnan(out)] = 0
+        return out
+
+    def readMPmaps(self):
+        m = self.mp.reshape(-1, self.M, self.M).copy()
+        m[np.isnan(m)] = 0
+        return m
+
+    def readAlignedPixels(self):
+        n = self.widthS * self.heightS
+        return np.stack([self.px[p * n:(p + 1) * n].reshape(self.heightS, self.widthS)
+                         for p in range(self.nPlanes)])
+
+
+# ----------------------------------------------------------------------------------------
+# Integer bookkeeping (gui/LiveSRRF_optimised_.java)
+# ----------------------------------------------------------------------------------------
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def frames_for_srrf(nSlices: int, nFromUser: int) -> int:
+    # :621-622
+    return nSlices if nFromUser == 0 else min(nSlices, nFromUser)
+
+
+def frame_gap(nFrameForSRRF: int, gapFromUser: int, rolling: bool) -> int:
+    # :624-625
+    return nFrameForSRRF if (gapFromUser == 0 or not rolling) else gapFromUser
+
+
+def n_srrf_frames(nSlices: int, nFrameForSRRF: int, gap: int) -> int:
+    # :627  (int)((float)(nSlices - nF) / (float) gap) + 1
+    return int(_f32(_f32(nSlices - nFrameForSRRF) / _f32(gap))) + 1
+
+
+def frames_on_gpu(nFrameForSRRF: int, maxFramesOnGPU: int) -> int:
+    # :856-858  two float ceilings then min
+    n = int(math.ceil(_f32(_f32(nFrameForSRRF) / _f32(maxFramesOnGPU))))
+    n = int(math.ceil(_f32(_f32(nFrameForSRRF) / _f32(n))))
+    return min(nFrameForSRRF, n)
+
+
+def loads_per_srrf_frame(nFrameForSRRF: int, nFrameOnGPU: int) -> int:
+    # :644
+    return int(math.ceil(_f32(_f32(nFrameForSRRF) / _f32(nFrameOnGPU))))
+
+
+def rolling_plan(nSlices, nFromUser, gapFromUser, rolling, maxFramesOnGPU):
+    """Every (sr_frame, load, first_stack_index(1-based), n_frames) of :276-323."""
+    nF = frames_for_srrf(nSlices, nFromUser)
+    gap = frame_gap(nF, gapFromUser, rolling)
+    nSR = n_srrf_frames(nSlices, nF, gap)
+    nOnGPU = frames_on_gpu(nF, maxFramesOnGPU)
+    nLoads = loads_per_srrf_frame(nF, nOnGPU)
+    plan = []
+    for r in range(1, nSR + 1):
+        start = (r - 1) * gap + 1
+        for l in range(nLoads):
+            n = min(nOnGPU, nF - nOnGPU * l)
+            plan.append((r, l, start + nOnGPU * l, n))
+    return dict(nFrameForSRRF=nF, gap=gap, nSRRFframe=nSR, nFrameOnGPU=nOnGPU,
+                nLoads=nLoads, plan=plan)
+
+
+def sweep_plan(fwhm0, dfwhm, n_fwhm, S0, dS, n_S, nf0, dnf, n_nf, nSlices, frc=False):
+    """gui/ParametersSweep_.java:720-747 arrays and the nf -> S -> fwhm loop order."""
+    fwhm = [_f32(np.float32(fwhm0) + np.float32(i) * np.float32(dfwhm)) for i in range(n_fwhm)]
+    sens = [S0 + i * dS for i in range(n_S)]
+    if frc:
+        n_use = min((nSlices // 2 - nf0) // dnf + 1, n_nf)
+    else:
+        n_use = min((nSlices - nf0) // dnf + 1, n_nf)
+    nfs = [nf0 + i * dnf for i in range(max(n_use, 0))]
+    order = [(nf, s, f) for nf in nfs for s in sens for f in fwhm]
+    return dict(fwhm=fwhm, sensitivity=sens, nframes=nfs, order=order)

@@ -1,0 +1,36 @@
+"""The C-ABI library loads and exports every symbol include/esrrf.h declares (CPU only:
+no compute calls)."""
+import ctypes
+import os
+import re
+
+from conftest import REPO
+
+
+def declared():
+    txt = open(os.path.join(REPO, "include", "esrrf.h")).read()
+    return sorted(set(re.findall(r"\b(esrrf_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree(pkg):
+    assert sorted(pkg._capi.EXPORTS) == declared()
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    L = ctypes.CDLL(pkg._capi.LIB_PATH)
+    missing = [s for s in declared() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_library_has_gfx950_code_object(pkg):
+    blob = open(pkg._capi.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_host_only_entry_points(pkg):
+    # pure host computations of the ABI (no device touched)
+    k = pkg._capi.bic_constants(1.5)
+    assert abs(k["radius"] - 2.0) < 1e-7
+    assert abs(k["tSO"] - 2.2744265) < 1e-6
+    f = pkg._capi.fht_constants(1.5, 5)
+    assert f["radius"] == 10.0

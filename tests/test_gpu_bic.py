@@ -1,0 +1,137 @@
+"""BIC-mode (liveSRRF.cl) parity of the HIP engine against the CPU oracle.
+
+Tolerance (the port's contract): per channel, both images divided by the oracle image's
+max |value|, |a - b| <= 1e-5 + 1e-4 |b|.  Frame counters and schedules: exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import normalised_close
+
+pytestmark = pytest.mark.gpu
+
+
+def stack(n, w, h, seed, emitters=None, fil=4):
+    from nanoj_esrrf_amd import synth
+    return synth.blinking_stack(n, w, h, emitters or max(20, w * h // 14), fil, seed=seed)
+
+
+def run_engine(pkg, st, M, fwhm, S, nOnGPU=None, nFor=None, intW=True, mp=True, drift=None,
+               loads=None):
+    n = st.shape[0]
+    nOnGPU = nOnGPU or n
+    nFor = nFor or n
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(st.shape[2], st.shape[1], M, fwhm, S, nOnGPU, nFor, 20000,
+                 e.allCLdevices[0], intW, mp, False, None, 0, 1)
+    e.resetFramePosition()
+    dx, dy = (np.zeros(nFor), np.zeros(nFor)) if drift is None else drift
+    e.loadDriftXYGPUbuffer(dx, dy)
+    for a, b in (loads or [(0, n)]):
+        e.calculateSRRF(st[a:b])
+    out = e.readSRRFbuffer()
+    counters = e.frameCounters()
+    mpm = e.readMPmaps() if mp else None
+    e.release()
+    return out, counters, mpm
+
+
+def run_oracle(O, st, M, fwhm, S, nOnGPU=None, nFor=None, intW=True, mp=True, drift=None,
+               loads=None):
+    n = st.shape[0]
+    o = O.OracleLiveSRRF(st.shape[2], st.shape[1], M, fwhm, S, nOnGPU or n, nFor or n, intW, mp,
+                         False)
+    if drift is not None:
+        o.loadDriftXYGPUbuffer(*drift)
+    for a, b in (loads or [(0, n)]):
+        o.calculateSRRF(st[a:b].astype(np.float32))
+    return o.readSRRFbuffer(), list(o.counters), o.mp
+
+
+def assert_parity(got, exp, what):
+    for c, name in enumerate(("AVG", "VAR", "TAC2", "INT")):
+        ok, ex, idx = normalised_close(got[c], exp[c])
+        assert ok, f"{what}: {name} differs (excess {ex:.3g} at {idx})"
+
+
+def test_plain_64x64x100_m5_parity(pkg, O):
+    """The CPU-runnable workload: 100 frames of 64x64, M=5, fwhm 1.5, S=1, AVG + MP."""
+    from nanoj_esrrf_amd import synth
+    st = synth.plain_64x64x100()
+    got, cnt, _ = run_engine(pkg, st, 5, 1.5, 1)
+    exp, ocnt, _ = run_oracle(O, st, 5, 1.5, 1)
+    assert cnt == ocnt == [100, 100]
+    assert_parity(got.reshape(4, 320, 320), exp.reshape(4, 320, 320), "64x64x100 M5")
+
+
+@pytest.mark.parametrize("M,fwhm,S,w,h", [
+    (1, 1.5, 1, 21, 17), (2, 1.0, 2, 40, 24), (3, 2.0, 1, 19, 33), (4, 2.5, 2, 30, 30),
+    (5, 3.0, 3, 27, 18), (5, 1.5, 2, 33, 40), (6, 1.2, 1, 16, 16), (8, 1.5, 2, 20, 14),
+    (9, 1.5, 2, 12, 12), (5, 1.5, 1.7, 24, 24),
+])
+def test_bic_configs_parity(pkg, O, M, fwhm, S, w, h):
+    st = stack(7, w, h, seed=100 + M * 7 + w)
+    got, cnt, _ = run_engine(pkg, st, M, fwhm, S)
+    exp, ocnt, _ = run_oracle(O, st, M, fwhm, S)
+    assert cnt == ocnt
+    assert_parity(got.reshape(4, h * M, w * M), exp.reshape(4, h * M, w * M),
+                  f"M{M} fwhm{fwhm} S{S} {w}x{h}")
+
+
+def test_bic_no_intensity_weighting(pkg, O):
+    st = stack(6, 24, 20, seed=5)
+    got, _, _ = run_engine(pkg, st, 4, 1.5, 2, intW=False)
+    exp, _, _ = run_oracle(O, st, 4, 1.5, 2, intW=False)
+    assert_parity(got.reshape(4, 80, 96), exp.reshape(4, 80, 96), "intWeighting off")
+
+
+def test_bic_uneven_loads_and_counters(pkg, O):
+    """One SR frame of 11 frames pushed as loads of 4, 4, 3 (gui/LiveSRRF_optimised_.java
+    :309-321): accumulators persist across launches, counters stay exact."""
+    st = stack(11, 24, 24, seed=11)
+    loads = [(0, 4), (4, 8), (8, 11)]
+    got, cnt, _ = run_engine(pkg, st, 5, 1.5, 2, nOnGPU=4, nFor=11, loads=loads)
+    exp, ocnt, _ = run_oracle(O, st, 5, 1.5, 2, nOnGPU=4, nFor=11, loads=loads)
+    assert cnt == ocnt == [11, 3]
+    assert_parity(got.reshape(4, 120, 120), exp.reshape(4, 120, 120), "uneven loads")
+
+
+def test_bic_drift_generic_path(pkg, O):
+    """Per-frame drift (x M on upload, LiveSRRF_CL.java:587-588) takes the per-pixel path."""
+    st = stack(5, 20, 20, seed=3)
+    rng = np.random.default_rng(0)
+    dx = rng.uniform(-0.3, 0.3, 5).astype(np.float32)
+    dy = rng.uniform(-0.3, 0.3, 5).astype(np.float32)
+    got, _, _ = run_engine(pkg, st, 3, 1.5, 1, drift=(dx, dy))
+    exp, _, _ = run_oracle(O, st, 3, 1.5, 1, drift=(dx, dy))
+    assert_parity(got.reshape(4, 60, 60), exp.reshape(4, 60, 60), "drift")
+
+
+def test_bic_mp_map_and_gradients(pkg, O):
+    st = stack(4, 18, 22, seed=9)
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(18, 22, 3, 1.5, 1, 4, 4, 20000, e.allCLdevices[0], True, True, False, None, 0, 1)
+    e.resetFramePosition()
+    e.calculateSRRF(st)
+    g = e.readGradientBuffers()
+    e.readSRRFbuffer()
+    mp = e.readMPmaps()
+    e.release()
+    o = O.OracleLiveSRRF(18, 22, 3, 1.5, 1, 4, 4, True, True, False)
+    o.calculateSRRF(st.astype(np.float32))
+    og = o.readGradientBuffers()
+    o.readSRRFbuffer()
+    # gradients of integer data are exact (Catmull-Rom half-point weights are dyadic)
+    assert np.array_equal(g, og)
+    np.testing.assert_allclose(mp.reshape(-1), o.mp, rtol=1e-4)
+
+
+def test_bic_constant_stack(pkg):
+    """Constant frames: zero gradient -> CGLH = 0 -> AVG = VAR = TAC2 = 0, INT = value."""
+    st = np.full((5, 16, 16), 123, np.uint16)
+    got, _, _ = run_engine(pkg, st, 3, 1.5, 1, mp=False)
+    got = got.reshape(4, 48, 48)
+    assert np.all(got[0] == 0) and np.all(got[1] == 0) and np.all(got[2] == 0)
+    np.testing.assert_allclose(got[3], 123.0, rtol=1e-6)
