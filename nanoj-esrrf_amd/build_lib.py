@@ -34,6 +34,8 @@ DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # keep the RGC kernel's per-phase accumulators in VGPRs under uniform indexing
+    "-mllvm", "-amdgpu-promote-alloca-to-vector-limit=1024",
 ]
 HOST_FLAGS = ["-x", "c++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off",
               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]

@@ -69,7 +69,7 @@ struct esrrf_engine {
     int *d_tapk = nullptr;
     BicGeometry geo;
     bool tiled = false;          // tiled BIC kernel usable
-    int ppw = 1, nw = 1;
+    int ppw = 1, nw = 1, stage = 0, taps_in_lds = 0;
     std::vector<float> h_drift;  // host copy of the drift buffer (x M)
     ModeState ms;                // FHT / 3-D state
     // bookkeeping
@@ -225,6 +225,9 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
         a.tiles_x = (e->w + 7) / 8;
         a.tiles_y = (e->h + 7) / 8;
         a.nw = e->nw;
+        a.stage = e->stage;
+        a.taps_in_lds = e->taps_in_lds;
+        a.max_class_taps = e->geo.max_class_taps;
         ProfScope ps(e, 0);
         HIPCHK(e, launch_bic_rgc(a, e->ppw, e->stream));
         return ESRRF_OK;
@@ -508,17 +511,25 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         if ((rc = dalloc(e, &e->d_px, sizeof(float) * npx * nF))) return bad(rc);
         if ((rc = dalloc(e, &e->d_U, sizeof(float2) * 4 * npx * nF))) return bad(rc);
         if (e->M <= 8 && build_bic_geometry(e->w, e->h, e->M, e->radius, e->tSS, e->tSO, e->geo)) {
-            // phases per wave: one block (<= 16 waves) covers every phase of its tile
+            // Waves per block: a multiple of 4 (a block's waves are dealt round-robin to the
+            // CU's four SIMDs, so other counts overload one SIMD at every frame barrier),
+            // about 4 sub-pixel phases per wave, at most 16 waves.
             const int MM = e->M * e->M;
-            int ppw = MM <= 4 ? 1 : 4;
-            if (const char *ov = getenv("ESRRF_BIC_PPW")) {
+            int nw = MM <= 4 ? MM : std::min(16, 4 * (((MM + 3) / 4 + 3) / 4));
+            if (const char *ov = getenv("ESRRF_BIC_NW")) {
                 const int v = atoi(ov);
-                if (v >= 1 && v <= 8) ppw = v;
+                if (v >= 1 && v <= 16) nw = v;
             }
-            while ((MM + ppw - 1) / ppw > (ppw <= 4 ? 16 : 8)) ppw++;
+            if ((MM + nw - 1) / nw > 8) nw = std::min(16, (MM + 7) / 8);
+            const int ppw = (MM + nw - 1) / nw;
+            e->stage = 0;
+            if (const char *ov = getenv("ESRRF_BIC_STAGE")) e->stage = atoi(ov) != 0;
             e->ppw = ppw;
-            e->nw = (MM + ppw - 1) / ppw;
+            e->nw = nw;
             e->tiled = true;
+            // tap tables in LDS when a block's share stays small enough for 2 blocks per CU
+            e->taps_in_lds = (size_t)(e->geo.max_class_taps + 4) * sizeof(Tap) <= 48 * 1024;
+            if (const char *ov = getenv("ESRRF_BIC_TLDS")) e->taps_in_lds = atoi(ov) != 0;
             const size_t ntap = std::max<size_t>(e->geo.taps.size(), 1);
             if ((rc = dalloc(e, &e->d_taps, sizeof(Tap) * ntap))) return bad(rc);
             if ((rc = dalloc(e, &e->d_tapk, sizeof(int) * ntap))) return bad(rc);

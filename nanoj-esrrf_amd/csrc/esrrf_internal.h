@@ -52,6 +52,9 @@ struct BicRgcArgs {
     int tiles_x, tiles_y;
     int nw;                // waves per block
     int zero;              // always 0 (defeats loop-invariant hoisting of phase tables)
+    int stage;             // 1: stage the next frame's tiles in registers
+    int taps_in_lds;       // 1: stage each block's tap tables in LDS
+    int max_class_taps;    // largest tap-table size of one x/y class (LDS sizing)
 };
 
 struct GenericRgcArgs {

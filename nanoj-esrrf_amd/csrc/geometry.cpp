@@ -9,6 +9,7 @@
 // one binade.  Below Xs the rounding error of xc is < 2^-21 and one representative is used.
 // Each (y-class, x-class, phase) entry lists its taps in the reference's j-then-i order
 // (liveSRRF.cl:279-284) with exactly the reference's float arithmetic.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -88,6 +89,7 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
     for (int cy = 0; cy < g.ncls_y; cy++) {
         for (int cx = 0; cx < g.ncls_x; cx++) {
             const int Xr = rep_of(cx, lg), Yr = rep_of(cy, lg);
+            const int cls_first = (int)g.taps.size();
             for (int sy = 0; sy < M; sy++) {
                 for (int sx = 0; sx < M; sx++) {
                     Phase &P = g.phases[((size_t)cy * g.ncls_x + cx) * MM + sy * M + sx];
@@ -159,7 +161,17 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
                     }
                 }
             }
+            g.max_class_taps = std::max(g.max_class_taps, (int)g.taps.size() - cls_first);
         }
+    }
+    // one spare zero chunk so the kernel's prefetch of "the next 4 taps" never reads past
+    // the table
+    for (int k = 0; k < 4; k++) {
+        Tap z;
+        z.off = 0;
+        z.a = z.b = z.W = 0.0f;
+        g.taps.push_back(z);
+        g.tapk.push_back(128 | (128 << 8));
     }
     // every lane (lx, ly in [0, 8)) must stay inside the 4-plane LDS tile
     if (g.max_off + 7 * g.PP + 7 >= 4 * g.PP * g.PH) return false;

@@ -14,6 +14,7 @@ struct BicGeometry {
     int xs_log2 = 3;
     int ncls_x = 1, ncls_y = 1;
     int max_off = 0;
+    int max_class_taps = 0;   // taps of the largest x/y class (all its phases)
     std::vector<Tap> taps;
     std::vector<Phase> phases;
     std::vector<int> tapk;

@@ -131,6 +131,37 @@ __device__ __forceinline__ float2 unit_or_zero(float tx, float ty)
     return make_float2(sx * r, sy * r);
 }
 
+// GxInt / GyInt at an integer point (x, y) of the 2x grid: interp(G, x/2, y/2).  The
+// fractional parts are 0 or 1/2, so the Catmull-Rom weights are (0, 1, 0, 0) or
+// (-1, 9, 9, -1)/16 (cubic(0) = 1, cubic(1) = cubic(2) = 0, cubic(1/2) = 9/16,
+// cubic(3/2) = -1/16 exactly); sums run in the reference's order (liveSRRF.cl:49-57).
+__device__ __forceinline__ float gint_at(const float *t, int pitch, int ox, int oy, int w, int h,
+                                         int x, int y)
+{
+    const int u0 = x >> 1, v0 = y >> 1;
+    if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+        const float c0 = -0.0625f, c1 = 0.5625f;
+        const float wx0 = (x & 1) ? c0 : 0.0f, wx1 = (x & 1) ? c1 : 1.0f;
+        const float wx2 = (x & 1) ? c1 : 0.0f, wx3 = (x & 1) ? c0 : 0.0f;
+        const float wy[4] = {(y & 1) ? c0 : 0.0f, (y & 1) ? c1 : 1.0f, (y & 1) ? c1 : 0.0f,
+                             (y & 1) ? c0 : 0.0f};
+        const float *r = t + (v0 - 1 - oy) * pitch + (u0 - 1 - ox);
+        float q = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float p = 0.0f;
+            p = p + r[0] * wx0;
+            p = p + r[1] * wx1;
+            p = p + r[2] * wx2;
+            p = p + r[3] * wx3;
+            q = q + p * wy[j];
+            r += pitch;
+        }
+        return q;
+    }
+    return interp_tile(t, pitch, ox, oy, w, h, (float)x / 2.0f, (float)y / 2.0f);
+}
+
 __global__ void __launch_bounds__(256) k_bic_prep(const float *__restrict__ px,
                                                   float2 *__restrict__ U, int w, int h)
 {
@@ -147,10 +178,8 @@ __global__ void __launch_bounds__(256) k_bic_prep(const float *__restrict__ px,
     if (kx >= wI || ky >= hI) return;
     float2 u = make_float2(0.0f, 0.0f);
     if (kx >= 1 && ky >= 1) {   // gates 0 < vx, 0 < vy of liveSRRF.cl:282,285
-        const float tx = interp_tile(lgx, PREP_G, gx0, gy0, w, h, (float)kx / 2.0f,
-                                     (float)(ky - 1) / 2.0f);
-        const float ty = interp_tile(lgy, PREP_G, gx0, gy0, w, h, (float)(kx - 1) / 2.0f,
-                                     (float)ky / 2.0f);
+        const float tx = gint_at(lgx, PREP_G, gx0, gy0, w, h, kx, ky - 1);
+        const float ty = gint_at(lgy, PREP_G, gx0, gy0, w, h, kx - 1, ky);
         u = unit_or_zero(tx, ty);
     }
     U[(long)f * wI * hI + (long)ky * wI + kx] = u;
@@ -169,9 +198,8 @@ __global__ void __launch_bounds__(256) k_bic_gint(const float *__restrict__ px,
     load_grad_tiles(px, w, h, gx0, gy0, lraw, lgx, lgy);
     const int x = kx0 + threadIdx.x, y = ky0 + threadIdx.y;
     if (x >= 2 * w || y >= 2 * h) return;
-    const float fx = (float)x / 2.0f, fy = (float)y / 2.0f;
-    gxi[(long)y * 2 * w + x] = interp_tile(lgx, PREP_G, gx0, gy0, w, h, fx, fy);
-    gyi[(long)y * 2 * w + x] = interp_tile(lgy, PREP_G, gx0, gy0, w, h, fx, fy);
+    gxi[(long)y * 2 * w + x] = gint_at(lgx, PREP_G, gx0, gy0, w, h, x, y);
+    gyi[(long)y * 2 * w + x] = gint_at(lgy, PREP_G, gx0, gy0, w, h, x, y);
 }
 
 hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s)
@@ -195,7 +223,8 @@ hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h
 // Tiled RGC kernel
 // ---------------------------------------------------------------------------------------
 constexpr int RAW_T = 12;   // raw intensity tile: image columns X0-2 .. X0+9
-constexpr int RAW_P = 13;
+constexpr int RAW_P = 72;   // pitch = 8 (mod 64): the 8x8 lanes of a read hit distinct banks
+constexpr int RAW_SZ = RAW_T * RAW_P;
 
 __device__ __forceinline__ int axis_class(int X0, int xs_log2)
 {
@@ -215,6 +244,15 @@ __device__ __forceinline__ float apply_sens(float C, float sens, int sens_int)
     }
 }
 
+// Intensity of a border pixel (the reference's bilinear-extrapolation regime).
+__device__ __forceinline__ float border_intensity(const float *lR, int X0, int Y0,
+                                                           int w, int h, int M, int xM, int yM)
+{
+    const float xf = (float)xM / (float)M - 0.5f;
+    const float yf = (float)yM / (float)M - 0.5f;
+    return interp_tile(lR, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
+}
+
 // One RGC sample: acc += W * med3(-1, -|s|, -c - 1), i.e. W * ((1 - |sin|)[cos < 0] - 1)
 // with c = U.(a,b) and s = U x (a,b) (liveSRRF.cl:296-341 with G normalised).
 __device__ __forceinline__ float tap_acc(float2 u, const Tap &tp, float acc)
@@ -227,9 +265,11 @@ __device__ __forceinline__ float tap_acc(float2 u, const Tap &tp, float acc)
     return fmaf(tp.W, hh, acc);
 }
 
-template <int PPW>
-__global__ void __launch_bounds__(PPW <= 4 ? 1024 : 512)
-__attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArgs a,
+// PPW: sub-pixel phases per wave; STAGE_U: U-tile elements each thread stages in
+// registers for the next frame (0: no staging, tiles are filled after the compute).
+template <int PPW, int STAGE_U, bool TLDS>
+__global__ void __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArgs a,
                                                  const Tap *__restrict__ taps,
                                                  const Phase *__restrict__ phases,
                                                  const int *__restrict__ tapk)
@@ -237,7 +277,10 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
     extern __shared__ __align__(16) unsigned char smem[];
     float2 *lU = reinterpret_cast<float2 *>(smem);
     const int usz = 4 * a.PP * a.PH;
-    float *lR = reinterpret_cast<float *>(lU + usz);
+    // U buffers: two when the next frame is staged in registers, one otherwise (then the
+    // block refills it between two barriers)
+    constexpr int NBUF = STAGE_U > 0 ? 2 : 1;
+    float *lR = reinterpret_cast<float *>(lU + NBUF * usz);
 
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63;
@@ -255,7 +298,9 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
     const bool inimg = (X < w) && (Y < h);
     const long whM = (long)a.wM * a.hM;
 
-    float st[PPW][5];
+    // per-phase state of this lane: AVG, E[x^2], TAC2, INT, Prev, 1/sum(w), bicubic base
+    // (separate small arrays: each stays in VGPRs under uniform dynamic indexing)
+    float s_avg[PPW], s_e2[PPW], s_tac[PPW], s_int[PPW], s_prv[PPW];
     float invw[PPW];
     int ibase[PPW];   // LDS index of the lane's bicubic support, -1: bilinear border path
 #pragma unroll
@@ -263,7 +308,7 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
         const int ph = wave + p * a.nw;
         invw[p] = 0.0f;
         ibase[p] = -1;
-        st[p][0] = st[p][1] = st[p][2] = st[p][3] = st[p][4] = 0.0f;
+        s_avg[p] = s_e2[p] = s_tac[p] = s_int[p] = s_prv[p] = 0.0f;
         if (ph < MM) {
             const Phase P = PHS[ph];
             float ws = P.wsum;
@@ -284,11 +329,11 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
                 ibase[p] = (v0 - 1 - (Y0 - 2)) * RAW_P + (u0 - 1 - (X0 - 2));
             if (!a.first_init && inimg) {
                 const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
-                st[p][0] = a.out[pix];
-                st[p][1] = a.out[pix + whM];
-                st[p][2] = a.out[pix + 2 * whM];
-                st[p][3] = a.out[pix + 3 * whM];
-                st[p][4] = a.prev[pix];
+                s_avg[p] = a.out[pix];
+                s_e2[p] = a.out[pix + whM];
+                s_tac[p] = a.out[pix + 2 * whM];
+                s_int[p] = a.out[pix + 3 * whM];
+                s_prv[p] = a.prev[pix];
             }
         }
     }
@@ -302,66 +347,139 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
     const bool intw = a.int_weighting != 0;
     const bool prev_scaled = a.tac_prev_scaled != 0;
 
-    for (int fi = 0; fi < a.nframes; fi++) {
-        const int f = a.frame0 + fi;
-        __syncthreads();
-        {
-            const float2 *Uf = a.U + whI * f;
-            const int n = a.TW * a.TH;
-            for (int e = tid; e < n; e += nthr) {
-                const int gy = e / a.TW, gx = e - gy * a.TW;
-                const int kx = kxo + gx, ky = kyo + gy;
-                float2 v = make_float2(0.0f, 0.0f);
-                if (kx >= 0 && kx < wI && ky >= 0 && ky < hI) v = Uf[(long)ky * wI + kx];
-                lU[((gy & 1) * 2 + (gx & 1)) * planesz + (gy >> 1) * a.PP + (gx >> 1)] = v;
-            }
-            const float *Pf = a.px + wh * f;
-            for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
-                const int r = e / RAW_T, c = e - r * RAW_T;
-                const int ix = min(max(X0 - 2 + c, 0), w - 1);
-                const int iy = min(max(Y0 - 2 + r, 0), h - 1);
-                lR[r * RAW_P + c] = Pf[(long)iy * w + ix];
-            }
+    // Per-thread staging plan of the per-frame tiles, computed once: U-tile element
+    // k*nthr + tid is taken plane-major (consecutive threads write consecutive LDS words),
+    // goff < 0 marks a zero (outside the 2x grid) or unused slot.
+    const int pw = a.TW >> 1;
+    const int nU = 4 * a.PH * pw;
+    constexpr int SU = STAGE_U > 0 ? STAGE_U : 1;
+    int goff[SU], loff[SU];
+#pragma unroll
+    for (int k = 0; k < SU; k++) {
+        const int e = tid + k * nthr;
+        goff[k] = -2;
+        loff[k] = 0;
+        if (e < nU) {
+            const int pl = e / (a.PH * pw);
+            const int r = (e - pl * a.PH * pw) / pw;
+            const int c = e - pl * a.PH * pw - r * pw;
+            const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+            const int kx = kxo + gx, ky = kyo + gy;
+            loff[k] = pl * planesz + r * a.PP + c;
+            goff[k] = (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI)
+                          ? ky * wI + kx : -1;
         }
-        __syncthreads();
+    }
+    const bool staged = STAGE_U > 0 && nthr * STAGE_U >= nU && nthr >= RAW_T * RAW_T;
+    int roff = -1, rl = 0;
+    if (tid < RAW_T * RAW_T) {
+        const int r = tid / RAW_T, c = tid - (tid / RAW_T) * RAW_T;
+        roff = min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+        rl = r * RAW_P + c;
+    }
+
+
+    // direct (unstaged) tile fill: used for the first frame and when a block has too few
+    // threads to stage a whole tile in registers
+    auto fill_direct = [&](int f, float2 *dU, float *dR) {
+        const float2 *Uf = a.U + whI * f;
+        for (int e = tid; e < nU; e += nthr) {
+            const int pl = e / (a.PH * pw);
+            const int r = (e - pl * a.PH * pw) / pw;
+            const int c = e - pl * a.PH * pw - r * pw;
+            const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+            const int kx = kxo + gx, ky = kyo + gy;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI)
+                v = Uf[(long)ky * wI + kx];
+            dU[pl * planesz + r * a.PP + c] = v;
+        }
+        const float *Pf = a.px + wh * f;
+        for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
+            const int r = e / RAW_T, c = e - r * RAW_T;
+            dR[r * RAW_P + c] = Pf[(long)min(max(Y0 - 2 + r, 0), h - 1) * w +
+                                   min(max(X0 - 2 + c, 0), w - 1)];
+        }
+    };
+
+    // TLDS: this block's tap tables (every phase of its x/y class, contiguous on the host)
+    // are staged once in LDS and read with uniform-address ds_read_b128 in the tap loop,
+    // so the loop's loads are all in-order LDS traffic
+    Tap *lT = reinterpret_cast<Tap *>(lR + NBUF * RAW_SZ);
+    const int tcls0 = PHS[0].first_tap;
+    if (TLDS) {
+        const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
+        for (int e = tid; e < tcls1 - tcls0 + 4; e += nthr) lT[e] = taps[tcls0 + e];
+    }
+    fill_direct(a.frame0, lU, lR);
+    __syncthreads();
+
+    for (int fi = 0; fi < a.nframes; fi++) {
+        const int cur = NBUF == 2 ? (fi & 1) : 0;
+        const bool more = fi + 1 < a.nframes;
+        // issue the global loads of the next frame's tiles; they land during the compute
+        float2 su[SU];
+        float sr = 0.0f;
+        if (more && staged) {
+            const float2 *Un = a.U + whI * (a.frame0 + fi + 1);
+#pragma unroll
+            for (int k = 0; k < SU; k++)
+                su[k] = goff[k] >= 0 ? Un[goff[k]] : make_float2(0.0f, 0.0f);
+            if (roff >= 0) sr = a.px[wh * (a.frame0 + fi + 1) + roff];
+        }
+        // (buffer selection by offset arithmetic on the __shared__ base keeps the LDS
+        // address space visible to the compiler: ds_read, not flat loads)
+        const float2 *cU = lU + cur * usz;
+        const float *cR = lR + cur * RAW_SZ;
         const bool first = (fi == 0) && a.first_init;
-        // Re-read the per-phase tables every frame and hide the lane coordinates from
-        // loop-invariant code motion: hoisting PPW copies of the phase geometry out of the
-        // frame loop costs far more registers than re-reading it through the scalar cache.
-        // (a.zero is 0 at run time; the compiler cannot prove it, so nothing below moves
-        // out of the frame loop)
+        // (a.zero is 0 at run time; the compiler cannot prove it, so the per-phase table
+        // reads below stay inside the frame loop instead of being hoisted PPW times)
         const int zf = a.zero * fi;
-        const int Xl = X + zf, Yl = Y + zf;
         const Phase *PHf = PHS + zf;
         const Tap *TAPf = taps + zf;
 
-#pragma unroll
+#pragma unroll 1
         for (int p = 0; p < PPW; p++) {
             const int ph = wave + p * a.nw;
             if (ph < MM) {
                 const Phase P = PHf[ph];
-                const Tap *T = TAPf + P.first_tap;
-                // Tap lists are padded to a multiple of 4 with W = 0 taps.  The constants
-                // of chunk c+1 are fetched (scalar cache) while the LDS reads of chunk c are
-                // in flight, so one wait covers both latencies.
+                // Tap lists are padded to a multiple of 4 with W = 0 taps and the table
+                // ends with one spare chunk, so the constants of chunk c+1 can always be
+                // fetched (scalar cache) while the LDS reads of chunk c are in flight.
                 float acc = 0.0f;
-                Tap c0 = T[0], c1 = T[1], c2 = T[2], c3 = T[3];
-                const int nch = P.ntaps >> 2;
-                for (int c = 0; c < nch; c++) {
-                    const float2 u0 = lU[base + c0.off];
-                    const float2 u1 = lU[base + c1.off];
-                    const float2 u2 = lU[base + c2.off];
-                    const float2 u3 = lU[base + c3.off];
-                    const int cn = (c + 1 < nch) ? 4 * (c + 1) : 4 * c;
-                    const Tap n0 = T[cn], n1 = T[cn + 1], n2 = T[cn + 2], n3 = T[cn + 3];
-                    acc = tap_acc(u0, c0, acc);
-                    acc = tap_acc(u1, c1, acc);
-                    acc = tap_acc(u2, c2, acc);
-                    acc = tap_acc(u3, c3, acc);
-                    c0 = n0;
-                    c1 = n1;
-                    c2 = n2;
-                    c3 = n3;
+                if (TLDS) {
+                    const Tap *Tl = lT + (P.first_tap - tcls0);
+                    for (int t = 0; t < P.ntaps; t += 4) {
+                        const Tap c0 = Tl[t], c1 = Tl[t + 1], c2 = Tl[t + 2], c3 = Tl[t + 3];
+                        const float2 u0 = cU[base + c0.off];
+                        const float2 u1 = cU[base + c1.off];
+                        const float2 u2 = cU[base + c2.off];
+                        const float2 u3 = cU[base + c3.off];
+                        acc = tap_acc(u0, c0, acc);
+                        acc = tap_acc(u1, c1, acc);
+                        acc = tap_acc(u2, c2, acc);
+                        acc = tap_acc(u3, c3, acc);
+                    }
+                } else {
+                    const Tap *T = TAPf + P.first_tap;
+                    Tap c0 = T[0], c1 = T[1], c2 = T[2], c3 = T[3];
+                    const int nch = P.ntaps >> 2;
+                    for (int c = 0; c < nch; c++) {
+                        const float2 u0 = cU[base + c0.off];
+                        const float2 u1 = cU[base + c1.off];
+                        const float2 u2 = cU[base + c2.off];
+                        const float2 u3 = cU[base + c3.off];
+                        T += 4;
+                        const Tap n0 = T[0], n1 = T[1], n2 = T[2], n3 = T[3];
+                        acc = tap_acc(u0, c0, acc);
+                        acc = tap_acc(u1, c1, acc);
+                        acc = tap_acc(u2, c2, acc);
+                        acc = tap_acc(u3, c3, acc);
+                        c0 = n0;
+                        c1 = n1;
+                        c2 = n2;
+                        c3 = n3;
+                    }
                 }
                 float C = (P.wsum + acc) * invw[p];
                 C = apply_sens(C, a.sens, a.sens_int);
@@ -369,7 +487,7 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
                 // interpolated intensity at (xM/M - 0.5, yM/M - 0.5) (liveSRRF.cl:361)
                 float v;
                 if (ibase[p] >= 0) {
-                    const float *row = lR + ibase[p];
+                    const float *row = cR + ibase[p];
                     float q = 0.0f;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
@@ -382,27 +500,45 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
                     }
                     v = q;
                 } else {
-                    const float xf = (float)(Xl * M + P.sx) / (float)M - 0.5f;
-                    const float yf = (float)(Yl * M + P.sy) / (float)M - 0.5f;
-                    v = interp_tile(lR, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
+                    v = border_intensity(cR, X0, Y0, w, h, M, (X + zf) * M + P.sx,
+                                         (Y + zf) * M + P.sy);
                 }
 
                 // temporal accumulation (liveSRRF.cl:363-393)
                 const float t = intw ? v * C : C;
                 const float e2 = intw ? (t * v) * C : C * C;
                 if (first) {
-                    st[p][0] = t * rn;
-                    st[p][1] = e2 * rn;
-                    st[p][2] = 0.0f;
-                    st[p][3] = v * rn;
+                    s_avg[p] = t * rn;
+                    s_e2[p] = e2 * rn;
+                    s_tac[p] = 0.0f;
+                    s_int[p] = v * rn;
                 } else {
-                    st[p][0] = st[p][0] + t * rn;
-                    st[p][1] = st[p][1] + e2 * rn;
-                    st[p][2] = st[p][2] + (t * st[p][4]) * rn1;
-                    st[p][3] = st[p][3] + v * rn;
+                    s_avg[p] = s_avg[p] + t * rn;
+                    s_e2[p] = s_e2[p] + e2 * rn;
+                    s_tac[p] = s_tac[p] + (t * s_prv[p]) * rn1;
+                    s_int[p] = s_int[p] + v * rn;
                 }
-                st[p][4] = prev_scaled ? t * rn1 : t;
+                s_prv[p] = prev_scaled ? t * rn1 : t;
             }
+        }
+        if (NBUF == 2) {
+            if (more) {
+                float2 *nU_ = lU + (cur ^ 1) * usz;
+                float *nR_ = lR + (cur ^ 1) * RAW_SZ;
+                if (staged) {
+#pragma unroll
+                    for (int k = 0; k < SU; k++)
+                        if (goff[k] != -2) nU_[loff[k]] = su[k];
+                    if (roff >= 0) nR_[rl] = sr;
+                } else {
+                    fill_direct(a.frame0 + fi + 1, nU_, nR_);
+                }
+            }
+            __syncthreads();
+        } else if (more) {
+            __syncthreads();   // every wave is done with this frame's tiles
+            fill_direct(a.frame0 + fi + 1, lU, lR);
+            __syncthreads();
         }
     }
 
@@ -413,22 +549,42 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 4 ? 4 : 2))) k_bic_rgc(const BicRgcArg
             if (ph >= MM) continue;
             const Phase P = PHS[ph];
             const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
-            a.out[pix] = st[p][0];
-            a.out[pix + whM] = st[p][1];
-            a.out[pix + 2 * whM] = st[p][2];
-            a.out[pix + 3 * whM] = st[p][3];
-            a.prev[pix] = st[p][4];
+            a.out[pix] = s_avg[p];
+            a.out[pix + whM] = s_e2[p];
+            a.out[pix + 2 * whM] = s_tac[p];
+            a.out[pix + 3 * whM] = s_int[p];
+            a.prev[pix] = s_prv[p];
         }
     }
+}
+
+template <int PPW, bool TLDS>
+static hipError_t launch_ppw_t(const BicRgcArgs &a, hipStream_t s)
+{
+    const size_t one = sizeof(float2) * 4 * a.PP * a.PH + sizeof(float) * RAW_SZ;
+    const size_t tl = TLDS ? sizeof(Tap) * ((size_t)a.max_class_taps + 4) : 0;
+    const int nthr = 64 * a.nw;
+    const int nU = 4 * a.PH * (a.TW >> 1);
+    const int need = (nU + nthr - 1) / nthr;
+    const dim3 grid(a.tiles_x * a.tiles_y), block(nthr);
+    // register staging of the next frame's tiles costs VGPRs; where it would push the
+    // kernel below its occupancy target the tiles are filled directly after the compute
+    if (a.stage && nthr >= RAW_T * RAW_T && need <= 2)
+        hipLaunchKernelGGL((k_bic_rgc<PPW, 2, TLDS>), grid, block, 2 * one + tl, s, a, a.taps,
+                           a.phases, a.tapk);
+    else if (a.stage && nthr >= RAW_T * RAW_T && need <= 4)
+        hipLaunchKernelGGL((k_bic_rgc<PPW, 4, TLDS>), grid, block, 2 * one + tl, s, a, a.taps,
+                           a.phases, a.tapk);
+    else
+        hipLaunchKernelGGL((k_bic_rgc<PPW, 0, TLDS>), grid, block, one + tl, s, a, a.taps,
+                           a.phases, a.tapk);
+    return hipGetLastError();
 }
 
 template <int PPW>
 static hipError_t launch_ppw(const BicRgcArgs &a, hipStream_t s)
 {
-    const size_t lds = sizeof(float2) * 4 * a.PP * a.PH + sizeof(float) * RAW_T * RAW_P;
-    hipLaunchKernelGGL(k_bic_rgc<PPW>, dim3(a.tiles_x * a.tiles_y), dim3(64 * a.nw), lds, s, a,
-                       a.taps, a.phases, a.tapk);
-    return hipGetLastError();
+    return a.taps_in_lds ? launch_ppw_t<PPW, true>(a, s) : launch_ppw_t<PPW, false>(a, s);
 }
 
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s)
