@@ -1,30 +1,308 @@
-// modes.cpp -- dispatch of the FHT and 3-D pipelines (filled in by k_fht.hip / k_3d.hip).
+// modes.cpp -- host side of the FHT (Fourier-upsampled) and 3-D (multi-focus) pipelines.
+//
+//  FHT : LiveSRRF_CL.prepareBuffersFHTinterpolation (LiveSRRF_CL.java:462-579),
+//        prepareDataFHT_SRRF (:598-632) and the RGC loop of calculateSRRF (:699-746);
+//        MP correction is never applied in this mode (:762).
+//  3-D : prepareBuffersBICinterpolation with live3DSRRF.cl (:266-459), the montage split
+//        of reshapeImageStack3D (:963-994) and calculateSRRF's 3-D branch (:655-696).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "esrrf_internal.h"
+#include "fht_internal.h"
 #include "modes.h"
+#include "mode3d_internal.h"
+
+#pragma STDC FP_CONTRACT OFF
 
 namespace esrrf {
 
-int mode_create(const ModeCreate &c, ModeState &ms, float **, float2 **, float **, size_t *)
+namespace {
+
+int err(ModeState &ms, int code, const char *msg, hipError_t st = hipSuccess)
+{
+    ms.err = msg;
+    if (st != hipSuccess) {
+        ms.err += ": ";
+        ms.err += hipGetErrorString(st);
+    }
+    return code;
+}
+
+template <class T>
+bool dmalloc(T **p, size_t bytes)
+{
+    *p = nullptr;
+    return hipMalloc((void **)p, bytes ? bytes : 4) == hipSuccess;
+}
+
+// Java: (int) Math.ceil(Math.log(n)/Math.log(2)) then (int) Math.pow(2, .)
+int pow2_ceil_java(int n)
+{
+    const int e = (int)std::ceil(std::log((double)n) / std::log(2.0));
+    return (int)std::pow(2.0, (double)e);
+}
+
+// Tap table of the FHT-mode RGC for unshifted frames: the window offsets (i, j) in the
+// reference's j-then-i order (liveSRRF_noInterpolation.cl:276-289), dx = i, dy = j exactly.
+void build_fht_taps(ModeState &ms, float radius, float tSS, float tSO)
+{
+    const int R = (int)radius, Rhi = (int)(radius + 1.0f);
+    const int TW = 16 + R + Rhi;
+    int pitch = 8;
+    while (pitch < TW) pitch += 32;
+    ms.fht_R = R;
+    ms.fht_TW = TW;
+    ms.fht_pitch = pitch;
+    ms.h_taps.clear();
+    ms.h_tapk.clear();
+    float wsum = 0.0f;
+    for (int j = -R; j <= Rhi; j++) {
+        for (int i = -R; i <= Rhi; i++) {
+            volatile float dx = (float)i, dy = (float)j;
+            volatile float d2 = dx * dx;
+            volatile float dy2 = dy * dy;
+            d2 = d2 + dy2;
+            const float d = std::sqrt((float)d2);
+            if (!(d != 0.0f && d <= tSO)) continue;
+            volatile float e = -(d * d);
+            e = e / tSS;
+            volatile float w1 = d * std::exp((float)e);
+            volatile float W = w1 * w1;
+            W = W * w1;
+            W = W * w1;
+            wsum = wsum + (float)W;
+            Tap t;
+            t.off = j * pitch + i;
+            t.a = (float)dx / d;
+            t.b = (float)dy / d;
+            t.W = (float)W;
+            ms.h_taps.push_back(t);
+            ms.h_tapk.push_back((i + 128) | ((j + 128) << 8));
+        }
+    }
+    ms.ntaps = (int)ms.h_taps.size();
+    ms.tap_wsum = wsum;
+    while (ms.h_taps.size() % 4 != 0 || ms.h_taps.size() < (size_t)ms.ntaps + 4) {
+        Tap z;
+        z.off = 0;
+        z.a = z.b = z.W = 0.0f;
+        ms.h_taps.push_back(z);
+        ms.h_tapk.push_back(128 | (128 << 8));
+    }
+}
+
+}  // namespace
+
+int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, float **d_aux,
+                size_t *aux_bytes)
 {
     ms.mode = c.mode;
-    ms.err = "mode not implemented yet";
-    return ESRRF_ERR_UNSUPPORTED;
+    ms.nF = c.nF;
+    *aux_bytes = 0;
+    *d_aux = nullptr;
+    if (c.mode == 1) {
+        const long wM = (long)c.w * c.M, hM = (long)c.h * c.M;
+        if (!dmalloc(d_px, sizeof(float) * wM * hM * c.nF) ||
+            !dmalloc(d_U, sizeof(float2) * wM * hM * c.nF) ||
+            !dmalloc(&ms.d_rawf, sizeof(float) * (size_t)c.w * c.h * c.nF))
+            return err(ms, ESRRF_ERR_NOMEM, "FHT buffers: hipMalloc failed");
+        if (c.M > 1) {
+            // mirror padding to a square power of two (FourierInterpolation.java:118-172)
+            const int bw = pow2_ceil_java(c.w), bh = pow2_ceil_java(c.h);
+            const int N = std::max(bw, bh);
+            if (N > 4096) return err(ms, ESRRF_ERR_UNSUPPORTED, "FHT: frames larger than 4096");
+            int sW = (int)std::ceil((double)N / c.w);
+            if (sW % 2 == 0) sW += 1;
+            int sH = (int)std::ceil((double)N / c.h);
+            if (sH % 2 == 0) sH += 1;
+            FhtPlan &p = ms.plan;
+            p.N = N;
+            p.log2N = 0;
+            while ((1 << p.log2N) < N) p.log2N++;
+            p.M = c.M;
+            p.w = c.w;
+            p.h = c.h;
+            p.scaleW = sW;
+            p.scaleH = sH;
+            p.xROIp = (sW * c.w - N) / 2;
+            p.yROIp = (sH * c.h - N) / 2;
+            p.xROI = (c.M * N - c.M * c.w) / 2;
+            p.yROI = (c.M * N - c.M * c.h) / 2;
+            std::vector<float2> tw(std::max(N / 2, 1));
+            for (int k = 0; k < N / 2; k++) {
+                const double ang = -2.0 * M_PI * k / N;
+                tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+            }
+            float2 *dtw = nullptr;
+            if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N) ||
+                !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * c.M) ||
+                !dmalloc(&dtw, sizeof(float2) * tw.size()))
+                return err(ms, ESRRF_ERR_NOMEM, "FHT spectra: hipMalloc failed");
+            hipError_t st = hipMemcpy(dtw, tw.data(), sizeof(float2) * tw.size(),
+                                      hipMemcpyHostToDevice);
+            if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT twiddles", st);
+            p.tw = dtw;
+        }
+        build_fht_taps(ms, c.radius, c.tSS, c.tSO);
+        if (!dmalloc(&ms.d_taps, sizeof(Tap) * ms.h_taps.size()) ||
+            !dmalloc(&ms.d_tapk, sizeof(int) * ms.h_tapk.size()))
+            return err(ms, ESRRF_ERR_NOMEM, "FHT taps: hipMalloc failed");
+        hipError_t st = hipMemcpy(ms.d_taps, ms.h_taps.data(), sizeof(Tap) * ms.h_taps.size(),
+                                  hipMemcpyHostToDevice);
+        if (st == hipSuccess)
+            st = hipMemcpy(ms.d_tapk, ms.h_tapk.data(), sizeof(int) * ms.h_tapk.size(),
+                           hipMemcpyHostToDevice);
+        if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT taps upload", st);
+        // gated samples of one frame: sum over taps of the pixels keeping (i, j) in range
+        long long act = 0;
+        for (int t = 0; t < ms.ntaps; t++) {
+            const int i = (ms.h_tapk[t] & 255) - 128, j = ((ms.h_tapk[t] >> 8) & 255) - 128;
+            act += (long long)std::max<long>(0, wM - std::abs(i)) * std::max<long>(0, hM - std::abs(j));
+        }
+        ms.active = act;
+        ms.wM = (int)wM;
+        ms.hM = (int)hM;
+        return ESRRF_OK;
+    }
+    return mode3d_create(c, ms, d_px, d_U);
 }
-int mode_prepare(int, const ModeCtx &, ModeState &ms, const void *, int, int)
+
+int mode_prepare(int mode, const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n)
 {
-    ms.err = "mode not implemented yet";
-    return ESRRF_ERR_UNSUPPORTED;
+    if (mode == 2) return mode3d_prepare(c, ms, src, dtype, n);
+    const long npx = (long)c.w * c.h;
+    const long nM = (long)c.wM * c.hM;
+    hipError_t st = launch_convert_frames(src, dtype, ms.d_rawf, npx * n, c.s);
+    if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "frame conversion", st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c.profiling) {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, c.s);
+    }
+    if (c.M > 1) {
+        for (int f = 0; f < n; f++) {
+            st = launch_fht_upsample(ms.plan, ms.d_rawf + npx * f, c.d_px + nM * f, c.s);
+            if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT upsampling", st);
+        }
+    } else {
+        st = hipMemcpyAsync(c.d_px, ms.d_rawf, sizeof(float) * npx * n, hipMemcpyDeviceToDevice,
+                            c.s);
+        if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "frame copy", st);
+    }
+    if (c.profiling) {
+        (void)hipEventRecord(e1, c.s);
+        float ms_ = 0.0f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms_, e0, e1) == hipSuccess) {
+            ms.prof_up_ms += ms_;
+            ms.prof_up_n += 1;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    st = launch_fht_prep(c.d_px, c.d_U, c.wM, c.hM, n, c.s);
+    if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT gradients", st);
+    return ESRRF_OK;
 }
-int mode_rgc(int, const ModeCtx &, ModeState &ms, const ModeRgc &)
+
+int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
 {
-    ms.err = "mode not implemented yet";
-    return ESRRF_ERR_UNSUPPORTED;
+    if (mode == 2) return mode3d_rgc(c, ms, r);
+    const int N = r.nFrameForSRRF;
+    const float rn = 1.0f / (float)N, rn1 = 1.0f / (float)(N - 1);
+    hipError_t st;
+    // the tiled kernel keeps the window's tile and the tap table in LDS (64 KiB per block)
+    if (r.drift_zero && fht_rgc_lds_bytes(ms.fht_TW, ms.fht_pitch, ms.ntaps) <= 65536) {
+        FhtRgcArgs a{};
+        a.U = c.d_U;
+        a.px = c.d_px;
+        a.out = r.out;
+        a.prev = r.prev;
+        a.taps = ms.d_taps;
+        a.tapk = ms.d_tapk;
+        a.W = c.wM;
+        a.H = c.hM;
+        a.frame0 = r.frame0;
+        a.nframes = r.nframes;
+        a.first_init = r.count0 == 0;
+        a.rn = rn;
+        a.rn1 = rn1;
+        a.sens = r.sens;
+        a.sens_int = r.sens_int;
+        a.int_weighting = r.int_weighting;
+        a.R = ms.fht_R;
+        a.TW = ms.fht_TW;
+        a.TH = ms.fht_TW;
+        a.pitch = ms.fht_pitch;
+        a.ntaps = ms.ntaps;
+        a.wsum = ms.tap_wsum;
+        a.tiles_x = (c.wM + 15) / 16;
+        a.tiles_y = (c.hM + 15) / 16;
+        st = launch_fht_rgc(a, c.s);
+    } else {
+        FhtGenericArgs g{};
+        g.U = c.d_U;
+        g.px = c.d_px;
+        g.drift = r.drift;
+        g.out = r.out;
+        g.prev = r.prev;
+        g.W = c.wM;
+        g.H = c.hM;
+        g.frame0 = r.frame0;
+        g.nframes = r.nframes;
+        g.count0 = r.count0;
+        g.nFrameForSRRF = N;
+        g.rn = rn;
+        g.rn1 = rn1;
+        g.sens = r.sens;
+        g.sens_int = r.sens_int;
+        g.int_weighting = r.int_weighting;
+        g.radius = r.radius;
+        g.tSS = r.tSS;
+        g.tSO = r.tSO;
+        st = launch_fht_generic(g, c.s);
+    }
+    if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT RGC launch", st);
+    return ESRRF_OK;
 }
-int mode_read_gradients(int, const ModeCtx &, ModeState &ms, float *)
+
+int mode_read_gradients(int mode, const ModeCtx &c, ModeState &ms, float *out)
 {
-    ms.err = "mode not implemented yet";
-    return ESRRF_ERR_UNSUPPORTED;
+    if (mode == 2) return mode3d_read_gradients(c, ms, out);
+    const long nM = (long)c.wM * c.hM;
+    float *tmp = nullptr;
+    if (!dmalloc(&tmp, sizeof(float) * 2 * nM)) return err(ms, ESRRF_ERR_NOMEM, "gradient read");
+    hipError_t st = launch_fht_grad(c.d_px, tmp, tmp + nM, c.wM, c.hM, c.s);
+    if (st == hipSuccess) st = launch_nan_to_zero_copy(tmp, tmp, 2 * nM, c.s);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(out, tmp, sizeof(float) * 2 * nM, hipMemcpyDeviceToHost, c.s);
+    if (st == hipSuccess) st = hipStreamSynchronize(c.s);
+    (void)hipFree(tmp);
+    if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "gradient read", st);
+    return ESRRF_OK;
 }
+
 long long mode_active_samples(int, const ModeState &ms) { return ms.active; }
-void free_mode_state(ModeState &) {}
+
+void free_mode_state(ModeState &ms)
+{
+    void *ptrs[] = {ms.d_rawf, ms.plan.F, ms.plan.Y, (void *)ms.plan.tw, ms.d_taps, ms.d_tapk,
+                    ms.d_st, ms.d_roi, ms.d_ic, ms.d_gi, ms.d_g3, ms.d_planes};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    ms.d_rawf = nullptr;
+    ms.plan.F = ms.plan.Y = nullptr;
+    ms.plan.tw = nullptr;
+    ms.d_taps = nullptr;
+    ms.d_tapk = nullptr;
+    ms.d_st = nullptr;
+    ms.d_roi = ms.d_ic = nullptr;
+    ms.d_gi = ms.d_g3 = ms.d_planes = nullptr;
+}
 
 }  // namespace esrrf

@@ -3,8 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/esrrf.h"
+#include "esrrf_internal.h"
+#include "fht_internal.h"
 
 namespace esrrf {
 
@@ -14,21 +17,25 @@ struct ModeState {
     double prof_up_ms = 0.0;      // upsampling kernel time (FHT)
     long prof_up_n = 0;
     long long active = 0;         // gated RGC samples per frame (geometry only)
+    int nF = 0;
+    int wM = 0, hM = 0;
     // FHT
-    int N = 0;                    // padded square size of the forward transform
-    float2 *d_spec = nullptr;     // spectra / FFT scratch
-    size_t spec_bytes = 0;
-    float *d_g = nullptr;         // RobX gradients of the load [2][nF][hM][wM]
-    void *taps = nullptr;         // device tap table (FHT / 3-D)
+    float *d_rawf = nullptr;      // converted raw frames of the load [nF][h][w]
+    FhtPlan plan{};
+    std::vector<Tap> h_taps;      // unshifted-window tap table (magnified grid)
+    std::vector<int> h_tapk;
+    Tap *d_taps = nullptr;
+    int *d_tapk = nullptr;
     int ntaps = 0;
     float tap_wsum = 0.0f;
-    int tap_pp = 0, tap_ph = 0, tap_tw = 0, tap_th = 0, tap_r = 0, tap_rhi = 0;
+    int fht_R = 0, fht_TW = 0, fht_pitch = 0;
     // 3-D
     float *d_st = nullptr;        // [3][nP] theta(rad), shiftX, shiftY (LiveSRRF_CL.java:440-445)
     double *d_roi = nullptr, *d_ic = nullptr;
     float bg = 0.0f;
-    float *d_gi = nullptr;        // interpolated gradients [3][nF][nP][2h][2w]
-    int nF = 0;
+    float *d_gi = nullptr;        // interpolated gradients
+    float *d_g3 = nullptr;        // 3-D gradients
+    float *d_planes = nullptr;    // montage split into planes
 };
 
 struct ModeCtx {

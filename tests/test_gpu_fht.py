@@ -1,0 +1,147 @@
+"""FHT-mode (Fourier upsampling + liveSRRF_noInterpolation.cl) parity of the HIP engine
+against the CPU oracle (oracle/fht.py for the upsampling, oracle/esrrf_oracle.c for the
+gradients and the RGC).
+
+Tolerance: per channel, both images divided by the oracle image's max |value|,
+|a - b| <= 1e-5 + 1e-4 |b|.  Frame counters: exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import normalised_close
+
+pytestmark = pytest.mark.gpu
+
+
+def stack(n, w, h, seed):
+    from nanoj_esrrf_amd import synth
+    return synth.blinking_stack(n, w, h, max(20, w * h // 14), 4, seed=seed)
+
+
+def run_engine(pkg, st, M, fwhm, S, intW=True, drift=None, loads=None, nOnGPU=None):
+    n = st.shape[0]
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(st.shape[2], st.shape[1], M, fwhm, S, nOnGPU or n, n, 20000,
+                 e.allCLdevices[0], intW, True, True, None, 0, 1)
+    e.resetFramePosition()
+    dx, dy = (np.zeros(n), np.zeros(n)) if drift is None else drift
+    e.loadDriftXYGPUbuffer(dx, dy)
+    grads = None
+    for a, b in (loads or [(0, n)]):
+        e.calculateSRRF(st[a:b])
+        if grads is None:
+            grads = e.readGradientBuffers()
+    out = e.readSRRFbuffer()
+    counters = e.frameCounters()
+    e.release()
+    return out, counters, grads
+
+
+def run_oracle(O, st, M, fwhm, S, intW=True, drift=None, loads=None, nOnGPU=None):
+    n = st.shape[0]
+    o = O.OracleLiveSRRF(st.shape[2], st.shape[1], M, fwhm, S, nOnGPU or n, n, intW, True, True)
+    if drift is not None:
+        o.loadDriftXYGPUbuffer(*drift)
+    grads = None
+    for a, b in (loads or [(0, n)]):
+        o.calculateSRRF(st[a:b].astype(np.float32))
+        if grads is None:
+            grads = o.readGradientBuffers()
+    return o.readSRRFbuffer(), list(o.counters), grads
+
+
+def assert_parity(got, exp, what):
+    for c, name in enumerate(("AVG", "VAR", "TAC2", "INT")):
+        ok, ex, idx = normalised_close(got[c], exp[c])
+        assert ok, f"{what}: {name} differs (excess {ex:.3g} at {idx})"
+
+
+@pytest.mark.parametrize("M,fwhm,S,w,h", [
+    (2, 1.5, 1, 16, 16),      # power-of-two square: mirror padding is the identity
+    (3, 1.5, 2, 21, 17),      # odd sizes: mirror tiles 3 x 3 then centre crop to 32
+    (4, 2.0, 1, 30, 24),
+    (5, 1.5, 2, 32, 20),
+    (5, 1.2, 1.7, 24, 24),    # non-integer sensitivity (powf)
+    (1, 1.5, 1, 20, 20),      # M = 1: no upsampling
+])
+def test_fht_configs_parity(pkg, O, M, fwhm, S, w, h):
+    st = stack(6, w, h, seed=300 + 11 * M + w)
+    got, cnt, g = run_engine(pkg, st, M, fwhm, S)
+    exp, ocnt, og = run_oracle(O, st, M, fwhm, S)
+    assert cnt == ocnt == [6, 6]
+    assert_parity(got.reshape(4, h * M, w * M), exp.reshape(4, h * M, w * M),
+                  f"FHT M{M} fwhm{fwhm} S{S} {w}x{h}")
+    # RobX gradients of the upsampled first frame (linear in the upsampled pixels)
+    g = np.asarray(g).reshape(2, -1)
+    og = np.asarray(og).reshape(2, -1)
+    for c in range(2):
+        scale = np.abs(og[c]).max()
+        assert np.abs(g[c] - og[c]).max() <= 2e-5 * scale, f"gradient {c} differs"
+
+
+def test_fht_no_intensity_weighting(pkg, O):
+    st = stack(5, 20, 20, seed=17)
+    got, _, _ = run_engine(pkg, st, 3, 1.5, 1, intW=False)
+    exp, _, _ = run_oracle(O, st, 3, 1.5, 1, intW=False)
+    assert_parity(got.reshape(4, 60, 60), exp.reshape(4, 60, 60), "FHT intW off")
+
+
+def test_fht_uneven_loads_and_counters(pkg, O):
+    st = stack(11, 16, 16, seed=23)
+    loads = [(0, 4), (4, 8), (8, 11)]
+    got, cnt, _ = run_engine(pkg, st, 4, 1.5, 1, loads=loads, nOnGPU=4)
+    exp, ocnt, _ = run_oracle(O, st, 4, 1.5, 1, loads=loads, nOnGPU=4)
+    assert cnt == ocnt == [11, 3]
+    assert_parity(got.reshape(4, 64, 64), exp.reshape(4, 64, 64), "FHT loads 4+4+3")
+
+
+def test_fht_drift_generic_path(pkg, O):
+    st = stack(6, 16, 16, seed=31)
+    rng = np.random.default_rng(3)
+    drift = (rng.uniform(-0.3, 0.3, 6).astype(np.float32),
+             rng.uniform(-0.3, 0.3, 6).astype(np.float32))
+    drift[0][2] = drift[1][2] = 0.0          # one unshifted frame inside the series
+    got, _, _ = run_engine(pkg, st, 3, 1.5, 1, drift=drift)
+    exp, _, _ = run_oracle(O, st, 3, 1.5, 1, drift=drift)
+    assert_parity(got.reshape(4, 48, 48), exp.reshape(4, 48, 48), "FHT drift")
+
+
+def test_fht_mp_maps_are_absent(pkg):
+    from nanoj_esrrf_amd import _capi
+    st = stack(3, 16, 16, seed=1)
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(16, 16, 2, 1.5, 1, 3, 3, 20000, e.allCLdevices[0], True, True, True, None, 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(3), np.zeros(3))
+    e.calculateSRRF(st)
+    with pytest.raises(_capi.EsrrfError):
+        e.readMPmaps()
+    e.release()
+
+
+def test_fht_upsampling_of_band_limited_frames(pkg):
+    """A frame that is a sum of DFT modes of the 32 x 32 grid (no Nyquist term) is upsampled
+    exactly: every M-th magnified pixel reproduces the input, and the magnified frame's
+    RobX gradient equals the closed form of the continuous cosine field (up to 1/M^2)."""
+    w = h = 32
+    M = 3
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    f = 5.0 + np.cos(2 * np.pi * (3 * x + 2 * y) / 32) + 0.5 * np.sin(2 * np.pi * (x - 5 * y) / 32)
+    st = np.repeat(f[None].astype(np.float32), 3, axis=0)
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(w, h, M, 1.5, 1, 3, 3, 20000, e.allCLdevices[0], True, False, True, None, 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(3), np.zeros(3))
+    e.calculateSRRF(st)
+    out = e.readSRRFbuffer().reshape(4, h * M, w * M)
+    e.release()
+    # INT channel = mean of the (scale 1/M^2) upsampled frames
+    up = out[3] * M * M
+    assert np.abs(up[::M, ::M] - f).max() < 1e-4 * np.abs(f).max()
+    ym, xm = np.mgrid[0:h * M, 0:w * M].astype(np.float64) / M
+    fm = 5.0 + np.cos(2 * np.pi * (3 * xm + 2 * ym) / 32) + \
+        0.5 * np.sin(2 * np.pi * (xm - 5 * ym) / 32)
+    assert np.abs(up - fm).max() < 2e-4 * np.abs(fm).max()

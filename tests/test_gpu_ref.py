@@ -100,7 +100,7 @@ def test_bic_drift_matches_reference_kernels(pkg, O, name):
 
 
 @pytest.mark.parametrize("name", FHT)
-def test_fht_oracle_matches_reference_kernels(O, name):
+def test_fht_oracle_and_engine_match_reference_kernels(pkg, O, name):
     from oracle import fht
     cfg = refrun.configs()[name]
     _, w, h, _, M, fwhm, S, nF, iw = cfg
@@ -111,6 +111,18 @@ def test_fht_oracle_matches_reference_kernels(O, name):
     o.calculateSRRF(st.astype(np.float32))
     exp = o.readSRRFbuffer()
     assert_close(exp, ref, f"oracle vs reference {name}")
+    # HIP engine (its own FFT upsampling) against the reference kernels fed with the
+    # oracle's upsampled frames
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(w, h, M, fwhm, S, nF, nF, 20000, e.allCLdevices[0], bool(iw), True, True,
+                 None, 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(nF), np.zeros(nF))
+    e.calculateSRRF(st)
+    got = e.readSRRFbuffer()
+    e.release()
+    assert_close(got, ref, f"HIP vs reference {name}")
 
 
 @pytest.mark.parametrize("name", D3)
