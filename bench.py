@@ -39,7 +39,25 @@ def parse():
     ap.add_argument("--frames", type=int, default=NF, help="frames per SR frame (default 1000)")
     ap.add_argument("--cpu-sample-frames", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=("bic", "fht"), default="bic",
+                    help="bic: the production workload (default); fht: the same stack through "
+                         "the Fourier-upsampling mode")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 plain 1 GiB device copies (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Three device-to-device copies of 1 GiB: the reference reading for rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters (known traffic 1 GiB read + 1 GiB written per copy)."""
+    import torch
+    n = 256 * 1024 * 1024
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 3}))
 
 
 def cpu_baseline(stack, nframes):
@@ -62,21 +80,28 @@ def cpu_baseline(stack, nframes):
 
 
 def load_traffic():
-    """HBM bytes per RGC launch from the committed rocprofv3 PMC summary (or None)."""
-    p = os.path.join(REPO, "profiles", "rgc_bic_256x256x1000_m5_traffic.json")
+    """HBM bytes per RGC launch from the committed rocprofv3 PMC summary (or None): the
+    line `hbm_bytes_per_launch: <int>` of profiles/k_bic_rgc_256x256x1000_m5_traffic.md."""
+    p = os.path.join(REPO, "profiles", "k_bic_rgc_256x256x1000_m5_traffic.md")
     try:
         with open(p) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
+            for line in f:
+                if line.strip().startswith("hbm_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
     except (OSError, ValueError):
         return None
+    return None
 
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design "
-                                   "only (see DESIGN.md)", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not "
+                                   "supported (multi-GPU sharding is a design only)",
+                          "n_gpus": a.gpus}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -98,7 +123,8 @@ def main():
     p.width, p.height, p.magnification = W, H, M
     p.fwhm, p.sensitivity = FWHM, SENS
     p.n_frames_on_gpu, p.n_frame_for_srrf, p.block_length, p.device = nf, nf, 20000, 0
-    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, 1, 0
+    fht = a.mode == "fht"
+    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, 1, int(fht)
     eng = pkg.Engine(p)
     L, h = eng.L, eng.h
     eng.check(L.esrrf_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
@@ -130,6 +156,8 @@ def main():
     eng.check(L.esrrf_profile_read(h, 0, ctypes.byref(rgc_ms), ctypes.byref(rgc_n)))
     prep_ms, prep_n = ctypes.c_double(), ctypes.c_long()
     eng.check(L.esrrf_profile_read(h, 1, ctypes.byref(prep_ms), ctypes.byref(prep_n)))
+    up_ms, up_n = ctypes.c_double(), ctypes.c_long()
+    eng.check(L.esrrf_profile_read(h, 2, ctypes.byref(up_ms), ctypes.byref(up_n)))
     eng.check(L.esrrf_profile_enable(h, 0))
 
     ms_step = ms_total / a.steps
@@ -153,22 +181,27 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded blinking Bezier filaments, Poisson + read noise, uint16)",
         "config": {"workload": "production 2-D eSRRF: 1000 frames 256x256 uint16 -> 1280x1280, "
-                               "M=5, fwhm 1.5, S=2, VAR+AVG+TAC2+INT, MP correction, BIC",
+                               "M=5, fwhm 1.5, S=2, VAR+AVG+TAC2+INT, MP correction, "
+                               + ("FHT upsampling (MP skipped as in the reference)" if fht
+                                  else "BIC"),
                    "frames": nf, "width": W, "height": H, "magnification": M,
                    "fwhm": FWHM, "sensitivity": SENS},
-        "roofline": {"bound": "valu", "kernel": "k_bic_rgc",
+        "roofline": {"bound": "valu", "kernel": "k_fht_rgc" if fht else "k_bic_rgc",
                      "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": load_traffic(),
+                     "traffic": None if fht else load_traffic(),
                      "kernel_ms": round(rgc_avg, 3), "launches": rgc_n.value,
                      "algorithmic_flop_per_launch": flops,
                      "gated_samples_per_frame": nact.value,
-                     "prep_kernel_ms": round(prep_ms.value / max(prep_n.value, 1), 3)},
+                     "prep_kernel_ms": round(prep_ms.value / max(prep_n.value, 1), 3)
+                     if not fht else None,
+                     "upsample_ms_per_load": round(up_ms.value / max(up_n.value, 1), 3)
+                     if fht else None},
         "wall_s": round(wall, 3),
     }
     eng.close()
     del d_frames, d_out
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and not fht:
         res["cpu_baseline"] = cpu_baseline(stack, min(a.cpu_sample_frames, nf))
     print(json.dumps(res))
 
