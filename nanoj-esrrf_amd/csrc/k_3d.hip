@@ -1,0 +1,411 @@
+// k_3d.hip -- 3-D (multi-focus microscopy) eSRRF: resources/live3DSRRF.cl and the montage
+// split of LiveSRRF_CL.reshapeImageStack3D (LiveSRRF_CL.java:963-994).
+//
+// Per load:
+//   k3d_reshape   montage cell -> plane, (p - background) / intensity scaling  (:983-986)
+//   k3d_align     rotate about the plane centre by -theta, shift (live3DSRRF.cl:204-224)
+//   k3d_grad      2-point gradients Gx, Gy, Gz, edge-clamped (live3DSRRF.cl:229-279)
+//   k3d_unit      the three staggered reads of the lateral 2x interpolated gradients the
+//                 RGC makes at a sample point (live3DSRRF.cl:282-320, :389-391)
+//                     T(n, m, z) = (GxInt[z][m][n+1], GyInt[z][m+1][n], GzInt[z][m][n])
+//                 folded into one unit 3-vector per point of the 2x grid.
+// Per launch of frames:
+//   k3d_lateral   lateral Catmull-Rom intensity of every plane at every magnified (x, y)
+//                 (the four per-voxel plane samples of live3DSRRF.cl:441-447 come from it)
+//   k3d_rgc       3-D radial gradient convergence + axial cubic + temporal cumulants
+//                 (live3DSRRF.cl:325-498), cumulants in registers across frames.  Frames
+//                 without drift share one geometry, evaluated once per chunk of FC frames.
+#include "device_interp.h"
+#include "esrrf_internal.h"
+#include "mode3d_internal.h"
+
+namespace esrrf {
+
+static inline unsigned grid1(long n, int bs = 256)
+{
+    long g = (n + bs - 1) / bs;
+    if (g > 65535L * 64) g = 65535L * 64;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------------------
+__global__ void k3d_reshape(const void *__restrict__ src, int dtype, float *__restrict__ planes,
+                            int W, int H, int w, int h, int nP, const int *__restrict__ cell,
+                            const float *__restrict__ ic, float bg, long n)
+{
+    const long wh = (long)w * h, whd = wh * nP;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const long f = o / whd;
+        const int z = (int)((o - f * whd) / wh);
+        const long r = o - f * whd - (long)z * wh;
+        const int yy = (int)(r / w), xx = (int)(r - (long)yy * w);
+        const long s = (long)W * H * f + (long)(cell[nP + z] + yy) * W + (cell[z] + xx);
+        const float p = dtype == ESRRF_U16 ? (float)((const unsigned short *)src)[s]
+                                           : ((const float *)src)[s];
+        planes[o] = (p - bg) / ic[z];
+    }
+}
+
+__global__ void k3d_align(const float *__restrict__ planes, float *__restrict__ aligned, int w,
+                          int h, int nP, const float *__restrict__ st, long n)
+{
+    const long wh = (long)w * h, whd = wh * nP;
+    const float cw = (float)(w - 1) / 2.0f, ch = (float)(h - 1) / 2.0f;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const long f = o / whd;
+        const int z = (int)((o - f * whd) / wh);
+        const long r = o - f * whd - (long)z * wh;
+        const int y = (int)(r / w), x = (int)(r - (long)y * w);
+        const float ct = cosf(st[z]), sn = sinf(st[z]);
+        const float xr = (float)x - cw, yr = (float)y - ch;
+        float xa = __fmul_rn(xr, ct);
+        xa = __fsub_rn(xa, __fmul_rn(yr, sn));
+        xa = __fadd_rn(xa, cw);
+        xa = __fsub_rn(xa, st[z + nP]);
+        float ya = __fmul_rn(xr, sn);
+        ya = __fadd_rn(ya, __fmul_rn(yr, ct));
+        ya = __fadd_rn(ya, ch);
+        ya = __fsub_rn(ya, st[z + 2 * nP]);
+        aligned[o] = interp_global(planes + whd * f + wh * z, w, h, xa, ya);
+    }
+}
+
+// g: [3][nF*nP][h][w]
+__global__ void k3d_grad(const float *__restrict__ px, float *__restrict__ g, int w, int h,
+                         int nP, long n)
+{
+    const long wh = (long)w * h, whd = wh * nP;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const long f = o / whd;
+        const int z1 = (int)((o - f * whd) / wh);
+        const long r = o - f * whd - (long)z1 * wh;
+        const int y1 = (int)(r / w), x1 = (int)(r - (long)y1 * w);
+        const int x0 = max(x1 - 1, 0), y0 = max(y1 - 1, 0), z0 = max(z1 - 1, 0);
+        const float p = px[o];
+        const long base = whd * f;
+        g[o] = p - px[base + wh * z1 + (long)y1 * w + x0];
+        g[n + o] = p - px[base + wh * z1 + (long)y0 * w + x1];
+        g[2 * n + o] = p - px[base + wh * z0 + (long)y1 * w + x1];
+    }
+}
+
+__device__ __forceinline__ float4 unit3_or_zero(float tx, float ty, float tz)
+{
+    const float m = fmaxf(fmaxf(fabsf(tx), fabsf(ty)), fabsf(tz));
+    if (!(m > 0.0f) || !(m <= 3.4e38f)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float sx = tx / m, sy = ty / m, sz = tz / m;
+    const float r = rsqrtf(sx * sx + sy * sy + sz * sz);
+    return make_float4(sx * r, sy * r, sz * r, 0.0f);
+}
+
+// U: [nF*nP][2h][2w]; the last column and row of the 2x grid are never sampled (gates).
+__global__ void k3d_unit(const float *__restrict__ g, float4 *__restrict__ U, int w, int h,
+                         long npl)
+{
+    const int wI = 2 * w, hI = 2 * h;
+    const long whI = (long)wI * hI, wh = (long)w * h;
+    const long n = npl * whI;
+    const long gn = npl * wh;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const long pl = o / whI;
+        const long r = o - pl * whI;
+        const int m = (int)(r / wI), nn = (int)(r - (long)m * wI);
+        float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (nn <= wI - 2 && m <= hI - 2) {
+            const float *Gx = g + wh * pl, *Gy = g + gn + wh * pl, *Gz = g + 2 * gn + wh * pl;
+            const float tx = interp_global(Gx, w, h, (float)(nn + 1) / 2.0f, (float)m / 2.0f);
+            const float ty = interp_global(Gy, w, h, (float)nn / 2.0f, (float)(m + 1) / 2.0f);
+            const float tz = interp_global(Gz, w, h, (float)nn / 2.0f, (float)m / 2.0f);
+            u = unit3_or_zero(tx, ty, tz);
+        }
+        U[o] = u;
+    }
+}
+
+// GxInt, GyInt, GzInt of frame 0, every plane: out [nP][3][2h][2w] (LiveSRRF_CL.java:855-872)
+__global__ void k3d_gint(const float *__restrict__ g, float *__restrict__ out, int w, int h,
+                         int nP, long gn)
+{
+    const int wI = 2 * w, hI = 2 * h;
+    const long whI = (long)wI * hI, wh = (long)w * h;
+    const long n = (long)nP * 3 * whI;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const int z = (int)(o / (3 * whI));
+        const int c = (int)((o - (long)z * 3 * whI) / whI);
+        const long r = o - (long)z * 3 * whI - (long)c * whI;
+        const int y = (int)(r / wI), x = (int)(r - (long)y * wI);
+        out[o] = interp_global(g + c * gn + wh * z, w, h, (float)x / 2.0f, (float)y / 2.0f);
+    }
+}
+
+// L: [nframes][nP][hM][wM]
+__global__ void k3d_lateral(const float *__restrict__ px, float *__restrict__ L, int w, int h,
+                            int nP, int M, int frame0, int nframes, const float *__restrict__ drift,
+                            int count0, int nFor)
+{
+    const int wM = w * M, hM = h * M;
+    const long whM = (long)wM * hM, wh = (long)w * h;
+    const long n = (long)nframes * nP * whM;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const int fi = (int)(o / (nP * whM));
+        const int z = (int)((o - (long)fi * nP * whM) / whM);
+        const long r = o - (long)fi * nP * whM - (long)z * whM;
+        const int yM = (int)(r / wM), xM = (int)(r - (long)yM * wM);
+        const float dX = drift[count0 + fi], dY = drift[count0 + fi + nFor];
+        const float xi = __fsub_rn(__fadd_rn((float)xM / (float)M, dX), 0.5f);
+        const float yi = __fsub_rn(__fadd_rn((float)yM / (float)M, dY), 0.5f);
+        L[o] = interp_global(px + wh * ((long)(frame0 + fi) * nP + z), w, h, xi, yi);
+    }
+}
+
+__device__ __forceinline__ float apply_sens3(float C, float sens, int sens_int)
+{
+    if (!(C >= 0.0f)) return 0.0f;
+    switch (sens_int) {
+    case 1: return C;
+    case 2: return C * C;
+    case 3: return C * C * C;
+    case 4: { const float c2 = C * C; return c2 * c2; }
+    default: return powf(C, sens);
+    }
+}
+
+__device__ __forceinline__ float cross_mag3(float a1, float a2, float a3, float b1, float b2,
+                                            float b3)
+{
+    const float G1 = a2 * b3 - a3 * b2;
+    const float G2 = a3 * b1 - a1 * b3;
+    const float G3 = a1 * b2 - a2 * b1;
+    return sqrtf(G1 * G1 + G2 * G2 + G3 * G3);
+}
+
+// One thread per output voxel; block 64 x 4 voxels of one magnified z-slice.
+template <int FC>
+__global__ void __launch_bounds__(256) k3d_rgc(const Rgc3dArgs a)
+{
+    const int xM = blockIdx.x * 64 + threadIdx.x;
+    const int yM = blockIdx.y * 4 + threadIdx.y;
+    const int zM = blockIdx.z;
+    if (xM >= a.wM || yM >= a.hM) return;
+    const int w = a.w, h = a.h, nP = a.nP, M = a.M;
+    const long whM = (long)a.wM * a.hM;
+    const long whdM = whM * nP * M;
+    const long vox = whM * zM + (long)yM * a.wM + xM;
+    const long whI = 4L * w * h;
+    const int wI = 2 * w;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+    if (a.count0 > 0) {
+        s0 = a.out[vox];
+        s1 = a.out[vox + whdM];
+        s2 = a.out[vox + 2 * whdM];
+        s3 = a.out[vox + 3 * whdM];
+        s4 = a.prev[vox];
+    }
+    const float zc = (float)((zM + 0.5) / M);
+    int z0;
+    const float zcof = zc - 0.5f;
+    if (zcof < 0.0f) z0 = 0;
+    else if (zcof >= (float)(nP - 1)) z0 = nP - 2;
+    else z0 = (int)floorf(zcof);
+    const int zm1 = max(z0 - 1, 0), zp2 = min(z0 + 1, nP - 1);
+    const float z01 = zc - ((float)z0 + 0.5f);
+    const int jlo = -(int)(2.0f * a.radius), jhi = (int)(2.0f * a.radius + 1.0f);
+    const int klo = -(int)a.radius, khi = (int)a.radius + 1;
+    const float nFf = (float)a.nFor, nF1 = (float)(a.nFor - 1);
+
+    for (int f0 = 0; f0 < a.nframes; f0 += FC) {
+        const int nf = min(FC, a.nframes - f0);
+        const int c00 = a.count0 + f0;
+        const float driftX = a.drift[c00], driftY = a.drift[c00 + a.nFor];
+        const float xc = (float)((xM + 0.5) / M + (double)driftX);
+        const float yc = (float)((yM + 0.5) / M + (double)driftY);
+        float acc[FC];
+#pragma unroll
+        for (int q = 0; q < FC; q++) acc[q] = 0.0f;
+        float sumW = 0.0f;
+        const float4 *Ub = a.U + (long)(a.frame0 + f0) * nP * whI;
+        const float bx = (float)((int)(2.0f * xc)), by = (float)((int)(2.0f * yc));
+        for (int j = jlo; j <= jhi; j++) {
+            const float vy = (by + (float)j) / 2.0f;
+            if (!(vy > 0.0f && vy < (float)h)) continue;
+            const int m = (int)(2.0 * ((double)vy - 0.5));
+            for (int i = jlo; i <= jhi; i++) {
+                const float vx = (bx + (float)i) / 2.0f;
+                if (!(vx > 0.0f && vx < (float)w)) continue;
+                const int nn = (int)(2.0 * ((double)vx - 0.5));
+                const float dx = vx - xc, dy = vy - yc;
+                for (int k = klo; k <= khi; k++) {
+                    const float vz = (float)((int)zc) + (float)k;
+                    if (!(vz > 0.0f && vz < (float)nP)) continue;
+                    const float dz = vz - zc;
+                    float d2 = __fmul_rn(dx, dx);
+                    d2 = __fadd_rn(d2, __fmul_rn(dy, dy));
+                    d2 = __fadd_rn(d2, __fmul_rn(dz, dz));
+                    const float d = sqrtf(d2);
+                    if (!(d != 0.0f && d <= a.tSO)) continue;
+                    float dw = d * expf(-(d * d) / a.tSS);
+                    dw = dw * dw * dw * dw;
+                    sumW += dw;
+                    const float invd = 1.0f / d;
+                    const long idx = whI * (int)vz + (long)m * wI + nn;
+#pragma unroll
+                    for (int q = 0; q < FC; q++) {
+                        if (q < nf) {
+                            const float4 u = Ub[(long)q * nP * whI + idx];
+                            const float gd = u.x * dx + u.y * dy + u.z * dz;
+                            const float Dk = 1.0f - cross_mag3(dx, dy, dz, u.x, u.y, u.z) * invd;
+                            acc[q] += gd < 0.0f ? Dk * dw : 0.0f;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < FC; q++) {
+            if (q < nf) {
+                const int fi = f0 + q;
+                float C = acc[q] / sumW;
+                C = apply_sens3(C, a.sens, a.sens_int);
+                const float *Lf = a.L + (long)fi * nP * whM + (long)yM * a.wM + xM;
+                const float v_1 = Lf[whM * zm1], v0 = Lf[whM * z0];
+                const float v1 = Lf[whM * (z0 + 1)], v2 = Lf[whM * zp2];
+                // axial cubic (live3DSRRF.cl:448-452), left-to-right as written
+                float aa = __fmul_rn(-0.5f, v_1);
+                aa = __fadd_rn(aa, __fmul_rn(1.5f, v0));
+                aa = __fsub_rn(aa, __fmul_rn(1.5f, v1));
+                aa = __fadd_rn(aa, __fmul_rn(0.5f, v2));
+                float bb = __fsub_rn(v_1, __fmul_rn(2.5f, v0));
+                bb = __fadd_rn(bb, __fmul_rn(2.0f, v1));
+                bb = __fsub_rn(bb, __fmul_rn(0.5f, v2));
+                const float cc = __fadd_rn(__fmul_rn(-0.5f, v_1), __fmul_rn(0.5f, v1));
+                float v = __fmul_rn(__fmul_rn(__fmul_rn(aa, z01), z01), z01);
+                v = __fadd_rn(v, __fmul_rn(__fmul_rn(bb, z01), z01));
+                v = __fadd_rn(v, __fmul_rn(cc, z01));
+                v = __fadd_rn(v, v0);
+                const float t = a.int_weighting ? v * C : C;
+                const float e2 = a.int_weighting ? ((t * v) * C) : C * C;
+                if (c00 + q == 0) {
+                    s0 = t / nFf;
+                    s1 = e2 / nFf;
+                    s2 = 0.0f;
+                    s3 = v / nFf;
+                } else {
+                    s0 = s0 + t / nFf;
+                    s1 = s1 + e2 / nFf;
+                    s2 = s2 + (t * s4) / nF1;
+                    s3 = s3 + v / nFf;
+                }
+                s4 = t;   // single 1/(n-1) in TAC2 (live3DSRRF.cl:472)
+            }
+        }
+    }
+    a.out[vox] = s0;
+    a.out[vox + whdM] = s1;
+    a.out[vox + 2 * whdM] = s2;
+    a.out[vox + 3 * whdM] = s3;
+    a.prev[vox] = s4;
+}
+
+// Gated samples of one frame without drift, summed over all voxels.
+__global__ void __launch_bounds__(256) k3d_count(int w, int h, int nP, int M, float radius,
+                                                 float tSO, unsigned long long *total)
+{
+    const int wM = w * M, hM = h * M;
+    const long n = (long)wM * hM * nP * M;
+    unsigned long long cnt = 0;
+    const int jlo = -(int)(2.0f * radius), jhi = (int)(2.0f * radius + 1.0f);
+    const int klo = -(int)radius, khi = (int)radius + 1;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
+         o += (long)gridDim.x * blockDim.x) {
+        const int zM = (int)(o / ((long)wM * hM));
+        const long r = o - (long)zM * wM * hM;
+        const int yM = (int)(r / wM), xM = (int)(r - (long)yM * wM);
+        const float xc = (float)((xM + 0.5) / M), yc = (float)((yM + 0.5) / M);
+        const float zc = (float)((zM + 0.5) / M);
+        const float bx = (float)((int)(2.0f * xc)), by = (float)((int)(2.0f * yc));
+        for (int j = jlo; j <= jhi; j++) {
+            const float vy = (by + (float)j) / 2.0f;
+            if (!(vy > 0.0f && vy < (float)h)) continue;
+            for (int i = jlo; i <= jhi; i++) {
+                const float vx = (bx + (float)i) / 2.0f;
+                if (!(vx > 0.0f && vx < (float)w)) continue;
+                const float dx = vx - xc, dy = vy - yc;
+                for (int k = klo; k <= khi; k++) {
+                    const float vz = (float)((int)zc) + (float)k;
+                    if (!(vz > 0.0f && vz < (float)nP)) continue;
+                    const float dz = vz - zc;
+                    float d2 = __fmul_rn(dx, dx);
+                    d2 = __fadd_rn(d2, __fmul_rn(dy, dy));
+                    d2 = __fadd_rn(d2, __fmul_rn(dz, dz));
+                    const float d = sqrtf(d2);
+                    if (d != 0.0f && d <= tSO) cnt++;
+                }
+            }
+        }
+    }
+    __shared__ unsigned long long red[256];
+    const int tid = threadIdx.x;
+    red[tid] = cnt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) atomicAdd(total, red[0]);
+}
+
+// ---------------------------------------------------------------------------------------
+hipError_t launch3d_prepare(const void *src, int dtype, const Prep3dArgs &p, int n,
+                            hipStream_t s)
+{
+    const long npx = (long)n * p.nP * p.w * p.h;
+    if (npx == 0) return hipSuccess;
+    hipLaunchKernelGGL(k3d_reshape, dim3(grid1(npx)), dim3(256), 0, s, src, dtype, p.planes,
+                       p.W, p.H, p.w, p.h, p.nP, p.cell, p.ic, p.bg, npx);
+    hipLaunchKernelGGL(k3d_align, dim3(grid1(npx)), dim3(256), 0, s, p.planes, p.aligned, p.w,
+                       p.h, p.nP, p.st, npx);
+    hipLaunchKernelGGL(k3d_grad, dim3(grid1(npx)), dim3(256), 0, s, p.aligned, p.g, p.w, p.h,
+                       p.nP, npx);
+    hipLaunchKernelGGL(k3d_unit, dim3(grid1(4 * npx)), dim3(256), 0, s, p.g, p.U, p.w, p.h,
+                       (long)n * p.nP);
+    return hipGetLastError();
+}
+
+hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long gn,
+                         hipStream_t s)
+{
+    const long n = (long)nP * 3 * 4 * w * h;
+    hipLaunchKernelGGL(k3d_gint, dim3(grid1(n)), dim3(256), 0, s, g, out, w, h, nP, gn);
+    return hipGetLastError();
+}
+
+hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s)
+{
+    if (a.nframes <= 0) return hipSuccess;
+    const long nl = (long)a.nframes * a.nP * a.wM * a.hM;
+    hipLaunchKernelGGL(k3d_lateral, dim3(grid1(nl)), dim3(256), 0, s, a.px, a.L, a.w, a.h, a.nP,
+                       a.M, a.frame0, a.nframes, a.drift, a.count0, a.nFor);
+    const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.M);
+    if (drift_zero)
+        hipLaunchKernelGGL(k3d_rgc<8>, grid, dim3(64, 4), 0, s, a);
+    else
+        hipLaunchKernelGGL(k3d_rgc<1>, grid, dim3(64, 4), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch3d_count(int w, int h, int nP, int M, float radius, float tSO,
+                          unsigned long long *d_total, hipStream_t s)
+{
+    const long n = (long)w * M * h * M * nP * M;
+    hipLaunchKernelGGL(k3d_count, dim3(grid1(n)), dim3(256), 0, s, w, h, nP, M, radius, tSO,
+                       d_total);
+    return hipGetLastError();
+}
+
+}  // namespace esrrf

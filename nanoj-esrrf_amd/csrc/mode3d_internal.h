@@ -1,8 +1,43 @@
-// mode3d_internal.h -- entry points of the 3-D (multi-focus) pipeline.
+// mode3d_internal.h -- the 3-D (multi-focus) pipeline: host entry points and kernel
+// launch parameters.
 #pragma once
 #include "modes.h"
 
 namespace esrrf {
+
+struct Prep3dArgs {
+    int W, H;              // montage frame
+    int w, h, nP;          // one plane
+    const int *cell;       // [2][nP] montage cell origin x, y (LiveSRRF_CL.java:973-979)
+    const float *ic;       // [nP] intensity scaling
+    float bg;              // background level
+    const float *st;       // [3][nP] -theta (rad), shiftX, shiftY (LiveSRRF_CL.java:440-445)
+    float *planes;         // [nF][nP][h][w] reshaped
+    float *aligned;        // [nF][nP][h][w] aligned
+    float *g;              // [3][nF*nP][h][w] 2-point gradients
+    float4 *U;             // [nF*nP][2h][2w] unit tap gradients
+};
+
+struct Rgc3dArgs {
+    const float4 *U;
+    const float *px;       // aligned planes
+    float *L;              // [nframes][nP][hM][wM] lateral intensities (scratch)
+    const float *drift;    // [2][nFor]
+    float *out, *prev;
+    int w, h, nP, M, wM, hM;
+    int frame0, nframes, count0, nFor;
+    float sens;
+    int sens_int, int_weighting;
+    float radius, tSS, tSO;
+};
+
+hipError_t launch3d_prepare(const void *src, int dtype, const Prep3dArgs &p, int n,
+                            hipStream_t s);
+hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long gn,
+                         hipStream_t s);
+hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s);
+hipError_t launch3d_count(int w, int h, int nP, int M, float radius, float tSO,
+                          unsigned long long *d_total, hipStream_t s);
 
 int mode3d_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U);
 int mode3d_prepare(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n);

@@ -291,8 +291,9 @@ long long mode_active_samples(int, const ModeState &ms) { return ms.active; }
 
 void free_mode_state(ModeState &ms)
 {
-    void *ptrs[] = {ms.d_rawf, ms.plan.F, ms.plan.Y, (void *)ms.plan.tw, ms.d_taps, ms.d_tapk,
-                    ms.d_st, ms.d_roi, ms.d_ic, ms.d_gi, ms.d_g3, ms.d_planes};
+    void *ptrs[] = {ms.d_rawf, ms.plan.F,   ms.plan.Y, (void *)ms.plan.tw, ms.d_taps,
+                    ms.d_tapk, ms.d_st,     ms.d_cell, ms.d_icf,           ms.d_planes,
+                    ms.d_g3,   ms.d_u3,     ms.d_lat};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ms.d_rawf = nullptr;
@@ -301,8 +302,10 @@ void free_mode_state(ModeState &ms)
     ms.d_taps = nullptr;
     ms.d_tapk = nullptr;
     ms.d_st = nullptr;
-    ms.d_roi = ms.d_ic = nullptr;
-    ms.d_gi = ms.d_g3 = ms.d_planes = nullptr;
+    ms.d_cell = nullptr;
+    ms.d_icf = nullptr;
+    ms.d_planes = ms.d_g3 = ms.d_lat = nullptr;
+    ms.d_u3 = nullptr;
 }
 
 }  // namespace esrrf

@@ -30,12 +30,15 @@ struct ModeState {
     float tap_wsum = 0.0f;
     int fht_R = 0, fht_TW = 0, fht_pitch = 0;
     // 3-D
-    float *d_st = nullptr;        // [3][nP] theta(rad), shiftX, shiftY (LiveSRRF_CL.java:440-445)
-    double *d_roi = nullptr, *d_ic = nullptr;
+    float *d_st = nullptr;        // [3][nP] -theta (rad), shiftX, shiftY (LiveSRRF_CL.java:440-445)
+    int *d_cell = nullptr;        // [2][nP] montage cell origins
+    float *d_icf = nullptr;       // [nP] intensity scaling
     float bg = 0.0f;
-    float *d_gi = nullptr;        // interpolated gradients
-    float *d_g3 = nullptr;        // 3-D gradients
-    float *d_planes = nullptr;    // montage split into planes
+    float *d_planes = nullptr;    // reshaped planes [nF][nP][h][w]
+    float *d_g3 = nullptr;        // 2-point gradients [3][nF*nP][h][w]
+    float4 *d_u3 = nullptr;       // unit tap gradients [nF*nP][2h][2w]
+    float *d_lat = nullptr;       // lateral intensities [nF][nP][hM][wM]
+    int n_loaded = 0;             // frames of the current load (gradient plane stride)
 };
 
 struct ModeCtx {

@@ -122,6 +122,7 @@ class LiveSRRF_CL:
                     ("X-shift (pixels)", "Y-shift (pixels)", "Theta (degrees)", "ROI #",
                      "Intensity scaling")}
             self._calib = cols
+            self._bg = float(np.asarray(t.get("Background level (ADC)", [0.0]))[0])
             dp = ctypes.POINTER(ctypes.c_double)
             p.n_planes = len(cols["X-shift (pixels)"])
             p.shift_x_3d = cols["X-shift (pixels)"].ctypes.data_as(dp)
@@ -129,7 +130,7 @@ class LiveSRRF_CL:
             p.theta_3d = cols["Theta (degrees)"].ctypes.data_as(dp)
             p.roi_3d = cols["ROI #"].ctypes.data_as(dp)
             p.int_coeffs_3d = cols["Intensity scaling"].ctypes.data_as(dp)
-            p.background_3d = float(np.asarray(t.get("Background level (ADC)", [0.0]))[0])
+            p.background_3d = self._bg
             p.axial_offset = float(axialOffset)
             p.pixel_size = float(pixelSize)
         self._eng = Engine(p)
@@ -231,7 +232,7 @@ class LiveSRRF_CL:
         ims = np.asarray(ims, dtype=np.float32)
         n, H, W = ims.shape
         wS, hS = W // 3, H // 3
-        bg = np.float32(0.0)
+        bg = np.float32(self._bg)
         out = np.empty((n, len(c["ROI #"]), hS, wS), np.float32)
         for z, roi in enumerate(c["ROI #"]):
             i = int(roi) % 3

@@ -1,0 +1,158 @@
+"""3-D (live3DSRRF.cl) parity of the HIP engine against the CPU oracle: montage split,
+alignment, 3-D gradients, lateral interpolation, 3-D RGC, axial cubic, cumulants, 3-D MP.
+
+Tolerance: per channel, both images divided by the oracle image's max |value|,
+|a - b| <= 1e-5 + 1e-4 |b|.  Frame counters: exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import normalised_close
+
+pytestmark = pytest.mark.gpu
+
+
+def planes_stack(n, w, h, nP, seed):
+    from nanoj_esrrf_amd import synth
+    return synth.blinking_stack_3d(n, w, h, nP, n_emitters=max(40, w * h // 5), n_filaments=4,
+                                   seed=seed)
+
+
+def montage(planes, rois, W=None, H=None):
+    """Place plane z at montage cell rois[z] using LiveSRRF_CL's cell origins."""
+    n, nP, h, w = planes.shape
+    W = W or 3 * w
+    H = H or 3 * h
+    out = np.zeros((n, H, W), planes.dtype)
+    for z in range(nP):
+        i = int(rois[z]) % 3
+        j = (int(rois[z]) - i) // 3
+        x = int(np.floor(np.float32(W) / np.float32(3) * np.float32(i) + np.float32(0.5)))
+        y = int(np.floor(np.float32(H) / np.float32(3) * np.float32(j) + np.float32(0.5)))
+        out[:, y:y + h, x:x + w] = planes[:, z]
+    return out
+
+
+def calib_dict(sx, sy, th, roi, ic, bg):
+    return {"X-shift (pixels)": sx, "Y-shift (pixels)": sy, "Theta (degrees)": th,
+            "ROI #": roi, "Intensity scaling": ic, "Background level (ADC)": [bg]}
+
+
+def run_both(pkg, O, mont, nP, M, fwhm, S, calib, intW=True, mp=True, drift=None, loads=None,
+             nOnGPU=None, extra=None):
+    n, H, W = mont.shape
+    nOnGPU = nOnGPU or n
+    cd = calib_dict(*calib)
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(W, H, M, fwhm, S, nOnGPU, n, 20000, e.allCLdevices[0], intW, mp, False, cd,
+                 0, 1)
+    e.resetFramePosition()
+    dx, dy = (np.zeros(n), np.zeros(n)) if drift is None else drift
+    e.loadDriftXYGPUbuffer(dx, dy)
+    o = O.OracleLiveSRRF(W, H, M, fwhm, S, nOnGPU, n, intW, mp, False,
+                         calib3d=O.Calib3D(*[np.asarray(c, np.float64) for c in calib[:5]],
+                                           float(calib[5])))
+    if drift is not None:
+        o.loadDriftXYGPUbuffer(*drift)
+    got_extra = {}
+    for a, b in (loads or [(0, n)]):
+        e.calculateSRRF(mont[a:b])
+        o.calculateSRRF(mont[a:b].astype(np.float32))
+        if extra and a == 0:
+            got_extra["grad"] = (e.readGradientBuffers(), o.readGradientBuffers())
+            got_extra["aligned"] = (e.readAlignedPixels(), o.readAlignedPixels())
+    got = e.readSRRFbuffer()
+    cnt = e.frameCounters()
+    exp = o.readSRRFbuffer()
+    if mp:
+        got_extra["mp"] = (e.readMPmaps(), o.readMPmaps())
+    e.release()
+    return got, exp, cnt, list(o.counters), got_extra
+
+
+def assert_parity(got, exp, what):
+    got = np.asarray(got).reshape(4, -1)
+    exp = np.asarray(exp).reshape(4, -1)
+    for c, name in enumerate(("AVG", "VAR", "TAC2", "INT")):
+        ok, ex, idx = normalised_close(got[c], exp[c])
+        assert ok, f"{what}: {name} differs (excess {ex:.3g} at {idx})"
+
+
+def zero_calib(nP):
+    z = np.zeros(nP)
+    return (z, z, z, np.arange(nP, dtype=np.float64), np.ones(nP), 0.0)
+
+
+@pytest.mark.parametrize("w,h,nP,M,fwhm,S", [
+    (16, 16, 4, 2, 1.5, 2), (12, 14, 3, 3, 1.5, 1), (20, 16, 9, 4, 1.5, 2),
+    (16, 12, 5, 2, 2.0, 1.5),
+])
+def test_3d_zero_calibration_parity(pkg, O, w, h, nP, M, fwhm, S):
+    planes = planes_stack(4, w, h, nP, seed=40 + w + nP)
+    mont = montage(planes, np.arange(nP))
+    got, exp, cnt, ocnt, _ = run_both(pkg, O, mont, nP, M, fwhm, S, zero_calib(nP))
+    assert cnt == ocnt == [4, 4]
+    assert_parity(got, exp, f"3D {w}x{h}x{nP} M{M} fwhm{fwhm} S{S}")
+
+
+def test_3d_calibrated_montage_parity(pkg, O):
+    """Rotation, sub-pixel shifts, permuted ROI cells, intensity scaling, background, and a
+    montage whose width and height are not multiples of 3."""
+    w, h, nP, M = 16, 15, 6, 2
+    planes = planes_stack(4, w, h, nP, seed=71)
+    rng = np.random.default_rng(5)
+    rois = rng.permutation(9)[:nP].astype(np.float64)
+    mont = montage(planes, rois, W=3 * w + 1, H=3 * h + 2)
+    calib = (rng.uniform(-0.7, 0.7, nP), rng.uniform(-0.7, 0.7, nP), rng.uniform(-4, 4, nP),
+             rois, rng.uniform(0.8, 1.25, nP), 90.0)
+    got, exp, _, _, ex = run_both(pkg, O, mont, nP, M, 1.5, 2, calib, extra=True)
+    assert_parity(got, exp, "3D calibrated")
+    ga, oa = ex["aligned"]
+    np.testing.assert_allclose(ga.reshape(-1), oa.reshape(-1), rtol=1e-5,
+                               atol=1e-5 * np.abs(oa).max())
+    gg, og = ex["grad"]
+    scale = np.abs(og).max()
+    assert np.abs(gg.reshape(-1) - og.reshape(-1)).max() <= 1e-5 * scale
+    gm, om = ex["mp"]
+    np.testing.assert_allclose(gm.reshape(-1), om.reshape(-1), rtol=1e-4)
+
+
+def test_3d_no_intensity_weighting_no_mp(pkg, O):
+    w, h, nP, M = 14, 14, 4, 3
+    planes = planes_stack(5, w, h, nP, seed=12)
+    mont = montage(planes, np.arange(nP))
+    got, exp, _, _, _ = run_both(pkg, O, mont, nP, M, 1.5, 1, zero_calib(nP), intW=False,
+                                 mp=False)
+    assert_parity(got, exp, "3D intW off, MP off")
+
+
+def test_3d_uneven_loads_and_counters(pkg, O):
+    w, h, nP, M = 12, 12, 3, 2
+    planes = planes_stack(11, w, h, nP, seed=13)
+    mont = montage(planes, np.arange(nP))
+    loads = [(0, 4), (4, 8), (8, 11)]
+    got, exp, cnt, ocnt, _ = run_both(pkg, O, mont, nP, M, 1.5, 2, zero_calib(nP), loads=loads,
+                                      nOnGPU=4)
+    assert cnt == ocnt == [11, 3]
+    assert_parity(got, exp, "3D loads 4+4+3")
+
+
+def test_3d_drift(pkg, O):
+    w, h, nP, M = 14, 12, 4, 2
+    planes = planes_stack(6, w, h, nP, seed=14)
+    mont = montage(planes, np.arange(nP))
+    rng = np.random.default_rng(8)
+    drift = (rng.uniform(-0.3, 0.3, 6).astype(np.float32),
+             rng.uniform(-0.3, 0.3, 6).astype(np.float32))
+    got, exp, _, _, _ = run_both(pkg, O, mont, nP, M, 1.5, 2, zero_calib(nP), drift=drift)
+    assert_parity(got, exp, "3D drift")
+
+
+def test_3d_single_plane_is_rejected(pkg):
+    from nanoj_esrrf_amd import _capi
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    with pytest.raises(_capi.EsrrfError):
+        e.initialise(48, 48, 2, 1.5, 1, 2, 2, 20000, e.allCLdevices[0], True, True, False,
+                     calib_dict(*zero_calib(1)), 0, 1)

@@ -140,3 +140,34 @@ def test_3d_oracle_matches_reference_kernels(O, name):
     ref, ref_mp, _ = refrun.run(name, planes, st3d=refrun.st3d_of(z, z, z))
     assert_close(exp, ref, f"oracle vs reference {name}")
     np.testing.assert_allclose(o.mp, ref_mp, rtol=1e-4)
+
+
+@pytest.mark.parametrize("calibrated", [False, True])
+@pytest.mark.parametrize("name", D3)
+def test_3d_engine_matches_reference_kernels(pkg, name, calibrated):
+    cfg = refrun.configs()[name]
+    _, w, h, nP, M, fwhm, S, nF, iw = cfg
+    planes = stack(cfg, seed=19).astype(np.float32)
+    from nanoj_esrrf_amd import synth
+    mont = synth.montage3x3(planes)
+    rng = np.random.default_rng(4)
+    if calibrated:
+        sx, sy, th = rng.uniform(-0.6, 0.6, nP), rng.uniform(-0.6, 0.6, nP), rng.uniform(-3, 3, nP)
+    else:
+        sx = sy = th = np.zeros(nP)
+    ref, ref_mp, _ = refrun.run(name, planes, st3d=refrun.st3d_of(th, sx, sy))
+    calib = {"X-shift (pixels)": sx, "Y-shift (pixels)": sy, "Theta (degrees)": th,
+             "ROI #": np.arange(nP, dtype=np.float64), "Intensity scaling": np.ones(nP),
+             "Background level (ADC)": [0.0]}
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(3 * w, 3 * h, M, fwhm, S, nF, nF, 20000, e.allCLdevices[0], bool(iw), True,
+                 False, calib, 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(nF), np.zeros(nF))
+    e.calculateSRRF(mont)
+    got = e.readSRRFbuffer()
+    mp = e.readMPmaps()
+    e.release()
+    assert_close(got, ref, f"HIP vs reference {name} calibrated={calibrated}")
+    np.testing.assert_allclose(mp.reshape(-1), ref_mp, rtol=1e-4)
