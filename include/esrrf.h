@@ -130,6 +130,52 @@ int esrrf_plan_rolling(int n_slices, int n_frame_for_srrf_user, int frame_gap_us
  * (gui/LiveSRRF_optimised_.java:281,311-321). */
 int esrrf_plan_load(const int *plan6, int r, int l, int *first_index, int *n_frames);
 
+/* Autotuner of the tiled 2-D kernel (no counterpart in the reference, whose launch shape is
+ * the GUI's blockLength, LiveSRRF_CL.java:699-735): times every supported launch shape
+ * (waves per block x tap tables in LDS or not) over the frames of the current load, `reps`
+ * launches each, into scratch outputs, and keeps the fastest for this engine.  Reports it. */
+int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_lds, float *ms);
+/* ---- Parameter sweep (gui/ParametersSweep_.java:328-519) --------------------------------
+ * Every (magnification, frame count, sensitivity, fwhm) configuration of a sweep over the
+ * first nf frames of one stack, 2-D BIC, no drift; for each the four channels that
+ * readSRRFbuffer returns after that configuration's initialise/calculateSRRF/readSRRFbuffer
+ * (LiveSRRF_CL.java:159,636,749), computed from one upload, one gradient pass, one RGC
+ * launch per (magnification, fwhm) and one accumulation launch for all frame counts and
+ * sensitivities.  The reference loops nf -> S -> fwhm and rebuilds its program per
+ * configuration (:328-380, :452). */
+typedef struct esrrf_sweep esrrf_sweep;
+typedef struct esrrf_sweep_params {
+    int width, height;                  /* raw frame size                                */
+    const int *magnifications;          /* 1..8 each                                      */
+    int n_magnifications;
+    const float *fwhm;                  /* the GUI's "Radius" values (:720-723)           */
+    int n_fwhm;
+    const float *sensitivity;           /* (:725-728), at most 32 values                  */
+    int n_sensitivity;
+    const int *n_frames;                /* frames per SR image, first n frames (:730-747), at most 8 */
+    int n_n_frames;
+    int int_weighting;                  /* the reference sweep passes 1 (:452)            */
+    int do_mp_map_correction;           /* the reference sweep passes 1 (:452)            */
+    int autotune;                       /* 1: time the RGC launch shapes per magnification */
+    int use_graph;                      /* 1: capture the device sequence in a hipGraph   */
+    int device;                         /* -1: fastest                                    */
+} esrrf_sweep_params;
+int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out);
+int esrrf_sweep_destroy(esrrf_sweep *s);
+int esrrf_sweep_set_stream(esrrf_sweep *s, void *hip_stream);
+/* Run every configuration on the first max(n_frames) frames of a host stack (blocking). */
+int esrrf_sweep_run(esrrf_sweep *s, const void *frames, int dtype, int n_frames);
+/* Same from device memory, asynchronous on the sweep's stream. */
+int esrrf_sweep_run_device(esrrf_sweep *s, const void *d_frames, int dtype, int n_frames);
+/* One configuration's [4][hM][wM] result (AVG, VAR, TAC2, INT; NaN -> 0), indices into the
+ * parameter lists. */
+int esrrf_sweep_read(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm, float *out);
+int esrrf_sweep_read_device(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm,
+                            float *d_out);
+/* Launch shape used for a magnification (autotuned or default). */
+int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int *nw, int *ppw, int *taps_in_lds, float *ms);
+const char *esrrf_sweep_last_error(esrrf_sweep *s);
+
 const char *esrrf_last_error(esrrf_engine *e);
 const char *esrrf_version(void);
 

@@ -28,7 +28,10 @@ EXPORTS = [
     "esrrf_read_frame_counters", "esrrf_get_dims", "esrrf_finish", "esrrf_set_stream",
     "esrrf_profile_enable", "esrrf_profile_read", "esrrf_bic_constants",
     "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
-    "esrrf_last_error", "esrrf_version",
+    "esrrf_last_error", "esrrf_version", "esrrf_autotune",
+    "esrrf_sweep_create", "esrrf_sweep_destroy", "esrrf_sweep_set_stream", "esrrf_sweep_run",
+    "esrrf_sweep_run_device", "esrrf_sweep_read", "esrrf_sweep_read_device",
+    "esrrf_sweep_shape", "esrrf_sweep_last_error",
 ]
 
 
@@ -53,6 +56,18 @@ class Params(ctypes.Structure):
         ("int_coeffs_3d", ctypes.POINTER(ctypes.c_double)),
         ("background_3d", ctypes.c_float), ("axial_offset", ctypes.c_float),
         ("pixel_size", ctypes.c_float),
+    ]
+
+
+class SweepParams(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int),
+        ("magnifications", ctypes.POINTER(ctypes.c_int)), ("n_magnifications", ctypes.c_int),
+        ("fwhm", ctypes.POINTER(ctypes.c_float)), ("n_fwhm", ctypes.c_int),
+        ("sensitivity", ctypes.POINTER(ctypes.c_float)), ("n_sensitivity", ctypes.c_int),
+        ("n_frames", ctypes.POINTER(ctypes.c_int)), ("n_n_frames", ctypes.c_int),
+        ("int_weighting", ctypes.c_int), ("do_mp_map_correction", ctypes.c_int),
+        ("autotune", ctypes.c_int), ("use_graph", ctypes.c_int), ("device", ctypes.c_int),
     ]
 
 
@@ -98,11 +113,22 @@ def lib():
         "esrrf_plan_load": [ip, c_int, c_int, ip, ip],
         "esrrf_last_error": [vp],
         "esrrf_version": [],
+        "esrrf_autotune": [vp, c_int, ip, ip, ip, fp],
+        "esrrf_sweep_create": [ctypes.POINTER(SweepParams), ctypes.POINTER(vp)],
+        "esrrf_sweep_destroy": [vp],
+        "esrrf_sweep_set_stream": [vp, vp],
+        "esrrf_sweep_run": [vp, vp, c_int, c_int],
+        "esrrf_sweep_run_device": [vp, vp, c_int, c_int],
+        "esrrf_sweep_read": [vp, c_int, c_int, c_int, c_int, fp],
+        "esrrf_sweep_read_device": [vp, c_int, c_int, c_int, c_int, vp],
+        "esrrf_sweep_shape": [vp, c_int, ip, ip, ip, fp],
+        "esrrf_sweep_last_error": [vp],
     }
+    strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error")
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
-        f.restype = ctypes.c_char_p if name in ("esrrf_last_error", "esrrf_version") else c_int
+        f.restype = ctypes.c_char_p if name in strings else c_int
     _LIB = L
     return L
 

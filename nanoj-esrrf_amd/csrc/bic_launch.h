@@ -1,0 +1,57 @@
+// bic_launch.h -- device tap tables, launch arguments, the default launch shape and the
+// autotuner of the tiled BIC RGC kernel; shared by the engine, the parameter sweep and
+// the RSE adjoint.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "esrrf_internal.h"
+#include "geometry.h"
+
+namespace esrrf {
+
+// Launch shape of k_bic_rgc: nw waves per block, ppw sub-pixel phases per wave,
+// tap tables staged in LDS or read from the global tables.
+struct BicShape {
+    int nw = 1, ppw = 1;
+    int taps_in_lds = 0;
+    float ms = 0.0f;      // measured time of the autotuner's sample launch (0: not tuned)
+};
+
+// Geometry of one (w, h, M, fwhm) plus its device copy.
+struct BicTables {
+    BicGeometry geo;
+    Tap *d_taps = nullptr;
+    Phase *d_phases = nullptr;
+    int *d_tapk = nullptr;
+    float *d_tapWd = nullptr;
+    float radius = 0, tSS = 0, tSO = 0;
+    bool ok = false;      // geometry fits the tiled kernel
+};
+
+// Build the tables for frames of w x h, magnification M and the constants of fwhm
+// (LiveSRRF_CL.java:304-329) and upload them (synchronous copies: not for capture).
+hipError_t bic_tables_create(int w, int h, int M, float fwhm, BicTables &t);
+void bic_tables_free(BicTables &t);
+
+// Default shape: waves per block a multiple of 4 (a block's waves are dealt round-robin
+// to the CU's four SIMDs), about 4 phases per wave, at most 16 waves and 8 phases per wave;
+// tap tables in LDS when a class's tables stay within 48 KiB (adj: with dW/dtSS).
+BicShape bic_default_shape(const BicTables &t, int M, bool adj = false);
+// All shapes the kernel supports for M (the autotuner's candidates).
+std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj = false);
+
+// Arguments common to every launch over unshifted frames (out, prev, frames, cumulant
+// constants and mode are filled by the caller).
+BicRgcArgs bic_args(const BicTables &t, const BicShape &sh, const float2 *U, const float *px,
+                    int w, int h, int M);
+
+// Time every candidate shape on `a` (reps launches each after one warm-up) and return the
+// fastest; the launches write whatever `a` points at, so callers pass scratch outputs.
+hipError_t bic_autotune(const BicTables &t, BicRgcArgs a, int M, int reps, hipStream_t s,
+                        BicShape *best, std::vector<BicShape> *all);
+
+int sens_exact_int(float s);
+
+}  // namespace esrrf

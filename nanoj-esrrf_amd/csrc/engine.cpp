@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/esrrf.h"
+#include "bic_launch.h"
 #include "esrrf_internal.h"
 #include "geometry.h"
 #include "modes.h"
@@ -64,12 +65,10 @@ struct esrrf_engine {
     float *d_tmp = nullptr;      // read-back staging [4][nvox]
     float *d_aux = nullptr;      // mode scratch (FHT spectra, 3-D gradients)
     size_t d_aux_bytes = 0;
-    Tap *d_taps = nullptr;
-    Phase *d_phases = nullptr;
-    int *d_tapk = nullptr;
-    BicGeometry geo;
+    BicTables tabs;              // tap tables of the tiled BIC kernel
+    BicShape shape;              // its launch shape (default or autotuned)
     bool tiled = false;          // tiled BIC kernel usable
-    int ppw = 1, nw = 1, stage = 0, taps_in_lds = 0;
+    int stage = 0;
     std::vector<float> h_drift;  // host copy of the drift buffer (x M)
     ModeState ms;                // FHT / 3-D state
     // bookkeeping
@@ -116,10 +115,11 @@ int dalloc(esrrf_engine *e, T **ptr, size_t bytes)
 void free_all(esrrf_engine *e)
 {
     void *ptrs[] = {e->d_in, e->d_px, e->d_U, e->d_out, e->d_prev, e->d_mp, e->d_drift,
-                    e->d_tmp, e->d_aux, e->d_taps, e->d_phases, e->d_tapk};
+                    e->d_tmp, e->d_aux};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_mode_state(e->ms);
+    bic_tables_free(e->tabs);
     for (auto &s : e->prof)
         for (auto &ev : s.pending) {
             (void)hipEventDestroy(ev.first);
@@ -128,12 +128,6 @@ void free_all(esrrf_engine *e)
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
 }
 
-int sens_exact_int(float s)
-{
-    for (int k = 1; k <= 4; k++)
-        if (s == (float)k) return k;
-    return 0;
-}
 
 // Record an event pair around a launch when profiling (HIP events on the launch stream).
 struct ProfScope {
@@ -190,19 +184,9 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
     const float rn = 1.0f / (float)N;
     const float rn1 = 1.0f / (float)(N - 1);
     if (e->tiled && drift_zero(e, c0, n)) {
-        BicRgcArgs a{};
-        a.U = e->d_U;
-        a.px = e->d_px;
-        a.taps = e->d_taps;
-        a.phases = e->d_phases;
-        a.tapk = e->d_tapk;
+        BicRgcArgs a = bic_args(e->tabs, e->shape, e->d_U, e->d_px, e->w, e->h, e->M);
         a.out = e->d_out;
         a.prev = e->d_prev;
-        a.w = e->w;
-        a.h = e->h;
-        a.M = e->M;
-        a.wM = e->wM;
-        a.hM = e->hM;
         a.frame0 = f0;
         a.nframes = n;
         a.first_init = (c0 == 0);
@@ -211,25 +195,9 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
         a.sens = e->p.sensitivity;
         a.sens_int = e->sens_int;
         a.int_weighting = e->p.int_weighting ? 1 : 0;
-        a.tac_prev_scaled = 1;
-        a.G0 = e->geo.G0;
-        a.PP = e->geo.PP;
-        a.PH = e->geo.PH;
-        a.TW = e->geo.TW;
-        a.TH = e->geo.TH;
-        a.R2lo = e->geo.R2lo;
-        a.R2hi = e->geo.R2hi;
-        a.xs_log2 = e->geo.xs_log2;
-        a.ncls_x = e->geo.ncls_x;
-        a.ncls_y = e->geo.ncls_y;
-        a.tiles_x = (e->w + 7) / 8;
-        a.tiles_y = (e->h + 7) / 8;
-        a.nw = e->nw;
         a.stage = e->stage;
-        a.taps_in_lds = e->taps_in_lds;
-        a.max_class_taps = e->geo.max_class_taps;
         ProfScope ps(e, 0);
-        HIPCHK(e, launch_bic_rgc(a, e->ppw, e->stream));
+        HIPCHK(e, launch_bic_rgc(a, e->shape.ppw, e->stream));
         return ESRRF_OK;
     }
     GenericRgcArgs g{};
@@ -510,40 +478,22 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
     if (e->mode == MODE_BIC) {
         if ((rc = dalloc(e, &e->d_px, sizeof(float) * npx * nF))) return bad(rc);
         if ((rc = dalloc(e, &e->d_U, sizeof(float2) * 4 * npx * nF))) return bad(rc);
-        if (e->M <= 8 && build_bic_geometry(e->w, e->h, e->M, e->radius, e->tSS, e->tSO, e->geo)) {
-            // Waves per block: a multiple of 4 (a block's waves are dealt round-robin to the
-            // CU's four SIMDs, so other counts overload one SIMD at every frame barrier),
-            // about 4 sub-pixel phases per wave, at most 16 waves.
+        if (bic_tables_create(e->w, e->h, e->M, p->fwhm, e->tabs) != hipSuccess)
+            return bad(fail(e, ESRRF_ERR_HIP, "tap table upload failed"));
+        e->tiled = e->tabs.ok;
+        if (e->tiled) {
+            e->shape = bic_default_shape(e->tabs, e->M);
             const int MM = e->M * e->M;
-            int nw = MM <= 4 ? MM : std::min(16, 4 * (((MM + 3) / 4 + 3) / 4));
             if (const char *ov = getenv("ESRRF_BIC_NW")) {
                 const int v = atoi(ov);
-                if (v >= 1 && v <= 16) nw = v;
+                if (v >= 1 && v <= 16 && (MM + v - 1) / v <= 8) {
+                    e->shape.nw = v;
+                    e->shape.ppw = (MM + v - 1) / v;
+                }
             }
-            if ((MM + nw - 1) / nw > 8) nw = std::min(16, (MM + 7) / 8);
-            const int ppw = (MM + nw - 1) / nw;
+            if (const char *ov = getenv("ESRRF_BIC_TLDS")) e->shape.taps_in_lds = atoi(ov) != 0;
             e->stage = 0;
             if (const char *ov = getenv("ESRRF_BIC_STAGE")) e->stage = atoi(ov) != 0;
-            e->ppw = ppw;
-            e->nw = nw;
-            e->tiled = true;
-            // tap tables in LDS when a block's share stays small enough for 2 blocks per CU
-            e->taps_in_lds = (size_t)(e->geo.max_class_taps + 4) * sizeof(Tap) <= 48 * 1024;
-            if (const char *ov = getenv("ESRRF_BIC_TLDS")) e->taps_in_lds = atoi(ov) != 0;
-            const size_t ntap = std::max<size_t>(e->geo.taps.size(), 1);
-            if ((rc = dalloc(e, &e->d_taps, sizeof(Tap) * ntap))) return bad(rc);
-            if ((rc = dalloc(e, &e->d_tapk, sizeof(int) * ntap))) return bad(rc);
-            if ((rc = dalloc(e, &e->d_phases, sizeof(Phase) * e->geo.phases.size()))) return bad(rc);
-            if (!e->geo.taps.empty()) {
-                if (hipMemcpy(e->d_taps, e->geo.taps.data(), sizeof(Tap) * e->geo.taps.size(),
-                              hipMemcpyHostToDevice) != hipSuccess ||
-                    hipMemcpy(e->d_tapk, e->geo.tapk.data(), sizeof(int) * e->geo.tapk.size(),
-                              hipMemcpyHostToDevice) != hipSuccess)
-                    return bad(fail(e, ESRRF_ERR_HIP, "table upload failed"));
-            }
-            if (hipMemcpy(e->d_phases, e->geo.phases.data(), sizeof(Phase) * e->geo.phases.size(),
-                          hipMemcpyHostToDevice) != hipSuccess)
-                return bad(fail(e, ESRRF_ERR_HIP, "table upload failed"));
         }
     } else {
         ModeCreate mc{};
@@ -783,6 +733,42 @@ int esrrf_active_samples(esrrf_engine *e, long long *per_frame)
         *per_frame = count_active_samples_bic(e->w, e->h, e->M, e->radius, e->tSO);
     else
         *per_frame = mode_active_samples(e->mode == MODE_FHT ? 1 : 2, e->ms);
+    return ESRRF_OK;
+}
+
+int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_lds, float *ms)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->mode != MODE_BIC || !e->tiled)
+        return fail(e, ESRRF_ERR_STATE, "autotuning applies to the tiled 2-D BIC kernel only");
+    if (e->loaded == 0) return fail(e, ESRRF_ERR_STATE, "no frames loaded");
+    (void)hipSetDevice(e->dev);
+    // candidates run over the whole current load into scratch outputs: the engine's
+    // cumulants and frame counters are left untouched
+    float *prev = nullptr;
+    int rc = dalloc(e, &prev, sizeof(float) * e->nvox);
+    if (rc) return rc;
+    BicRgcArgs a = bic_args(e->tabs, e->shape, e->d_U, e->d_px, e->w, e->h, e->M);
+    a.out = e->d_tmp;
+    a.prev = prev;
+    a.frame0 = 0;
+    a.nframes = e->loaded;
+    a.first_init = 1;
+    a.rn = 1.0f / (float)e->p.n_frame_for_srrf;
+    a.rn1 = 1.0f / (float)(e->p.n_frame_for_srrf - 1);
+    a.sens = e->p.sensitivity;
+    a.sens_int = e->sens_int;
+    a.int_weighting = e->p.int_weighting ? 1 : 0;
+    BicShape best;
+    hipError_t st = bic_autotune(e->tabs, a, e->M, std::max(reps, 1), e->stream, &best, nullptr);
+    (void)hipFree(prev);
+    if (st != hipSuccess) return fail(e, ESRRF_ERR_HIP, "autotune: %s", hipGetErrorString(st));
+    e->shape = best;
+    if (nw) *nw = best.nw;
+    if (ppw) *ppw = best.ppw;
+    if (taps_in_lds) *taps_in_lds = best.taps_in_lds;
+    if (ms) *ms = best.ms;
     return ESRRF_OK;
 }
 

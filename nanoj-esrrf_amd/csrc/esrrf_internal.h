@@ -26,7 +26,8 @@ struct Phase {
     float wx[4];    // Catmull-Rom weights along x for u0-1 .. u0+2
     float wy[4];
     int sx, sy;     // sub-pixel phase (xM mod M, yM mod M)
-    int pad[2];
+    float wsumd;    // sum of dW/dtSS over all taps (same order as wsum)
+    int pad1;
 };
 
 // Launch parameters of the BIC 2-D radial-gradient-convergence kernel.
@@ -55,6 +56,13 @@ struct BicRgcArgs {
     int stage;             // 1: stage the next frame's tiles in registers
     int taps_in_lds;       // 1: stage each block's tap tables in LDS
     int max_class_taps;    // largest tap-table size of one x/y class (LDS sizing)
+    // kernel variant: 0 = accumulate the four cumulants (liveSRRF.cl:363-393);
+    // 1 = emit the normalised CGLH (before the sensitivity power) and the intensity of
+    //     every pixel and frame, phase-major [frame][M*M][h][w], into emitC / emitV;
+    // 2 = accumulate AVG, dAVG/dtSS, dAVG/dS and INT (out channels 0..3) for the RSE adjoint
+    int mode;
+    float *emitC, *emitV;
+    const float *tapWd;    // per tap dW/dtSS (mode 2)
 };
 
 struct GenericRgcArgs {

@@ -84,6 +84,7 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
     const int planesz = g.PP * g.PH;
     g.taps.clear();
     g.tapk.clear();
+    g.tapWd.clear();
     g.phases.assign((size_t)g.ncls_x * g.ncls_y * MM, Phase{});
     g.max_off = 0;
     for (int cy = 0; cy < g.ncls_y; cy++) {
@@ -98,7 +99,7 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
                     const float yc = (float)((yM + 0.5) / M);
                     const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc);
                     P.first_tap = (int)g.taps.size();
-                    float wsum = 0.0f;
+                    float wsum = 0.0f, wsumd = 0.0f;
                     for (int j = g.R2lo; j <= g.R2hi; j++) {
                         const float vy = ((float)by + (float)j) / 2.0f;
                         for (int i = g.R2lo; i <= g.R2hi; i++) {
@@ -130,6 +131,11 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
                             if (t.off > g.max_off) g.max_off = t.off;
                             g.taps.push_back(t);
                             g.tapk.push_back((kxr + 128) | ((kyr + 128) << 8));
+                            // dW/dtSS of W = d^4 exp(-4 d^2 / tSS)
+                            const float Wd = (float)((double)(float)W * 4.0 * (double)d * d /
+                                                     ((double)tSS * tSS));
+                            g.tapWd.push_back(Wd);
+                            wsumd = wsumd + Wd;
                         }
                     }
                     // pad to a multiple of 4 with zero-weight taps (the kernel walks the
@@ -142,8 +148,10 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
                         z.W = 0.0f;
                         g.taps.push_back(z);
                         g.tapk.push_back(128 | (128 << 8));
+                        g.tapWd.push_back(0.0f);
                     }
                     P.ntaps = (int)g.taps.size() - P.first_tap;
+                    P.wsumd = wsumd;
                     P.sx = sx;
                     P.sy = sy;
                     P.wsum = wsum;
@@ -172,6 +180,7 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
         z.a = z.b = z.W = 0.0f;
         g.taps.push_back(z);
         g.tapk.push_back(128 | (128 << 8));
+        g.tapWd.push_back(0.0f);
     }
     // every lane (lx, ly in [0, 8)) must stay inside the 4-plane LDS tile
     if (g.max_off + 7 * g.PP + 7 >= 4 * g.PP * g.PH) return false;

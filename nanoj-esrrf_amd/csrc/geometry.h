@@ -18,6 +18,7 @@ struct BicGeometry {
     std::vector<Tap> taps;
     std::vector<Phase> phases;
     std::vector<int> tapk;
+    std::vector<float> tapWd;   // per tap dW/dtSS = W * 4 d^2 / tSS^2 (0 for padding)
 };
 
 bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO, BicGeometry &g);

@@ -196,21 +196,41 @@ __device__ __forceinline__ float border_intensity(const float *lR, int X0, int Y
     return interp_tile(lR, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
 }
 
-// One RGC sample: acc += W * med3(-1, -|s|, -c - 1), i.e. W * ((1 - |sin|)[cos < 0] - 1)
-// with c = U.(a,b) and s = U x (a,b) (liveSRRF.cl:296-341 with G normalised).
-__device__ __forceinline__ float tap_acc(float2 u, const Tap &tp, float acc)
+// One RGC sample: h = med3(-1, -|s|, -c - 1) = (1 - |sin|)[cos < 0] - 1 with c = U.(a,b)
+// and s = U x (a,b) (liveSRRF.cl:296-341 with G normalised); acc += W * h.
+__device__ __forceinline__ float tap_h(float2 u, const Tap &tp)
 {
     float c2 = fmaf(u.x, -tp.a, -1.0f);
     c2 = fmaf(u.y, -tp.b, c2);
     float s = u.y * tp.a;
     s = fmaf(u.x, -tp.b, s);
-    const float hh = __builtin_amdgcn_fmed3f(-1.0f, -fabsf(s), c2);
-    return fmaf(tp.W, hh, acc);
+    return __builtin_amdgcn_fmed3f(-1.0f, -fabsf(s), c2);
+}
+
+__device__ __forceinline__ float tap_acc(float2 u, const Tap &tp, float acc)
+{
+    return fmaf(tp.W, tap_h(u, tp), acc);
+}
+
+// RSE adjoint: t = v C^S (C^S without intensity weighting); returns dt/dC, dt/dS.
+__device__ __forceinline__ void sens_derivs(float C, float sens, int sens_int, float v,
+                                            bool intw, float &t, float &dtdC, float &dtdS)
+{
+    if (!(C >= 0.0f)) {
+        t = dtdC = dtdS = 0.0f;
+        return;
+    }
+    const float Cs = apply_sens(C, sens, sens_int);
+    const float Cs1 = sens_int == 1 ? 1.0f : (sens_int == 2 ? C : powf(C, sens - 1.0f));
+    const float vv = intw ? v : 1.0f;
+    t = vv * Cs;
+    dtdC = vv * sens * Cs1;
+    dtdS = C > 0.0f ? t * logf(C) : 0.0f;
 }
 
 // PPW: sub-pixel phases per wave; STAGE_U: U-tile elements each thread stages in
 // registers for the next frame (0: no staging, tiles are filled after the compute).
-template <int PPW, int STAGE_U, bool TLDS>
+template <int PPW, int STAGE_U, bool TLDS, int MODE>
 __global__ void __launch_bounds__(1024)
 __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArgs a,
                                                  const Tap *__restrict__ taps,
@@ -243,34 +263,41 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
 
     // per-phase state of this lane: AVG, E[x^2], TAC2, INT, Prev, 1/sum(w), bicubic base
     // (separate small arrays: each stays in VGPRs under uniform dynamic indexing)
+    // (MODE 2 reuses them as AVG, dAVG/dtSS, dAVG/dS, INT)
     float s_avg[PPW], s_e2[PPW], s_tac[PPW], s_int[PPW], s_prv[PPW];
     float invw[PPW];
+    float rwd[PPW];   // MODE 2: sum of gated dW/dtSS over sum of gated W
     int ibase[PPW];   // LDS index of the lane's bicubic support, -1: bilinear border path
 #pragma unroll
     for (int p = 0; p < PPW; p++) {
         const int ph = wave + p * a.nw;
         invw[p] = 0.0f;
+        rwd[p] = 0.0f;
         ibase[p] = -1;
         s_avg[p] = s_e2[p] = s_tac[p] = s_int[p] = s_prv[p] = 0.0f;
         if (ph < MM) {
             const Phase P = PHS[ph];
-            float ws = P.wsum;
+            float ws = P.wsum, wsd = P.wsumd;
             if (!interior) {
                 // border: sum only the taps inside the 0 < v < size gates, reference order
                 ws = 0.0f;
+                wsd = 0.0f;
                 for (int t = 0; t < P.ntaps; t++) {
                     const int k = tapk[P.first_tap + t];
                     const int kx = 2 * X + (k & 255) - 128;
                     const int ky = 2 * Y + ((k >> 8) & 255) - 128;
-                    if (kx >= 1 && kx <= 2 * w - 1 && ky >= 1 && ky <= 2 * h - 1)
+                    if (kx >= 1 && kx <= 2 * w - 1 && ky >= 1 && ky <= 2 * h - 1) {
                         ws += taps[P.first_tap + t].W;
+                        if (MODE == 2) wsd += a.tapWd[P.first_tap + t];
+                    }
                 }
             }
             invw[p] = 1.0f / ws;
+            if (MODE == 2) rwd[p] = wsd * invw[p];
             const int u0 = X + P.du, v0 = Y + P.dv;
             if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2)
                 ibase[p] = (v0 - 1 - (Y0 - 2)) * RAW_P + (u0 - 1 - (X0 - 2));
-            if (!a.first_init && inimg) {
+            if (MODE != 1 && !a.first_init && inimg) {
                 const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
                 s_avg[p] = a.out[pix];
                 s_e2[p] = a.out[pix + whM];
@@ -349,10 +376,13 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
     // are staged once in LDS and read with uniform-address ds_read_b128 in the tap loop,
     // so the loop's loads are all in-order LDS traffic
     Tap *lT = reinterpret_cast<Tap *>(lR + NBUF * RAW_SZ);
+    float *lWd = reinterpret_cast<float *>(lT + a.max_class_taps + 4);   // MODE 2
     const int tcls0 = PHS[0].first_tap;
     if (TLDS) {
         const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
         for (int e = tid; e < tcls1 - tcls0 + 4; e += nthr) lT[e] = taps[tcls0 + e];
+        if (MODE == 2)
+            for (int e = tid; e < tcls1 - tcls0 + 4; e += nthr) lWd[e] = a.tapWd[tcls0 + e];
     }
     fill_direct(a.frame0, lU, lR);
     __syncthreads();
@@ -389,8 +419,29 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
                 // Tap lists are padded to a multiple of 4 with W = 0 taps and the table
                 // ends with one spare chunk, so the constants of chunk c+1 can always be
                 // fetched (scalar cache) while the LDS reads of chunk c are in flight.
-                float acc = 0.0f;
-                if (TLDS) {
+                float acc = 0.0f, acc2 = 0.0f;
+                if (MODE == 2) {
+                    // adjoint: acc2 = sum dW/dtSS * h beside acc (taps and dW/dtSS from LDS
+                    // when staged there, else from the global tables)
+                    const Tap *Tl = TLDS ? lT + (P.first_tap - tcls0) : TAPf + P.first_tap;
+                    const float *Wl = TLDS ? lWd + (P.first_tap - tcls0) : a.tapWd + P.first_tap;
+                    for (int t = 0; t < P.ntaps; t += 4) {
+                        const Tap c0 = Tl[t], c1 = Tl[t + 1], c2 = Tl[t + 2], c3 = Tl[t + 3];
+                        const float4 wd = *reinterpret_cast<const float4 *>(Wl + t);
+                        const float h0 = tap_h(cU[base + c0.off], c0);
+                        const float h1 = tap_h(cU[base + c1.off], c1);
+                        const float h2 = tap_h(cU[base + c2.off], c2);
+                        const float h3 = tap_h(cU[base + c3.off], c3);
+                        acc = fmaf(c0.W, h0, acc);
+                        acc2 = fmaf(wd.x, h0, acc2);
+                        acc = fmaf(c1.W, h1, acc);
+                        acc2 = fmaf(wd.y, h1, acc2);
+                        acc = fmaf(c2.W, h2, acc);
+                        acc2 = fmaf(wd.z, h2, acc2);
+                        acc = fmaf(c3.W, h3, acc);
+                        acc2 = fmaf(wd.w, h3, acc2);
+                    }
+                } else if (TLDS) {
                     const Tap *Tl = lT + (P.first_tap - tcls0);
                     for (int t = 0; t < P.ntaps; t += 4) {
                         const Tap c0 = Tl[t], c1 = Tl[t + 1], c2 = Tl[t + 2], c3 = Tl[t + 3];
@@ -424,8 +475,8 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
                         c3 = n3;
                     }
                 }
-                float C = (P.wsum + acc) * invw[p];
-                C = apply_sens(C, a.sens, a.sens_int);
+                const float C0 = (P.wsum + acc) * invw[p];
+                const float C = MODE == 0 ? apply_sens(C0, a.sens, a.sens_int) : C0;
 
                 // interpolated intensity at (xM/M - 0.5, yM/M - 0.5) (liveSRRF.cl:361)
                 float v;
@@ -447,6 +498,32 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
                                          (Y + zf) * M + P.sy);
                 }
 
+                if (MODE == 1) {
+                    if (inimg) {
+                        const long e = (((long)(a.frame0 + fi) * MM + ph) * h + Y) * w + X;
+                        a.emitC[e] = C0;
+                        a.emitV[e] = v;
+                    }
+                    continue;
+                }
+                if (MODE == 2) {
+                    // dC/dtSS of C = (wsum + acc) / sum(W) with sum(W) over the gated taps
+                    const float Cd = (P.wsumd + acc2) * invw[p] - C0 * rwd[p];
+                    float t, dtdC, dtdS;
+                    sens_derivs(C0, a.sens, a.sens_int, v, intw, t, dtdC, dtdS);
+                    if (first) {
+                        s_avg[p] = t * rn;
+                        s_e2[p] = (dtdC * Cd) * rn;
+                        s_tac[p] = dtdS * rn;
+                        s_int[p] = v * rn;
+                    } else {
+                        s_avg[p] = s_avg[p] + t * rn;
+                        s_e2[p] = s_e2[p] + (dtdC * Cd) * rn;
+                        s_tac[p] = s_tac[p] + dtdS * rn;
+                        s_int[p] = s_int[p] + v * rn;
+                    }
+                    continue;
+                }
                 // temporal accumulation (liveSRRF.cl:363-393)
                 const float t = intw ? v * C : C;
                 const float e2 = intw ? (t * v) * C : C * C;
@@ -485,7 +562,7 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
         }
     }
 
-    if (inimg) {
+    if (MODE != 1 && inimg) {
 #pragma unroll
         for (int p = 0; p < PPW; p++) {
             const int ph = wave + p * a.nw;
@@ -496,16 +573,18 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
             a.out[pix + whM] = s_e2[p];
             a.out[pix + 2 * whM] = s_tac[p];
             a.out[pix + 3 * whM] = s_int[p];
-            a.prev[pix] = s_prv[p];
+            if (MODE == 0) a.prev[pix] = s_prv[p];
         }
     }
 }
 
-template <int PPW, bool TLDS>
-static hipError_t launch_ppw_t(const BicRgcArgs &a, hipStream_t s)
+template <int PPW, bool TLDS, int MODE>
+static hipError_t launch_ppw_m(const BicRgcArgs &a, hipStream_t s)
 {
     const size_t one = sizeof(float2) * 4 * a.PP * a.PH + sizeof(float) * RAW_SZ;
-    const size_t tl = TLDS ? sizeof(Tap) * ((size_t)a.max_class_taps + 4) : 0;
+    const size_t tl = TLDS ? (sizeof(Tap) + (MODE == 2 ? sizeof(float) : 0)) *
+                                 ((size_t)a.max_class_taps + 4)
+                           : 0;
     const int nthr = 64 * a.nw;
     const int nU = 4 * a.PH * (a.TW >> 1);
     const int need = (nU + nthr - 1) / nthr;
@@ -513,15 +592,25 @@ static hipError_t launch_ppw_t(const BicRgcArgs &a, hipStream_t s)
     // register staging of the next frame's tiles costs VGPRs; where it would push the
     // kernel below its occupancy target the tiles are filled directly after the compute
     if (a.stage && nthr >= RAW_T * RAW_T && need <= 2)
-        hipLaunchKernelGGL((k_bic_rgc<PPW, 2, TLDS>), grid, block, 2 * one + tl, s, a, a.taps,
-                           a.phases, a.tapk);
+        hipLaunchKernelGGL((k_bic_rgc<PPW, 2, TLDS, MODE>), grid, block, 2 * one + tl, s, a,
+                           a.taps, a.phases, a.tapk);
     else if (a.stage && nthr >= RAW_T * RAW_T && need <= 4)
-        hipLaunchKernelGGL((k_bic_rgc<PPW, 4, TLDS>), grid, block, 2 * one + tl, s, a, a.taps,
-                           a.phases, a.tapk);
+        hipLaunchKernelGGL((k_bic_rgc<PPW, 4, TLDS, MODE>), grid, block, 2 * one + tl, s, a,
+                           a.taps, a.phases, a.tapk);
     else
-        hipLaunchKernelGGL((k_bic_rgc<PPW, 0, TLDS>), grid, block, one + tl, s, a, a.taps,
-                           a.phases, a.tapk);
+        hipLaunchKernelGGL((k_bic_rgc<PPW, 0, TLDS, MODE>), grid, block, one + tl, s, a,
+                           a.taps, a.phases, a.tapk);
     return hipGetLastError();
+}
+
+template <int PPW, bool TLDS>
+static hipError_t launch_ppw_t(const BicRgcArgs &a, hipStream_t s)
+{
+    switch (a.mode) {
+    case 1: return launch_ppw_m<PPW, TLDS, 1>(a, s);
+    case 2: return launch_ppw_m<PPW, TLDS, 2>(a, s);
+    default: return launch_ppw_m<PPW, TLDS, 0>(a, s);
+    }
 }
 
 template <int PPW>

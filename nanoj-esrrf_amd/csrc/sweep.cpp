@@ -1,0 +1,413 @@
+// sweep.cpp -- the parameter sweep of gui/ParametersSweep_.java:328-519 as one fused device
+// sequence.
+//
+// The reference re-initialises LiveSRRF_CL (a program rebuild) for every
+// (frame count, sensitivity, fwhm) and pushes the first nf frames one at a time
+// (:452-458, :800-844).  Here:
+//   * the frames go to the device once and their unit tap gradients are computed once
+//     (they depend on neither magnification, fwhm nor sensitivity);
+//   * per (magnification, fwhm) one RGC launch over the largest frame count emits the
+//     normalised CGLH and intensity of every pixel and frame;
+//   * one accumulation launch turns them into the cumulants of every (frame count,
+//     sensitivity) configuration;
+//   * VAR and MP correction finish each configuration (liveSRRF.cl:427-505).
+// The whole device sequence can be captured once in a hipGraph and replayed, and the RGC
+// launch shape of each magnification can be picked by the autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/esrrf.h"
+#include "bic_launch.h"
+#include "esrrf_internal.h"
+#include "sweep_internal.h"
+
+using namespace esrrf;
+
+struct esrrf_sweep {
+    std::mutex mu;
+    std::string err;
+    int dev = 0;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    int w = 0, h = 0;
+    std::vector<int> Ms;
+    std::vector<float> fwhm, sens;
+    std::vector<int> nfs;
+    int nfmax = 0;
+    int int_weighting = 1, do_mp = 1, autotune = 0, use_graph = 0;
+    // device data
+    void *d_in = nullptr;          // frames as handed over (raw dtype), nfmax frames
+    float *d_px = nullptr;         // [nfmax][h][w]
+    float2 *d_U = nullptr;         // [nfmax][2h][2w]
+    float *d_C = nullptr, *d_V = nullptr;   // emitted CGLH / intensity, largest M
+    float *d_dummy = nullptr;      // out/prev of the emit launches (never written)
+    std::vector<std::vector<BicTables>> tabs;   // [iM][iF]
+    std::vector<BicShape> shape;   // [iM]
+    std::vector<float *> d_out;    // [iM] configurations [nf][S][fwhm][4][hM][wM]
+    std::vector<float *> d_mp;     // [iM] MP maps [nf][S][fwhm][3][M][M]
+    float *d_tmp = nullptr;        // read-back staging
+    size_t tmp_floats = 0;
+    // graph of the device sequence (valid for one dtype)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_dtype = -1;
+    bool tuned = false;
+    bool have_result = false;
+};
+
+namespace {
+
+int sfail(esrrf_sweep *s, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (s) s->err = buf;
+    return code;
+}
+
+#define SCHK(s, x)                                                                         \
+    do {                                                                                   \
+        hipError_t _st = (x);                                                              \
+        if (_st != hipSuccess)                                                             \
+            return sfail((s), ESRRF_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_st)); \
+    } while (0)
+
+void sweep_free(esrrf_sweep *s)
+{
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    void *ptrs[] = {s->d_in, s->d_px, s->d_U, s->d_C, s->d_V, s->d_dummy, s->d_tmp};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &v : s->tabs)
+        for (auto &t : v) bic_tables_free(t);
+    for (float *p : s->d_out)
+        if (p) (void)hipFree(p);
+    for (float *p : s->d_mp)
+        if (p) (void)hipFree(p);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+}
+
+size_t n_conf(const esrrf_sweep *s) { return s->nfs.size() * s->sens.size() * s->fwhm.size(); }
+
+// The device sequence of one run: frames in d_in -> every configuration's result.
+int enqueue_run(esrrf_sweep *s, int dtype)
+{
+    const hipStream_t st = s->stream;
+    const long wh = (long)s->w * s->h;
+    SCHK(s, launch_convert_frames(s->d_in, dtype, s->d_px, wh * s->nfmax, st));
+    SCHK(s, launch_bic_prep(s->d_px, s->d_U, s->w, s->h, s->nfmax, st));
+    const int nS = (int)s->sens.size(), nF = (int)s->fwhm.size(), nQ = (int)s->nfs.size();
+    for (size_t im = 0; im < s->Ms.size(); im++) {
+        const int M = s->Ms[im];
+        const long whM = wh * M * M;
+        for (int fi = 0; fi < nF; fi++) {
+            const BicTables &t = s->tabs[im][fi];
+            BicRgcArgs a = bic_args(t, s->shape[im], s->d_U, s->d_px, s->w, s->h, M);
+            a.mode = 1;
+            a.emitC = s->d_C;
+            a.emitV = s->d_V;
+            a.out = s->d_dummy;
+            a.prev = s->d_dummy;
+            a.frame0 = 0;
+            a.nframes = s->nfmax;
+            a.first_init = 1;
+            a.rn = 1.0f;
+            a.rn1 = 1.0f;
+            a.sens = 1.0f;
+            a.sens_int = 1;
+            a.int_weighting = s->int_weighting;
+            SCHK(s, launch_bic_rgc(a, s->shape[im].ppw, st));
+            SweepAccArgs c{};
+            c.C = s->d_C;
+            c.V = s->d_V;
+            c.out = s->d_out[im];
+            c.w = s->w;
+            c.h = s->h;
+            c.M = M;
+            c.nframes = s->nfmax;
+            c.nS = nS;
+            c.n_fwhm = nF;
+            c.i_fwhm = fi;
+            c.nq = nQ;
+            c.int_weighting = s->int_weighting;
+            for (int q = 0; q < nQ; q++) {
+                c.nf[q] = s->nfs[q];
+                c.rn[q] = 1.0f / (float)s->nfs[q];
+                c.rn1[q] = 1.0f / (float)(s->nfs[q] - 1);
+            }
+            for (int k = 0; k < nS; k++) {
+                c.sens[k] = s->sens[k];
+                c.sens_int[k] = sens_exact_int(s->sens[k]);
+            }
+            SCHK(s, launch_sweep_acc(c, st));
+            // VAR and MP correction of each configuration (LiveSRRF_CL.java:753-770)
+            for (int q = 0; q < nQ; q++)
+                for (int k = 0; k < nS; k++) {
+                    const long ci = ((long)q * nS + k) * nF + fi;
+                    float *out = s->d_out[im] + ci * 4 * whM;
+                    SCHK(s, launch_var(out, whM, st));
+                    if (s->do_mp) {
+                        float *mp = s->d_mp[im] + ci * 3 * M * M;
+                        SCHK(s, launch_mp_map(out, mp, s->w, s->h, 1, M, 2, st));
+                        SCHK(s, launch_mp_correct(out, mp, s->w, s->h, 1, M, 2, st));
+                    }
+                }
+        }
+    }
+    return ESRRF_OK;
+}
+
+int tune(esrrf_sweep *s)
+{
+    // per magnification: the emit launch of the first fwhm over up to 32 frames
+    for (size_t im = 0; im < s->Ms.size(); im++) {
+        const int M = s->Ms[im];
+        const BicTables &t = s->tabs[im][0];
+        BicRgcArgs a = bic_args(t, s->shape[im], s->d_U, s->d_px, s->w, s->h, M);
+        a.mode = 1;
+        a.emitC = s->d_C;
+        a.emitV = s->d_V;
+        a.out = s->d_dummy;
+        a.prev = s->d_dummy;
+        a.nframes = std::min(s->nfmax, 32);
+        a.first_init = 1;
+        a.rn = a.rn1 = a.sens = 1.0f;
+        a.sens_int = 1;
+        a.int_weighting = s->int_weighting;
+        BicShape best;
+        SCHK(s, bic_autotune(t, a, M, 2, s->stream, &best, nullptr));
+        s->shape[im] = best;
+    }
+    return ESRRF_OK;
+}
+
+int run_impl(esrrf_sweep *s, const void *frames, int dtype, int n, bool on_device)
+{
+    if (dtype != ESRRF_U16 && dtype != ESRRF_F32) return sfail(s, ESRRF_ERR_INVALID, "dtype");
+    if (n < s->nfmax)
+        return sfail(s, ESRRF_ERR_INVALID, "the sweep needs %d frames, got %d", s->nfmax, n);
+    (void)hipSetDevice(s->dev);
+    const size_t bytes = (size_t)s->w * s->h * s->nfmax * (dtype == ESRRF_U16 ? 2 : 4);
+    // first nfmax frames (gui/ParametersSweep_.java:808-816 takes frames 1..nf)
+    SCHK(s, hipMemcpyAsync(s->d_in, frames, bytes,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
+    if (s->autotune && !s->tuned) {
+        // tuning needs the gradients of these frames; it runs eagerly, before any capture
+        const long wh = (long)s->w * s->h;
+        SCHK(s, launch_convert_frames(s->d_in, dtype, s->d_px, wh * s->nfmax, s->stream));
+        SCHK(s, launch_bic_prep(s->d_px, s->d_U, s->w, s->h, s->nfmax, s->stream));
+        int rc = tune(s);
+        if (rc) return rc;
+        s->tuned = true;
+    }
+    if (!s->use_graph) {
+        int rc = enqueue_run(s, dtype);
+        if (rc) return rc;
+        s->have_result = true;
+        return ESRRF_OK;
+    }
+    if (!s->exec || s->graph_dtype != dtype) {
+        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        if (s->graph) (void)hipGraphDestroy(s->graph);
+        s->exec = nullptr;
+        s->graph = nullptr;
+        SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_run(s, dtype);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(s->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess)
+            return sfail(s, ESRRF_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+        s->graph = g;
+        SCHK(s, hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+        s->graph_dtype = dtype;
+    }
+    SCHK(s, hipGraphLaunch(s->exec, s->stream));
+    s->have_result = true;
+    return ESRRF_OK;
+}
+
+int check_index(esrrf_sweep *s, int im, int iq, int is, int ifw)
+{
+    if (im < 0 || im >= (int)s->Ms.size() || iq < 0 || iq >= (int)s->nfs.size() || is < 0 ||
+        is >= (int)s->sens.size() || ifw < 0 || ifw >= (int)s->fwhm.size())
+        return sfail(s, ESRRF_ERR_INVALID, "configuration index out of range");
+    if (!s->have_result) return sfail(s, ESRRF_ERR_STATE, "no sweep has run");
+    return ESRRF_OK;
+}
+
+}  // namespace
+
+int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
+{
+    if (!p || !out) return ESRRF_ERR_INVALID;
+    *out = nullptr;
+    if (p->width < 4 || p->height < 4 || p->n_magnifications < 1 || p->n_fwhm < 1 ||
+        p->n_sensitivity < 1 || p->n_n_frames < 1 || !p->magnifications || !p->fwhm ||
+        !p->sensitivity || !p->n_frames || p->n_sensitivity > SWEEP_MAX_S ||
+        p->n_n_frames > SWEEP_MAX_NF)
+        return ESRRF_ERR_INVALID;
+    esrrf_sweep *s = new esrrf_sweep();
+    auto bad = [&](int code) {
+        sweep_free(s);
+        delete s;
+        return code;
+    };
+    s->w = p->width;
+    s->h = p->height;
+    s->Ms.assign(p->magnifications, p->magnifications + p->n_magnifications);
+    s->fwhm.assign(p->fwhm, p->fwhm + p->n_fwhm);
+    s->sens.assign(p->sensitivity, p->sensitivity + p->n_sensitivity);
+    s->nfs.assign(p->n_frames, p->n_frames + p->n_n_frames);
+    s->int_weighting = p->int_weighting ? 1 : 0;
+    s->do_mp = p->do_mp_map_correction ? 1 : 0;
+    s->autotune = p->autotune ? 1 : 0;
+    s->use_graph = p->use_graph ? 1 : 0;
+    for (int M : s->Ms)
+        if (M < 1 || M > 8) return bad(ESRRF_ERR_UNSUPPORTED);
+    for (int nf : s->nfs) {
+        if (nf < 1) return bad(ESRRF_ERR_INVALID);
+        s->nfmax = std::max(s->nfmax, nf);
+    }
+    for (float f : s->fwhm)
+        if (!(f > 0.0f)) return bad(ESRRF_ERR_INVALID);
+    int dev = p->device;
+    if (dev < 0 && esrrf_fastest_device(&dev) != ESRRF_OK) return bad(ESRRF_ERR_HIP);
+    s->dev = dev;
+    if (hipSetDevice(dev) != hipSuccess) return bad(ESRRF_ERR_HIP);
+    if (hipStreamCreate(&s->own_stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
+    s->stream = s->own_stream;
+    const long wh = (long)s->w * s->h;
+    int MMmax = 1;
+    for (int M : s->Ms) MMmax = std::max(MMmax, M * M);
+    auto alloc = [&](void **ptr, size_t bytes) {
+        return hipMalloc(ptr, bytes ? bytes : 4) == hipSuccess;
+    };
+    if (!alloc(&s->d_in, sizeof(float) * wh * s->nfmax) ||
+        !alloc((void **)&s->d_px, sizeof(float) * wh * s->nfmax) ||
+        !alloc((void **)&s->d_U, sizeof(float2) * 4 * wh * s->nfmax) ||
+        !alloc((void **)&s->d_C, sizeof(float) * wh * MMmax * s->nfmax) ||
+        !alloc((void **)&s->d_V, sizeof(float) * wh * MMmax * s->nfmax) ||
+        !alloc((void **)&s->d_dummy, 64))
+        return bad(ESRRF_ERR_NOMEM);
+    s->tabs.resize(s->Ms.size());
+    s->shape.resize(s->Ms.size());
+    s->d_out.assign(s->Ms.size(), nullptr);
+    s->d_mp.assign(s->Ms.size(), nullptr);
+    const size_t nc = n_conf(s);
+    for (size_t im = 0; im < s->Ms.size(); im++) {
+        const int M = s->Ms[im];
+        s->tabs[im].resize(s->fwhm.size());
+        for (size_t fi = 0; fi < s->fwhm.size(); fi++) {
+            if (bic_tables_create(s->w, s->h, M, s->fwhm[fi], s->tabs[im][fi]) != hipSuccess)
+                return bad(ESRRF_ERR_HIP);
+            if (!s->tabs[im][fi].ok) return bad(ESRRF_ERR_UNSUPPORTED);
+        }
+        s->shape[im] = bic_default_shape(s->tabs[im][0], M);
+        // one shape per magnification: the largest fwhm decides whether the tables fit LDS
+        for (size_t fi = 1; fi < s->fwhm.size(); fi++)
+            if (!bic_default_shape(s->tabs[im][fi], M).taps_in_lds) s->shape[im].taps_in_lds = 0;
+        if (!alloc((void **)&s->d_out[im], sizeof(float) * nc * 4 * wh * M * M) ||
+            !alloc((void **)&s->d_mp[im], sizeof(float) * nc * 3 * M * M))
+            return bad(ESRRF_ERR_NOMEM);
+        s->tmp_floats = std::max(s->tmp_floats, (size_t)4 * wh * M * M);
+    }
+    if (!alloc((void **)&s->d_tmp, sizeof(float) * s->tmp_floats)) return bad(ESRRF_ERR_NOMEM);
+    *out = s;
+    return ESRRF_OK;
+}
+
+int esrrf_sweep_destroy(esrrf_sweep *s)
+{
+    if (!s) return ESRRF_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        (void)hipSetDevice(s->dev);
+        (void)hipStreamSynchronize(s->stream);
+        sweep_free(s);
+    }
+    delete s;
+    return ESRRF_OK;
+}
+
+int esrrf_sweep_set_stream(esrrf_sweep *s, void *hip_stream)
+{
+    if (!s) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stream = hip_stream ? (hipStream_t)hip_stream : s->own_stream;
+    return ESRRF_OK;
+}
+
+int esrrf_sweep_run(esrrf_sweep *s, const void *frames, int dtype, int n_frames)
+{
+    if (!s || !frames) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int rc = run_impl(s, frames, dtype, n_frames, false);
+    if (rc) return rc;
+    SCHK(s, hipStreamSynchronize(s->stream));
+    return ESRRF_OK;
+}
+
+int esrrf_sweep_run_device(esrrf_sweep *s, const void *d_frames, int dtype, int n_frames)
+{
+    if (!s || !d_frames) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return run_impl(s, d_frames, dtype, n_frames, true);
+}
+
+int esrrf_sweep_read(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm, float *out)
+{
+    if (!s || !out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int rc = check_index(s, i_mag, i_nf, i_sens, i_fwhm);
+    if (rc) return rc;
+    (void)hipSetDevice(s->dev);
+    const int M = s->Ms[i_mag];
+    const long n = 4L * s->w * s->h * M * M;
+    const long ci = ((long)i_nf * s->sens.size() + i_sens) * s->fwhm.size() + i_fwhm;
+    SCHK(s, launch_nan_to_zero_copy(s->d_out[i_mag] + ci * n, s->d_tmp, n, s->stream));
+    SCHK(s, hipMemcpyAsync(out, s->d_tmp, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+    SCHK(s, hipStreamSynchronize(s->stream));
+    return ESRRF_OK;
+}
+
+int esrrf_sweep_read_device(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm,
+                            float *d_out)
+{
+    if (!s || !d_out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int rc = check_index(s, i_mag, i_nf, i_sens, i_fwhm);
+    if (rc) return rc;
+    (void)hipSetDevice(s->dev);
+    const int M = s->Ms[i_mag];
+    const long n = 4L * s->w * s->h * M * M;
+    const long ci = ((long)i_nf * s->sens.size() + i_sens) * s->fwhm.size() + i_fwhm;
+    SCHK(s, launch_nan_to_zero_copy(s->d_out[i_mag] + ci * n, d_out, n, s->stream));
+    return ESRRF_OK;
+}
+
+int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int *nw, int *ppw, int *taps_in_lds, float *ms)
+{
+    if (!s || i_mag < 0 || i_mag >= (int)s->Ms.size()) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const BicShape &b = s->shape[i_mag];
+    if (nw) *nw = b.nw;
+    if (ppw) *ppw = b.ppw;
+    if (taps_in_lds) *taps_in_lds = b.taps_in_lds;
+    if (ms) *ms = b.ms;
+    return ESRRF_OK;
+}
+
+const char *esrrf_sweep_last_error(esrrf_sweep *s) { return s ? s->err.c_str() : "null sweep"; }

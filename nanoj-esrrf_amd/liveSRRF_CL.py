@@ -242,6 +242,16 @@ class LiveSRRF_CL:
             out[:, z] = (ims[:, y:y + hS, x:x + wS] - bg) / np.float32(c["Intensity scaling"][z])
         return out
 
+    def autotune(self, reps=2):
+        """Time the tiled kernel's launch shapes on the current load and keep the fastest
+        (esrrf_autotune; no counterpart in the reference)."""
+        e = self._e()
+        nw, ppw, tl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_float()
+        e.check(e.L.esrrf_autotune(e.h, int(reps), ctypes.byref(nw), ctypes.byref(ppw),
+                                   ctypes.byref(tl), ctypes.byref(ms)))
+        return dict(nw=nw.value, ppw=ppw.value, taps_in_lds=tl.value, ms=ms.value)
+
     def frameCounters(self):
         """nCurrentFrame[0..1] (exact bookkeeping hook)."""
         return self._e().counters()

@@ -419,10 +419,10 @@ def sweep_plan(fwhm0, dfwhm, n_fwhm, S0, dS, n_S, nf0, dnf, n_nf, nSlices, frc=F
     """gui/ParametersSweep_.java:720-747 arrays and the nf -> S -> fwhm loop order."""
     fwhm = [_f32(np.float32(fwhm0) + np.float32(i) * np.float32(dfwhm)) for i in range(n_fwhm)]
     sens = [S0 + i * dS for i in range(n_S)]
-    if frc:
-        n_use = min((nSlices // 2 - nf0) // dnf + 1, n_nf)
-    else:
-        n_use = min((nSlices - nf0) // dnf + 1, n_nf)
+    avail = (nSlices // 2 if frc else nSlices) - nf0
+    q = abs(avail) // abs(dnf)                    # Java int division truncates toward 0
+    q = q if (avail >= 0) == (dnf > 0) else -q
+    n_use = min(q + 1, n_nf)
     nfs = [nf0 + i * dnf for i in range(max(n_use, 0))]
     order = [(nf, s, f) for nf in nfs for s in sens for f in fwhm]
     return dict(fwhm=fwhm, sensitivity=sens, nframes=nfs, order=order)

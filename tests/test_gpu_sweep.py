@@ -1,0 +1,131 @@
+"""Fused parameter sweep (gui/ParametersSweep_.java:328-519) against the per-configuration
+path it replaces: for every (magnification, frame count, sensitivity, fwhm) the sweep must
+return what initialise + calculateSRRF(first nf frames) + readSRRFbuffer returns
+(intensity weighting and MP correction on, as the reference sweep sets them, :452).
+
+Tolerance: per channel, normalised by the reference image's max |value|,
+|a - b| <= 1e-5 + 1e-4 |b| (the same contract as the plain engine's parity tests).
+"""
+import numpy as np
+import pytest
+
+from conftest import normalised_close
+
+pytestmark = pytest.mark.gpu
+
+
+def stack(n, w, h, seed):
+    from nanoj_esrrf_amd import synth
+    return synth.blinking_stack(n, w, h, max(20, w * h // 14), 4, seed=seed)
+
+
+def plain(pkg, st, M, fwhm, S, nf):
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(st.shape[2], st.shape[1], M, fwhm, S, nf, nf, 20000, e.allCLdevices[0], True,
+                 True, False, None, 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(nf), np.zeros(nf))
+    e.calculateSRRF(st[:nf])
+    out = e.readSRRFbuffer()
+    e.release()
+    return out
+
+
+def assert_parity(got, exp, what):
+    for c, name in enumerate(("AVG", "VAR", "TAC2", "INT")):
+        ok, ex, idx = normalised_close(got[c], exp[c])
+        assert ok, f"{what}: {name} differs (excess {ex:.3g} at {idx})"
+
+
+MAGS, FWHM, SENS, NFS = [2, 3], [1.0, 1.5], [1.0, 2.0, 2.5], [5, 12]
+
+
+@pytest.fixture(scope="module")
+def swept(pkg):
+    st = stack(14, 24, 20, seed=61)
+    sw = pkg.ParametersSweep(24, 20, MAGS, FWHM, SENS, NFS, use_graph=False)
+    sw.run(st)
+    res = {(im, iq, i_s, i_f): sw.read(im, iq, i_s, i_f)
+           for im in range(len(MAGS)) for iq in range(len(NFS))
+           for i_s in range(len(SENS)) for i_f in range(len(FWHM))}
+    sw.close()
+    return st, res
+
+
+def test_sweep_matches_per_configuration_engine(pkg, swept):
+    st, res = swept
+    for (im, iq, i_s, i_f), got in res.items():
+        exp = plain(pkg, st, MAGS[im], FWHM[i_f], SENS[i_s], NFS[iq])
+        assert_parity(got, exp, f"M{MAGS[im]} nf{NFS[iq]} S{SENS[i_s]} fwhm{FWHM[i_f]}")
+
+
+def test_sweep_matches_oracle(pkg, O, swept):
+    st, res = swept
+    for im, iq, i_s, i_f in [(0, 1, 1, 1), (1, 0, 2, 0)]:
+        M, nf = MAGS[im], NFS[iq]
+        o = O.OracleLiveSRRF(24, 20, M, FWHM[i_f], SENS[i_s], nf, nf, True, True, False)
+        o.calculateSRRF(st[:nf].astype(np.float32))
+        exp = o.readSRRFbuffer()
+        assert_parity(res[(im, iq, i_s, i_f)], exp, f"oracle M{M} nf{nf}")
+
+
+def test_sweep_graph_replays_match_eager(pkg):
+    """The captured hipGraph is replayed several times, with eager work and host
+    synchronisation between the replays, and on a second stack; every replay must equal
+    the eager sequence bit for bit."""
+    a, b = stack(9, 20, 16, seed=1), stack(9, 20, 16, seed=2)
+    args = (20, 16, [2, 4], [1.5], [1.0, 2.0], [4, 9])
+    eager = pkg.ParametersSweep(*args, use_graph=False)
+    graph = pkg.ParametersSweep(*args, use_graph=True)
+    for st in (a, b, a):
+        eager.run(st)
+        for rep in range(3):
+            graph.run(st)
+            for cfg in [(0, 0, 0, 0), (1, 1, 1, 0), (0, 1, 1, 0)]:
+                assert np.array_equal(graph.read(*cfg), eager.read(*cfg)), (rep, cfg)
+            plain(pkg, st, 2, 1.5, 1.0, 4)          # unrelated eager work between replays
+    eager.close()
+    graph.close()
+
+
+def test_sweep_autotune_picks_a_supported_shape(pkg):
+    st = stack(8, 24, 24, seed=5)
+    tuned = pkg.ParametersSweep(24, 24, [5], [1.5], [2.0], [8], autotune=True, use_graph=False)
+    tuned.run(st)
+    sh = tuned.shape(0)
+    assert sh["ms"] > 0 and sh["nw"] in (4, 8, 12, 16) and sh["ppw"] * sh["nw"] >= 25
+    base = pkg.ParametersSweep(24, 24, [5], [1.5], [2.0], [8], autotune=False, use_graph=False)
+    base.run(st)
+    assert_parity(tuned.read(0, 0, 0, 0), base.read(0, 0, 0, 0), "autotuned vs default")
+    tuned.close()
+    base.close()
+
+
+def test_sweep_rejects_short_stacks_and_large_magnification(pkg):
+    from nanoj_esrrf_amd import _capi
+    sw = pkg.ParametersSweep(16, 16, [2], [1.5], [1.0], [10], use_graph=False)
+    with pytest.raises(_capi.EsrrfError):
+        sw.run(stack(6, 16, 16, seed=3))
+    sw.close()
+    with pytest.raises(_capi.EsrrfError):
+        pkg.ParametersSweep(16, 16, [9], [1.5], [1.0], [4])
+
+
+def test_engine_autotune_keeps_results(pkg):
+    st = stack(10, 32, 24, seed=9)
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(32, 24, 5, 1.5, 2, 10, 10, 20000, e.allCLdevices[0], True, True, False, None,
+                 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(10), np.zeros(10))
+    e.calculateSRRF(st)
+    before = e.readSRRFbuffer()
+    shape = e.autotune(reps=1)
+    assert shape["ms"] > 0
+    e.resetFramePosition()
+    e.calculateSRRF(st)
+    after = e.readSRRFbuffer()
+    e.release()
+    assert_parity(after, before, "after autotune")

@@ -326,3 +326,16 @@ def test_sweep_plan_order(O):
     assert s["fwhm"] == [1.0, 1.5, 2.0, 2.5, 3.0] and s["nframes"] == [100, 250, 400]
     assert s["order"][0] == (100, 1, 1.0) and s["order"][4] == (100, 1, 3.0)
     assert s["order"][5] == (100, 2, 1.0) and len(s["order"]) == 60
+
+
+def test_sweep_axes_match_oracle_plan(O, pkg):
+    """Host-side sweep axes (Java float / int arithmetic of ParametersSweep_.grabSettings)."""
+    for args in [(1.0, 0.5, 5, 1, 1, 4, 100, 150, 3, 500, False),
+                 (0.7, 0.3, 4, 2, 2, 3, 50, 100, 5, 260, True),
+                 (1.0, 0.5, 2, 1, 1, 2, 300, 100, 3, 200, False)]:
+        f, s, n = pkg.sweep_axes(*args)
+        plan = O.sweep_plan(*args[:10], frc=args[10])
+        assert f == [float(x) for x in plan["fwhm"]]
+        assert s == plan["sensitivity"] and n == plan["nframes"]
+    assert pkg.sweep_axes(1.0, 0.5, 5, 1, 1, 4, 100, 150, 3, 500)[2] == [100, 250, 400]
+    assert pkg.sweep_axes(1.0, 0.5, 5, 1, 1, 4, 300, 100, 3, 200)[2] == [300][:0]
