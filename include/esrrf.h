@@ -176,6 +176,44 @@ int esrrf_sweep_read_device(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int
 int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int *nw, int *ppw, int *taps_in_lds, float *ms);
 const char *esrrf_sweep_last_error(esrrf_sweep *s);
 
+/* ---- SQUIRREL RSE and its gradient (ErrorMapLiveSRRF.java:81-246) ------------------------
+ * RSE of the eSRRF AVG image of a loaded stack (2-D BIC, no drift, one SR image over all
+ * loaded frames) against its INT image, as the parameter sweep scores a configuration
+ * (gui/ParametersSweep_.java:473-476): crop crop_size*M SR pixels per side, blur with a
+ * Gaussian RSF of sigma SR pixels, alpha/beta least squares, RSE = sqrt(mean(residual^2)).
+ * The blur is a separable Gaussian truncated at ceil(5 sigma_max) with clamp-to-edge borders
+ * (stand-in for ImageJ's blurGaussian).  The gradient with respect to (fwhm, sensitivity,
+ * sigma) comes from an adjoint variant of the RGC kernel; no counterpart in the reference. */
+typedef struct esrrf_rse esrrf_rse;
+typedef struct esrrf_rse_params {
+    int width, height;        /* raw frame size                                             */
+    int magnification;        /* 1..8                                                       */
+    int n_frames;             /* largest stack to load (frames per SR image)                */
+    int crop_size;            /* raw pixels cropped per side, cropSize (default 3 in the GUI) */
+    int int_weighting;        /* 1 as the reference sweep                                   */
+    int do_mp_map_correction; /* 1 as the reference sweep                                   */
+    float sigma_max;          /* RSF sigma bound in SR pixels; <= 0: 4/2.35482*M (:75)      */
+    int device;               /* -1: fastest                                                */
+} esrrf_rse_params;
+int esrrf_rse_create(const esrrf_rse_params *p, esrrf_rse **out);
+int esrrf_rse_destroy(esrrf_rse *r);
+/* Upload a stack (2 <= n <= n_frames) and compute its unit tap gradients. */
+int esrrf_rse_load(esrrf_rse *r, const void *frames, int dtype, int n_frames);
+/* RSE at (fwhm, sensitivity, sigma); grad3 (may be NULL) receives dRSE/d(fwhm, S, sigma). */
+int esrrf_rse_value_and_grad(esrrf_rse *r, float fwhm, float sensitivity, float sigma,
+                             double *rse, double *grad3);
+/* The sigma that minimises the RSE on [0, sigma_max] (Brent, ErrorMapLiveSRRF.java:119-127). */
+int esrrf_rse_best_sigma(esrrf_rse *r, float fwhm, float sensitivity, double *sigma,
+                         double *rse, int *n_eval);
+/* Projected Adam descent on (fwhm, S, sigma) from x0[3] for `steps` gradient steps;
+ * lr/lo/hi (each [3], may be NULL for defaults) are step sizes and box bounds;
+ * trajectory[(steps+1)*4] receives (fwhm, S, sigma, RSE) of every evaluated point. */
+int esrrf_rse_descend(esrrf_rse *r, const float *x0, int steps, const float *lr,
+                      const float *lo, const float *hi, double *trajectory);
+/* Evaluations and RGC passes made so far (each new (fwhm, S) costs one RGC pass). */
+int esrrf_rse_counters(esrrf_rse *r, long long *evaluations, long long *rgc_passes);
+const char *esrrf_rse_last_error(esrrf_rse *r);
+
 const char *esrrf_last_error(esrrf_engine *e);
 const char *esrrf_version(void);
 

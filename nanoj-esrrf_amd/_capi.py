@@ -32,6 +32,8 @@ EXPORTS = [
     "esrrf_sweep_create", "esrrf_sweep_destroy", "esrrf_sweep_set_stream", "esrrf_sweep_run",
     "esrrf_sweep_run_device", "esrrf_sweep_read", "esrrf_sweep_read_device",
     "esrrf_sweep_shape", "esrrf_sweep_last_error",
+    "esrrf_rse_create", "esrrf_rse_destroy", "esrrf_rse_load", "esrrf_rse_value_and_grad",
+    "esrrf_rse_best_sigma", "esrrf_rse_descend", "esrrf_rse_counters", "esrrf_rse_last_error",
 ]
 
 
@@ -71,6 +73,15 @@ class SweepParams(ctypes.Structure):
     ]
 
 
+class RseParams(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int), ("magnification", ctypes.c_int),
+        ("n_frames", ctypes.c_int), ("crop_size", ctypes.c_int), ("int_weighting", ctypes.c_int),
+        ("do_mp_map_correction", ctypes.c_int), ("sigma_max", ctypes.c_float),
+        ("device", ctypes.c_int),
+    ]
+
+
 _LIB = None
 
 
@@ -83,6 +94,7 @@ def lib():
                            "(run `python nanoj-esrrf_amd/build_lib.py`)")
     L = ctypes.CDLL(LIB_PATH)
     vp, ip, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
     c_int, c_float = ctypes.c_int, ctypes.c_float
     sig = {
         "esrrf_device_count": [ip],
@@ -123,8 +135,18 @@ def lib():
         "esrrf_sweep_read_device": [vp, c_int, c_int, c_int, c_int, vp],
         "esrrf_sweep_shape": [vp, c_int, ip, ip, ip, fp],
         "esrrf_sweep_last_error": [vp],
+        "esrrf_rse_create": [ctypes.POINTER(RseParams), ctypes.POINTER(vp)],
+        "esrrf_rse_destroy": [vp],
+        "esrrf_rse_load": [vp, vp, c_int, c_int],
+        "esrrf_rse_value_and_grad": [vp, c_float, c_float, c_float, dp, dp],
+        "esrrf_rse_best_sigma": [vp, c_float, c_float, dp, dp, ip],
+        "esrrf_rse_descend": [vp, fp, c_int, fp, fp, fp, dp],
+        "esrrf_rse_counters": [vp, ctypes.POINTER(ctypes.c_longlong),
+                               ctypes.POINTER(ctypes.c_longlong)],
+        "esrrf_rse_last_error": [vp],
     }
-    strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error")
+    strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error",
+               "esrrf_rse_last_error")
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args

@@ -221,11 +221,18 @@ __device__ __forceinline__ void sens_derivs(float C, float sens, int sens_int, f
         return;
     }
     const float Cs = apply_sens(C, sens, sens_int);
-    const float Cs1 = sens_int == 1 ? 1.0f : (sens_int == 2 ? C : powf(C, sens - 1.0f));
     const float vv = intw ? v : 1.0f;
     t = vv * Cs;
-    dtdC = vv * sens * Cs1;
-    dtdS = C > 0.0f ? t * logf(C) : 0.0f;
+    if (C > 0.0f) {
+        const float Cs1 = sens_int == 1 ? 1.0f : (sens_int == 2 ? C : powf(C, sens - 1.0f));
+        dtdC = vv * sens * Cs1;
+        dtdS = t * logf(C);
+    } else {
+        // C = 0: d(C^S)/dC is 1 for S = 1, 0 for S > 1 and unbounded for S < 1 (the zero
+        // subgradient is taken); d(C^S)/dS = 0
+        dtdC = sens_int == 1 ? vv : 0.0f;
+        dtdS = 0.0f;
+    }
 }
 
 // PPW: sub-pixel phases per wave; STAGE_U: U-tile elements each thread stages in

@@ -1,0 +1,113 @@
+"""SQUIRREL RSE of the AVG image (ErrorMapLiveSRRF.java on the sweep's images) and its
+hand-written gradient.
+
+* value: against oracle/rse.py applied to the oracle's AVG/INT images, |a-b| <= 1e-4 |b|
+* gradient: against central finite differences (h = 1e-3) of the HIP forward and of the
+  CPU oracle forward, relative error < 1e-3 of the gradient's norm
+* sigma search: Brent on the device against Brent on the oracle
+* descent: 30 projected Adam steps reach the grid's best RSE within 1 %
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import rse as R
+
+pytestmark = pytest.mark.gpu
+
+W, H, M, NF, CROP = 32, 28, 4, 12, 3
+
+
+@pytest.fixture(scope="module")
+def data():
+    from nanoj_esrrf_amd import synth
+    return synth.blinking_stack(NF, W, H, 80, 4, seed=404)
+
+
+@pytest.fixture(scope="module")
+def rse_obj(pkg, data):
+    r = pkg.RSE(W, H, M, NF, crop_size=CROP)
+    r.load(data)
+    yield r
+    r.close()
+
+
+def oracle_rse(O, st, fwhm, S, sigma, mp=True):
+    o = O.OracleLiveSRRF(W, H, M, fwhm, S, NF, NF, True, mp, False)
+    o.calculateSRRF(st.astype(np.float32))
+    out = o.readSRRFbuffer()
+    sr, ref = R.crop(out[0], M, CROP), R.crop(out[3], M, CROP)
+    smax = 4.0 / 2.35482 * M
+    return R.rse(ref, sr, sigma, R.default_K(smax))
+
+
+@pytest.mark.parametrize("fwhm,S,sigma", [(1.5, 2.0, 3.0), (1.0, 1.0, 6.0), (2.2, 1.5, 1.2)])
+def test_rse_value_matches_oracle(O, data, rse_obj, fwhm, S, sigma):
+    got = rse_obj.value(fwhm, S, sigma)
+    exp = oracle_rse(O, data, fwhm, S, sigma)
+    assert math.isclose(got, exp, rel_tol=1e-4), (got, exp)
+
+
+def test_rse_value_without_mp_correction(pkg, O, data):
+    r = pkg.RSE(W, H, M, NF, crop_size=0, mp_correction=False)
+    r.load(data)
+    got = r.value(1.5, 2.0, 2.5)
+    r.close()
+    o = O.OracleLiveSRRF(W, H, M, 1.5, 2.0, NF, NF, True, False, False)
+    o.calculateSRRF(data.astype(np.float32))
+    out = o.readSRRFbuffer()
+    exp = R.rse(out[3], out[0], 2.5, R.default_K(4.0 / 2.35482 * M))
+    assert math.isclose(got, exp, rel_tol=1e-4), (got, exp)
+
+
+def fd_grad(f, x, h=1e-3):
+    g = np.zeros(3)
+    for i in range(3):
+        a, b = list(x), list(x)
+        a[i] += h
+        b[i] -= h
+        g[i] = (f(*a) - f(*b)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("x", [(1.3, 1.7, 4.0), (1.8, 2.0, 2.2)])
+def test_rse_gradient_matches_central_differences(O, data, rse_obj, x):
+    val, g = rse_obj.value_and_grad(*x)
+    fd_hip = fd_grad(rse_obj.value, x)
+    err = np.abs(g - fd_hip).max() / np.abs(fd_hip).max()
+    assert err < 1e-3, (g, fd_hip, err)
+    fd_cpu = fd_grad(lambda a, b, c: oracle_rse(O, data, a, b, c), x)
+    err = np.abs(g - fd_cpu).max() / np.abs(fd_cpu).max()
+    assert err < 1e-3, (g, fd_cpu, err)
+
+
+def test_best_sigma_matches_oracle_brent(O, data, rse_obj):
+    s, v, n = rse_obj.best_sigma(1.5, 2.0)
+    o = O.OracleLiveSRRF(W, H, M, 1.5, 2.0, NF, NF, True, True, False)
+    o.calculateSRRF(data.astype(np.float32))
+    out = o.readSRRFbuffer()
+    sr, ref = R.crop(out[0], M, CROP), R.crop(out[3], M, CROP)
+    smax = 4.0 / 2.35482 * M
+    xs, fs, _ = R.best_sigma(ref, sr, smax, R.default_K(smax))
+    assert math.isclose(v, fs, rel_tol=1e-4), (v, fs)
+    assert abs(s - xs) <= 0.02 * smax, (s, xs)
+    assert 0 < n < 1000
+
+
+def test_descent_reaches_grid_best(rse_obj):
+    """Grid (fwhm x S, sigma by Brent per point) and descent search the same box."""
+    grid = []
+    for fwhm in np.linspace(0.8, 3.0, 12):
+        for S in (1.0, 2.0, 3.0, 4.0):
+            s, v, _ = rse_obj.best_sigma(float(fwhm), S)
+            grid.append((v, fwhm, S, s))
+    best = min(grid)
+    smax = 4.0 / 2.35482 * M
+    traj = rse_obj.descend((1.0, 1.0, 1.5 * M), steps=30, lo=(0.8, 1.0, 0.01),
+                           hi=(3.0, 4.0, smax))
+    assert traj[:, 0].min() >= 0.8 - 1e-6 and traj[:, 1].min() >= 1.0 - 1e-6
+    print("grid best (rse, fwhm, S, sigma):", best)
+    print("descent trajectory (fwhm, S, sigma, rse):\n", traj)
+    assert traj[-1, 3] < traj[0, 3]
+    assert traj[:, 3].min() <= 1.01 * best[0], (traj[:, 3].min(), best)

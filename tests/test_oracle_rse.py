@@ -1,0 +1,90 @@
+"""Pins of the CPU RSE restatement (oracle/rse.py) by closed forms, brute force and
+invariants (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import rse as R
+
+
+def brute_blur(img, sigma, K):
+    """Direct 2-D convolution with the product kernel and clamp-to-edge indices."""
+    g = R.gauss_taps(sigma, K)
+    H, W = img.shape
+    out = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            acc = 0.0
+            for j in range(-K, K + 1):
+                for i in range(-K, K + 1):
+                    acc += g[j + K] * g[i + K] * img[min(max(y + j, 0), H - 1),
+                                                     min(max(x + i, 0), W - 1)]
+            out[y, x] = acc
+    return out
+
+
+def test_taps_normalised_symmetric_and_derivative():
+    for s in (0.4, 1.3, 3.7):
+        g = R.gauss_taps(s, 12)
+        assert abs(g.sum() - 1.0) < 1e-14 and np.allclose(g, g[::-1])
+        h = 1e-6
+        fd = (R.gauss_taps(s + h, 12) - R.gauss_taps(s - h, 12)) / (2 * h)
+        assert np.allclose(R.gauss_taps(s, 12, True), fd, atol=1e-8)
+        assert abs(R.gauss_taps(s, 12, True).sum()) < 1e-14   # normalisation is preserved
+
+
+def test_separable_blur_equals_bruteforce_with_clamped_edges():
+    rng = np.random.default_rng(0)
+    img = rng.normal(size=(7, 9))
+    assert np.allclose(R.blur(img, 1.1, 3), brute_blur(img, 1.1, 3), atol=1e-12)
+
+
+def test_blur_keeps_constants_and_ones():
+    assert np.allclose(R.blur(np.full((6, 5), 3.25), 2.0, 10), 3.25, atol=1e-12)
+
+
+def test_blur_sigma_derivative_matches_central_difference():
+    rng = np.random.default_rng(1)
+    img = rng.normal(size=(10, 8))
+    s, K, h = 1.7, 9, 1e-5
+    fd = (R.blur(img, s + h, K) - R.blur(img, s - h, K)) / (2 * h)
+    assert np.allclose(R.blur_dsigma(img, s, K), fd, atol=1e-7)
+
+
+def test_alpha_beta_recovers_an_affine_map_and_rse_zero():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(5, 6))
+    a, b = R.alpha_beta(x, 2.5 * x - 7.0)
+    assert abs(a - 2.5) < 1e-12 and abs(b + 7.0) < 1e-12
+    sr = rng.normal(size=(8, 8))
+    ref = 3.0 * R.blur(sr, 1.2, 6) + 4.0
+    assert R.rse(ref, sr, 1.2, 6) < 1e-12
+
+
+def test_rse_invariant_to_affine_changes_of_the_sr_image():
+    rng = np.random.default_rng(3)
+    sr, ref = rng.random((9, 9)), rng.random((9, 9))
+    r0 = R.rse(ref, sr, 1.5, 8)
+    assert math.isclose(R.rse(ref, -4.0 * sr + 11.0, 1.5, 8), r0, rel_tol=1e-10)
+
+
+def test_rse_matches_explicit_least_squares():
+    rng = np.random.default_rng(4)
+    sr, ref = rng.random((7, 8)), rng.random((7, 8))
+    x = R.blur(sr, 0.9, 5).ravel()
+    A = np.stack([x, np.ones_like(x)], 1)
+    coef, res, *_ = np.linalg.lstsq(A, ref.ravel(), rcond=None)
+    assert math.isclose(R.rse(ref, sr, 0.9, 5), math.sqrt(res[0] / x.size), rel_tol=1e-10)
+
+
+@pytest.mark.parametrize("f,lo,hi,xmin", [
+    (lambda x: (x - 0.3) ** 2 + 1.0, 0.0, 2.0, 0.3),
+    (lambda x: (x - 1.7) ** 4 - 0.5 * (x - 1.7) ** 2, 1.7, 4.0, 1.7 + 0.5),
+    (lambda x: abs(x - 0.7) + 0.1 * x, 0.0, 5.0, 0.7),
+    (lambda x: math.cosh(x - 3.0), 0.0, 3.0, 3.0),      # minimum on the boundary
+])
+def test_brent_finds_the_minimum(f, lo, hi, xmin):
+    x, fx, n = R.brent_min(f, lo, hi)
+    assert abs(x - xmin) < 1e-6 and n < 200
+    assert fx <= f(xmin) + 1e-9
