@@ -1,18 +1,26 @@
-"""Benchmark of the MI355X eSRRF engine on the production 2-D workload.
+"""Benchmark of the MI355X eSRRF engine.
 
-Workload: 1000 frames of 256x256 uint16 (synthetic blinking filaments, 2000 emitters on
-40 filaments, background 200), magnification 5, fwhm ("radius") 1.5, sensitivity 2,
-1000 frames per SR frame, intensity weighting and macro-pixel correction on, BIC
-interpolation -> one 1280x1280 float32 SR image (AVG, VAR, TAC2, INT).
+Headline (the JSON line's `value`): the production 2-D workload -- 1000 frames of 256x256
+uint16 (synthetic blinking filaments, 2000 emitters on 40 filaments, background 200),
+magnification 5, fwhm ("radius") 1.5, sensitivity 2, 1000 frames per SR frame, intensity
+weighting and macro-pixel correction on, BIC interpolation -> one 1280x1280 float32 SR image
+(AVG, VAR, TAC2, INT).  A step = resetFramePosition + calculateSRRF over the 1000 frames
+already resident in HBM + readSRRFbuffer's finalisation (VAR, MP map, MP correction, NaN
+scrub) into a device buffer.  Metric: output Mpx/s = 1280*1280*1000 / step time / 1e6.
 
-A step = resetFramePosition + calculateSRRF over the 1000 frames already resident in HBM
-+ readSRRFbuffer's finalisation (VAR, MP map, MP correction, NaN scrub) into a device
-buffer.  Metric: output Mpx/s = 1280*1280*1000 / step time / 1e6.
+`roofline` covers the RGC kernel: algorithmic FLOPs (14 per gated RGC sample + 42 per output
+pixel-frame) over its HIP-event-timed duration on the launch stream; `traffic` is its HBM
+bytes per launch from the committed rocprofv3 counter summary.  `cpu_baseline` times the
+CPU oracle (C restatement, OpenMP) on a bounded sample of the same workload.
 
-Prints ONE JSON line.  The RGC kernel's average duration is measured with HIP events on
-the launch stream; its algorithmic FLOPs follow the convention 14 per gated RGC sample +
-42 per output pixel-frame.  cpu_baseline times the CPU oracle (C restatement, OpenMP) on a
-bounded sample of the same workload.
+The metric's second part, `sweep`: the 180 configurations of a parameter sweep over a
+500-frame 256x256 stack (magnification {2,4,5} x fwhm {1,1.5,2,2.5,3} x sensitivity {1..4} x
+first {100,250,500} frames) as one fused device sequence (hipGraph, autotuned) against
+looping the plain engine once per configuration.  `rse_search`: on 500 frames of 128x128
+(M=5), 30 adjoint-driven descent steps on (fwhm, S, RSF sigma) against a 25x25 (fwhm, S)
+grid with the RSF sigma optimised per point, both on the GPU.
+
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -39,6 +47,10 @@ def parse():
     ap.add_argument("--frames", type=int, default=NF, help="frames per SR frame (default 1000)")
     ap.add_argument("--cpu-sample-frames", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-rse", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default RGC launch shape instead of esrrf_autotune's")
     ap.add_argument("--mode", choices=("bic", "fht"), default="bic",
                     help="bic: the production workload (default); fht: the same stack through "
                          "the Fourier-upsampling mode")
@@ -93,6 +105,109 @@ def load_traffic():
     return None
 
 
+def params(C, w, h, M_, fwhm, S, nf, fht=False):
+    p = C.Params()
+    p.width, p.height, p.magnification = w, h, M_
+    p.fwhm, p.sensitivity = fwhm, S
+    p.n_frames_on_gpu, p.n_frame_for_srrf, p.block_length, p.device = nf, nf, 20000, 0
+    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, 1, int(fht)
+    return p
+
+
+def bench_sweep(pkg, torch, stack):
+    """Fused sweep vs the plain engine looped per configuration (order nf -> S -> fwhm as
+    gui/ParametersSweep_.java:328-380, for each magnification)."""
+    import numpy as np
+    C = pkg._capi
+    mags, fwhms, senss, nfs = [2, 4, 5], [1.0, 1.5, 2.0, 2.5, 3.0], [1.0, 2.0, 3.0, 4.0], \
+        [100, 250, 500]
+    nconf = len(mags) * len(fwhms) * len(senss) * len(nfs)
+    n = max(nfs)
+    dev = torch.device("cuda:0")
+    d_frames = torch.from_numpy(stack[:n].view(np.int16).reshape(-1)).to(dev)
+    stream = torch.cuda.current_stream()
+    sw = pkg.ParametersSweep(W, H, mags, fwhms, senss, nfs, autotune=True, use_graph=True,
+                             device=0)
+    sw.set_stream(stream.cuda_stream)
+    sw.run_device(d_frames.data_ptr(), C.ESRRF_U16, n)     # autotune + capture + first replay
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sw.run_device(d_frames.data_ptr(), C.ESRRF_U16, n)
+    torch.cuda.synchronize()
+    fused_s = (time.perf_counter() - t0) / reps
+    shapes = {m: sw.shape(i) for i, m in enumerate(mags)}
+    sw.close()
+    # looped plain: a fresh engine per configuration, as the reference re-initialises
+    d_out = torch.empty(4 * W * H * 25, dtype=torch.float32, device=dev)
+    fptr = ctypes.c_void_p(d_frames.data_ptr())
+    optr = ctypes.c_void_p(d_out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in mags:
+        for nf in nfs:
+            for s in senss:
+                for f in fwhms:
+                    eng = pkg.Engine(params(C, W, H, m, f, s, nf))
+                    L, h = eng.L, eng.h
+                    eng.check(L.esrrf_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
+                    eng.check(L.esrrf_reset_frame_position(h))
+                    eng.check(L.esrrf_calculate_device(h, fptr, C.ESRRF_U16, nf, None, None))
+                    eng.check(L.esrrf_read_result_device(h, optr))
+                    eng.close()
+    torch.cuda.synchronize()
+    looped_s = time.perf_counter() - t0
+    del d_frames, d_out
+    return {"configs": nconf, "frames": n, "fused_s": round(fused_s, 4),
+            "looped_plain_s": round(looped_s, 4),
+            "fused_configs_per_s": round(nconf / fused_s, 2),
+            "looped_plain_configs_per_s": round(nconf / looped_s, 2),
+            "speedup": round(looped_s / fused_s, 2),
+            "autotuned_shapes": {str(k): [v["nw"], v["ppw"], v["taps_in_lds"]]
+                                 for k, v in shapes.items()},
+            "workload": "500x256x256 u16; M {2,4,5} x fwhm {1..3 step .5} x S {1..4} x "
+                        "first {100,250,500} frames; intensity weighting + MP on"}
+
+
+def bench_rse(pkg, stack):
+    """30 adjoint descent steps vs a 25x25 grid (sigma by Brent per point), same box."""
+    import numpy as np
+    Mr = 5
+    r = pkg.RSE(128, 128, Mr, stack.shape[0], crop_size=3, device=0)
+    r.load(stack)
+    smax = 4.0 / 2.35482 * Mr
+    # the start (fwhm 1, S 1, sigma 1.5 raw px) lies inside the box
+    lo, hi = (0.5, 0.5, 0.01), (3.0, 4.0, smax)
+    t0 = time.perf_counter()
+    best = None
+    for f in np.linspace(lo[0], hi[0], 25):
+        for s in np.linspace(lo[1], hi[1], 25):
+            sg, v, _ = r.best_sigma(float(f), float(s))
+            if best is None or v < best[0]:
+                best = (v, float(f), float(s), sg)
+    grid_s = time.perf_counter() - t0
+    e0, p0 = r.counters()
+    t0 = time.perf_counter()
+    traj = r.descend((1.0, 1.0, 1.5 * Mr), steps=30, lo=lo, hi=hi)
+    desc_s = time.perf_counter() - t0
+    e1, p1 = r.counters()
+    r.close()
+    k = int(np.argmin(traj[:, 3]))
+    return {"grid_points": 625, "grid_s": round(grid_s, 3),
+            "grid_best": {"rse": round(best[0], 5), "fwhm": round(best[1], 4),
+                          "sensitivity": round(best[2], 4), "sigma": round(best[3], 4)},
+            "descent_steps": 30, "descent_s": round(desc_s, 4),
+            "descent_best": {"rse": round(float(traj[k, 3]), 5), "fwhm": round(traj[k, 0], 4),
+                             "sensitivity": round(traj[k, 1], 4), "sigma": round(traj[k, 2], 4)},
+            "descent_within_1pct_of_grid": bool(traj[k, 3] <= 1.01 * best[0]),
+            "time_ratio_grid_over_descent": round(grid_s / desc_s, 2),
+            "descent_rgc_passes": p1 - p0,
+            "workload": "500x128x128 u16, M=5, AVG, crop 3, box fwhm [0.5,3] x S [0.5,4] x "
+                        "sigma [0.01, 8.49] SR px; grid sigma by Brent per point; start "
+                        "(1, 1, 7.5)"}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -119,13 +234,8 @@ def main():
     d_out = torch.empty(4 * W * M * H * M, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
 
-    p = C.Params()
-    p.width, p.height, p.magnification = W, H, M
-    p.fwhm, p.sensitivity = FWHM, SENS
-    p.n_frames_on_gpu, p.n_frame_for_srrf, p.block_length, p.device = nf, nf, 20000, 0
     fht = a.mode == "fht"
-    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, 1, int(fht)
-    eng = pkg.Engine(p)
+    eng = pkg.Engine(params(C, W, H, M, FWHM, SENS, nf, fht))
     L, h = eng.L, eng.h
     eng.check(L.esrrf_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
     nact = ctypes.c_longlong()
@@ -141,6 +251,16 @@ def main():
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
+    # pick the RGC launch shape on this load (esrrf_autotune), as a deployment would once
+    tuned = None
+    if not fht and not a.no_autotune:
+        nw_, ppw_, tl_, ms_ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        eng.check(L.esrrf_autotune(h, 2, ctypes.byref(nw_), ctypes.byref(ppw_),
+                                   ctypes.byref(tl_), ctypes.byref(ms_)))
+        tuned = {"waves_per_block": nw_.value, "phases_per_wave": ppw_.value,
+                 "taps_in_lds": tl_.value, "tuning_launch_ms": round(ms_.value, 3)}
+        step()
+        torch.cuda.synchronize()
     eng.check(L.esrrf_profile_enable(h, 1))
     t0 = time.perf_counter()
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -196,13 +316,18 @@ def main():
                      "prep_kernel_ms": round(prep_ms.value / max(prep_n.value, 1), 3)
                      if not fht else None,
                      "upsample_ms_per_load": round(up_ms.value / max(up_n.value, 1), 3)
-                     if fht else None},
+                     if fht else None,
+                     "launch_shape": tuned},
         "wall_s": round(wall, 3),
     }
     eng.close()
     del d_frames, d_out
     if not a.no_cpu_baseline and not fht:
         res["cpu_baseline"] = cpu_baseline(stack, min(a.cpu_sample_frames, nf))
+    if not a.no_sweep and not fht:
+        res["sweep"] = bench_sweep(pkg, torch, synth.sweep_256x256x500())
+    if not a.no_rse and not fht:
+        res["rse_search"] = bench_rse(pkg, synth.rse_128x128x500())
     print(json.dumps(res))
 
 

@@ -37,14 +37,15 @@ struct esrrf_rse {
     void *d_in = nullptr;
     float *d_px = nullptr;
     float2 *d_U = nullptr;
-    float *d_out4 = nullptr, *d_dummy = nullptr;
+    float *d_out4 = nullptr, *d_prev = nullptr;
     int loaded = 0;
     BicTables tabs;
     float tabs_fwhm = -1.0f;
     BicShape shape;
     RsePlan plan{};
-    // (fwhm, S) of the current SR image in plan.SR/Dt/DS
+    // (fwhm, S) of the current SR image in plan.SR (and of Dt/DS when cur_grad)
     float cur_fwhm = -1.0f, cur_sens = -1.0f;
+    bool cur_grad = false;
     long long evals = 0, rgc_passes = 0;
 };
 
@@ -71,7 +72,7 @@ int rfail(esrrf_rse *r, int code, const char *fmt, ...)
 void rse_free(esrrf_rse *r)
 {
     RsePlan &p = r->plan;
-    void *ptrs[] = {r->d_in, r->d_px, r->d_U, r->d_out4, r->d_dummy, p.stats, p.Y, p.SR, p.Dt,
+    void *ptrs[] = {r->d_in, r->d_px, r->d_U, r->d_out4, r->d_prev, p.stats, p.Y, p.SR, p.Dt,
                     p.DS, p.T1, p.T2, p.X, p.G1, p.G2, p.G3, p.taps, p.part, p.red};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -79,23 +80,24 @@ void rse_free(esrrf_rse *r)
     if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
 }
 
-// AVG, dAVG/dtSS, dAVG/dS, INT of (fwhm, S) over the loaded frames, then the cropped
-// MP-corrected SR image and its derivatives.
-int prepare_sr(esrrf_rse *r, float fwhm, float sens)
+// The cropped MP-corrected SR image of (fwhm, S) over the loaded frames: with grad, from the
+// adjoint launch (AVG, dAVG/dtSS, dAVG/dS, INT) and with its derivatives; without, from the
+// plain cumulant launch (AVG, E2, TAC2, INT), as the reference computes a configuration.
+int prepare_sr(esrrf_rse *r, float fwhm, float sens, bool grad)
 {
-    if (r->cur_fwhm == fwhm && r->cur_sens == sens) return ESRRF_OK;
+    if (r->cur_fwhm == fwhm && r->cur_sens == sens && (r->cur_grad || !grad)) return ESRRF_OK;
     if (r->tabs_fwhm != fwhm) {
         bic_tables_free(r->tabs);
         r->tabs_fwhm = -1.0f;
         RCHK(r, bic_tables_create(r->w, r->h, r->M, fwhm, r->tabs));
         if (!r->tabs.ok) return rfail(r, ESRRF_ERR_UNSUPPORTED, "fwhm %g: window too large", fwhm);
         r->tabs_fwhm = fwhm;
-        r->shape = bic_default_shape(r->tabs, r->M, true);
+        r->shape = bic_default_shape(r->tabs, r->M, true);   // fits both variants
     }
     BicRgcArgs a = bic_args(r->tabs, r->shape, r->d_U, r->d_px, r->w, r->h, r->M);
-    a.mode = 2;
+    a.mode = grad ? 2 : 0;
     a.out = r->d_out4;
-    a.prev = r->d_dummy;
+    a.prev = r->d_prev;
     a.frame0 = 0;
     a.nframes = r->loaded;
     a.first_init = 1;
@@ -108,6 +110,7 @@ int prepare_sr(esrrf_rse *r, float fwhm, float sens)
     RCHK(r, launch_rse_prep(r->plan, r->d_out4, r->stream));
     r->cur_fwhm = fwhm;
     r->cur_sens = sens;
+    r->cur_grad = grad;
     r->rgc_passes++;
     return ESRRF_OK;
 }
@@ -118,7 +121,7 @@ int eval_impl(esrrf_rse *r, float fwhm, float sens, float sigma, double *rse, do
     if (!(fwhm > 0.0f) || !(sens > 0.0f) || !(sigma >= 0.0f) || sigma > r->sigma_max)
         return rfail(r, ESRRF_ERR_INVALID, "parameters outside fwhm > 0, S > 0, 0 <= sigma <= %g",
                      r->sigma_max);
-    int rc = prepare_sr(r, fwhm, sens);
+    int rc = prepare_sr(r, fwhm, sens, grad != nullptr);
     if (rc) return rc;
     RCHK(r, launch_rse_eval(r->plan, sigma, grad != nullptr, r->stream));
     double red[10];
@@ -262,7 +265,7 @@ int esrrf_rse_create(const esrrf_rse_params *p, esrrf_rse **out)
               alloc((void **)&r->d_px, sizeof(float) * wh * r->nF) &&
               alloc((void **)&r->d_U, sizeof(float2) * 4 * wh * r->nF) &&
               alloc((void **)&r->d_out4, sizeof(float) * 4 * whM) &&
-              alloc((void **)&r->d_dummy, 64) &&
+              alloc((void **)&r->d_prev, sizeof(float) * whM) &&
               alloc((void **)&q.stats, sizeof(double) * 3 * r->M * r->M);
     float **bufs[] = {&q.Y, &q.SR, &q.Dt, &q.DS, &q.T1, &q.T2, &q.X, &q.G1, &q.G2, &q.G3};
     for (float **b : bufs) ok = ok && alloc((void **)b, cb);

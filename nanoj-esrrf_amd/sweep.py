@@ -73,6 +73,14 @@ class ParametersSweep:
         self._check(self.L.esrrf_sweep_run(self.h, a.ctypes.data_as(ctypes.c_void_p), dt,
                                            a.shape[0]))
 
+    def run_device(self, ptr, dtype, n_frames):
+        """Frames already in device memory (pointer), asynchronous on the sweep's stream."""
+        self._check(self.L.esrrf_sweep_run_device(self.h, ctypes.c_void_p(ptr), dtype,
+                                                  int(n_frames)))
+
+    def set_stream(self, stream_ptr):
+        self._check(self.L.esrrf_sweep_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
     def read(self, i_mag, i_nf, i_sens, i_fwhm):
         M = int(self.mags[i_mag])
         out = np.empty((4, self.height * M, self.width * M), np.float32)

@@ -109,3 +109,20 @@ def plain_64x64x100(seed=SEED_BASE + 1):
 
 def production_256x256x1000(seed=SEED_BASE + 2, n_frames=1000):
     return blinking_stack(n_frames, 256, 256, 2000, 40, background=200.0, seed=seed)
+
+
+def sweep_256x256x500(seed=SEED_BASE + 3):
+    """Parameter-sweep workload: 500 frames of 256x256, the production emitter density."""
+    return blinking_stack(500, 256, 256, 2000, 40, background=200.0, seed=seed)
+
+
+def rse_128x128x500(seed=SEED_BASE + 4):
+    """Parameter-selection workload: 500 frames of 128x128 at the production density
+    (a quarter of the field: 500 emitters on 10 filaments)."""
+    return blinking_stack(500, 128, 128, 500, 10, background=200.0, seed=seed)
+
+
+def mfm_9x128x128x512(seed=SEED_BASE + 5):
+    """3-D workload: 512 frames of 9 focal planes of 128x128 (1500 emitters on 30 filaments
+    in a 9-plane volume) as planes [n][9][128][128]."""
+    return blinking_stack_3d(512, 128, 128, 9, seed=seed)
