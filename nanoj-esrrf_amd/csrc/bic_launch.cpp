@@ -54,7 +54,8 @@ hipError_t bic_tables_create(int w, int h, int M, float fwhm, BicTables &t)
 // leaves room for two blocks per CU
 static bool tlds_fits(const BicTables &t, bool adj)
 {
-    return (size_t)(t.geo.max_class_taps + 4) * (sizeof(Tap) + (adj ? sizeof(float) : 0)) <=
+    return (size_t)(t.geo.max_class_taps + TAP_SPARE) *
+               (sizeof(Tap) + (adj ? sizeof(float) : 0)) <=
            48 * 1024;
 }
 

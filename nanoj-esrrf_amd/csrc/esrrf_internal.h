@@ -5,6 +5,10 @@
 
 namespace esrrf {
 
+// Zero-weight taps at the end of the BIC tap table: the tiled kernel's prefetch of the next
+// chunks of a list may read up to this many entries past a class's last tap.
+constexpr int TAP_SPARE = 8;
+
 // One radial-gradient-convergence sample ("tap") of a magnified pixel.
 //   off : float2 index of the tap's unit gradient inside the block's LDS tile,
 //         relative to the lane's base index (see tile layout in k_bic.hip)

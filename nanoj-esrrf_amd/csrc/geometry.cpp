@@ -172,9 +172,8 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
             g.max_class_taps = std::max(g.max_class_taps, (int)g.taps.size() - cls_first);
         }
     }
-    // one spare zero chunk so the kernel's prefetch of "the next 4 taps" never reads past
-    // the table
-    for (int k = 0; k < 4; k++) {
+    // spare zero taps so the kernel's prefetch of the next chunks never reads past the table
+    for (int k = 0; k < TAP_SPARE; k++) {
         Tap z;
         z.off = 0;
         z.a = z.b = z.W = 0.0f;

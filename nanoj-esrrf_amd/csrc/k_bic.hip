@@ -356,41 +356,54 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
     }
 
 
+    // TLDS: this block's tap tables (every phase of its x/y class, contiguous on the host)
+    // are staged once in LDS and read with uniform-address ds_read_b128 in the tap loop,
+    // so the loop's loads are all in-order LDS traffic
+    Tap *lT = reinterpret_cast<Tap *>(lR + NBUF * RAW_SZ);
+    float *lWd = reinterpret_cast<float *>(lT + a.max_class_taps + TAP_SPARE);   // MODE 2
+    // per-element plan of the U-tile fill (computed once per block)
+    int *lPlan = TLDS ? reinterpret_cast<int *>(lWd + (MODE == 2 ? a.max_class_taps + TAP_SPARE
+                                                                 : 0))
+                      : reinterpret_cast<int *>(lT);
     // direct (unstaged) tile fill: used for the first frame and when a block has too few
     // threads to stage a whole tile in registers
     auto fill_direct = [&](int f, float2 *dU, float *dR) {
         const float2 *Uf = a.U + whI * f;
         for (int e = tid; e < nU; e += nthr) {
-            const int pl = e / (a.PH * pw);
-            const int r = (e - pl * a.PH * pw) / pw;
-            const int c = e - pl * a.PH * pw - r * pw;
-            const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+            // packed plan: LDS index (15 bits), 2x-grid row (8) and column (8) in the tile
+            const int pk = lPlan[e];
+            const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
             const int kx = kxo + gx, ky = kyo + gy;
             float2 v = make_float2(0.0f, 0.0f);
             if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI)
                 v = Uf[(long)ky * wI + kx];
-            dU[pl * planesz + r * a.PP + c] = v;
+            dU[pk & 0x7fff] = v;
         }
         const float *Pf = a.px + wh * f;
-        for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
+        if (roff >= 0) dR[rl] = Pf[roff];
+        for (int e = tid + nthr; e < RAW_T * RAW_T; e += nthr) {
             const int r = e / RAW_T, c = e - r * RAW_T;
             dR[r * RAW_P + c] = Pf[(long)min(max(Y0 - 2 + r, 0), h - 1) * w +
                                    min(max(X0 - 2 + c, 0), w - 1)];
         }
     };
 
-    // TLDS: this block's tap tables (every phase of its x/y class, contiguous on the host)
-    // are staged once in LDS and read with uniform-address ds_read_b128 in the tap loop,
-    // so the loop's loads are all in-order LDS traffic
-    Tap *lT = reinterpret_cast<Tap *>(lR + NBUF * RAW_SZ);
-    float *lWd = reinterpret_cast<float *>(lT + a.max_class_taps + 4);   // MODE 2
     const int tcls0 = PHS[0].first_tap;
     if (TLDS) {
         const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
-        for (int e = tid; e < tcls1 - tcls0 + 4; e += nthr) lT[e] = taps[tcls0 + e];
+        for (int e = tid; e < tcls1 - tcls0 + TAP_SPARE; e += nthr) lT[e] = taps[tcls0 + e];
         if (MODE == 2)
-            for (int e = tid; e < tcls1 - tcls0 + 4; e += nthr) lWd[e] = a.tapWd[tcls0 + e];
+            for (int e = tid; e < tcls1 - tcls0 + TAP_SPARE; e += nthr)
+                lWd[e] = a.tapWd[tcls0 + e];
     }
+    for (int e = tid; e < nU; e += nthr) {
+        const int pl = e / (a.PH * pw);
+        const int r = (e - pl * a.PH * pw) / pw;
+        const int c = e - pl * a.PH * pw - r * pw;
+        const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+        lPlan[e] = (pl * planesz + r * a.PP + c) | (gy << 15) | (gx << 23);
+    }
+    __syncthreads();
     fill_direct(a.frame0, lU, lR);
     __syncthreads();
 
@@ -589,9 +602,10 @@ template <int PPW, bool TLDS, int MODE>
 static hipError_t launch_ppw_m(const BicRgcArgs &a, hipStream_t s)
 {
     const size_t one = sizeof(float2) * 4 * a.PP * a.PH + sizeof(float) * RAW_SZ;
-    const size_t tl = TLDS ? (sizeof(Tap) + (MODE == 2 ? sizeof(float) : 0)) *
-                                 ((size_t)a.max_class_taps + 4)
-                           : 0;
+    const size_t tl = (TLDS ? (sizeof(Tap) + (MODE == 2 ? sizeof(float) : 0)) *
+                                  ((size_t)a.max_class_taps + TAP_SPARE)
+                            : 0) +
+                      sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));   // fill plan
     const int nthr = 64 * a.nw;
     const int nU = 4 * a.PH * (a.TW >> 1);
     const int need = (nU + nthr - 1) / nthr;
