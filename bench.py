@@ -51,9 +51,10 @@ def parse():
     ap.add_argument("--no-rse", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default RGC launch shape instead of esrrf_autotune's")
-    ap.add_argument("--mode", choices=("bic", "fht"), default="bic",
+    ap.add_argument("--mode", choices=("bic", "fht", "3d"), default="bic",
                     help="bic: the production workload (default); fht: the same stack through "
-                         "the Fourier-upsampling mode")
+                         "the Fourier-upsampling mode; 3d: 512 frames of a 9-plane 128x128 "
+                         "multi-focus montage, M=4")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain 1 GiB device copies (HBM counter calibration)")
     return ap.parse_args()
@@ -208,6 +209,72 @@ def bench_rse(pkg, stack):
                         "(1, 1, 7.5)"}
 
 
+def bench_3d(a, pkg, torch):
+    """Multi-focus workload: 512 frames of 9 focal planes of 128x128 (a 384x384 3x3 montage,
+    uint16), lateral and axial magnification 4, fwhm 1.5, S=2, MP on -> 36 x 512 x 512."""
+    import numpy as np
+    C = pkg._capi
+    from nanoj_esrrf_amd import synth
+    planes = synth.mfm_9x128x128x512()
+    mont = synth.montage3x3(planes)
+    nf, nP, M3 = mont.shape[0], planes.shape[1], 4
+    Wm, Hm = mont.shape[2], mont.shape[1]
+    z = np.zeros(nP)
+    cols = [np.ascontiguousarray(v, dtype=np.float64) for v in
+            (z, z, z, np.arange(nP, dtype=np.float64), np.ones(nP))]
+    dp = ctypes.POINTER(ctypes.c_double)
+    p = params(C, Wm, Hm, M3, FWHM, SENS, nf)
+    p.n_planes = nP
+    p.shift_x_3d, p.shift_y_3d, p.theta_3d, p.roi_3d, p.int_coeffs_3d = \
+        [c.ctypes.data_as(dp) for c in cols]
+    dev = torch.device("cuda:0")
+    d_frames = torch.from_numpy(mont.view(np.int16).reshape(-1)).to(dev)
+    nvox = 128 * M3 * 128 * M3 * nP * M3
+    d_out = torch.empty(4 * nvox, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    eng = pkg.Engine(p)
+    L, h = eng.L, eng.h
+    eng.check(L.esrrf_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
+    nact = ctypes.c_longlong()
+    eng.check(L.esrrf_active_samples(h, ctypes.byref(nact)))
+
+    def step():
+        eng.check(L.esrrf_reset_frame_position(h))
+        eng.check(L.esrrf_calculate_device(h, ctypes.c_void_p(d_frames.data_ptr()), C.ESRRF_U16,
+                                           nf, None, None))
+        eng.check(L.esrrf_read_result_device(h, ctypes.c_void_p(d_out.data_ptr())))
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    eng.check(L.esrrf_profile_enable(h, 1))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(a.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_step = ev0.elapsed_time(ev1) / a.steps
+    rgc_ms, rgc_n = ctypes.c_double(), ctypes.c_long()
+    eng.check(L.esrrf_profile_read(h, 0, ctypes.byref(rgc_ms), ctypes.byref(rgc_n)))
+    eng.close()
+    rgc_avg = rgc_ms.value / max(rgc_n.value, 1)
+    flops = 24.0 * nact.value * nf
+    return {"metric": "eSRRF 3-D output Mvoxel/s (x4 lateral and axial, 512 frames)",
+            "value": round(nvox * nf / (ms_step / 1e3) / 1e6, 1), "unit": "Mvoxel/s",
+            "ms_per_step": round(ms_step, 3), "steps": a.steps, "warmup": a.warmup,
+            "dtype": "f32", "data": "synthetic (seeded 3-D blinking filaments, uint16)",
+            "config": {"workload": "3-D eSRRF: 512 frames x 9 planes x 128x128 uint16 (3x3 "
+                                   "montage 384x384) -> 36 x 512 x 512, M=4, fwhm 1.5, S=2, MP on"},
+            "roofline": {"bound": "valu", "kernel": "k3d_rgc", "unit": "TFLOP/s",
+                         "achieved": round(flops / (rgc_avg / 1e3) / 1e12, 2),
+                         "peak": PEAK_FP32_TFLOPS,
+                         "frac": round(flops / (rgc_avg / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                         "kernel_ms": round(rgc_avg, 3),
+                         "gated_samples_per_frame": nact.value,
+                         "flop_convention": "24 per gated 3-D sample"}}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -224,6 +291,9 @@ def main():
     pkg = __graft_entry__.load()
     from nanoj_esrrf_amd import synth
     C = pkg._capi
+    if a.mode == "3d":
+        print(json.dumps(bench_3d(a, pkg, torch)))
+        return
 
     nf = a.frames
     stack = synth.production_256x256x1000(n_frames=nf)
