@@ -10,9 +10,11 @@ struct FhtPlan {
     int N, log2N, M, w, h;
     int scaleW, scaleH, xROIp, yROIp;   // mirror padding (FourierInterpolation.java:118-172)
     int xROI, yROI;                     // crop of the MN x MN result (:50-54)
-    float2 *F;                          // N x N spectrum scratch
-    float2 *Y;                          // M^2 x N x N twiddled spectra
+    int chunk;                          // frames per batch of the scratch spectra
+    float2 *F;                          // [chunk][N][N] spectra
+    float2 *Y;                          // [chunk][M][N][N] row-transformed twiddled spectra
     const float2 *tw;                   // N/2 twiddles e^{-2 pi i k / N}
+    const float2 *twMN;                 // M*N twiddles e^{-2 pi i j / (M N)}
 };
 
 struct FhtRgcArgs {
@@ -43,9 +45,9 @@ struct FhtGenericArgs {
     float radius, tSS, tSO;
 };
 
-hipError_t fft2_batched(float2 *data, int N, int log2N, int batches, const float2 *tw,
-                        hipStream_t s);
-hipError_t launch_fht_upsample(const FhtPlan &p, const float *frame, float *out, hipStream_t s);
+int fht_group(int N);
+hipError_t launch_fht_upsample(const FhtPlan &p, const float *frames, float *out, int nf,
+                               hipStream_t s);
 hipError_t launch_fht_prep(const float *px, float2 *U, int W, int H, int nF, hipStream_t s);
 hipError_t launch_fht_grad(const float *px, float *gx, float *gy, int W, int H, hipStream_t s);
 size_t fht_rgc_lds_bytes(int TH, int pitch, int ntaps);

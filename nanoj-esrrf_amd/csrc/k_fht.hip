@@ -8,8 +8,11 @@
 //   u  = (Re - Im) FFT2_MN(H') / (MN)^2                   (normalised inverse Hartley)
 // The MN transform of the sparse H' is never formed: writing x = M a + b, y = M c + d,
 //   FFT2_MN(H')[Mc+d][Ma+b] = FFT2_N( H(ky,kx) e^{-2 pi i (k(ky) d + k(kx) b)/(MN)} )[c][a]
-// with k() the signed frequency, so one frame costs 1 + M^2 batched N x N FFTs, all power
-// of two, done by a radix-2 Stockham kernel in LDS (G signals of one batch per block).
+// with k() the signed frequency.  The twiddle is separable, t_b(kx) t_d(ky), so the row
+// transforms are shared by the M values of d: per frame one forward N x N FFT, M row passes
+// and M^2 column passes of N-point FFTs, all power of two (radix-2 Stockham in LDS, G signals
+// per block), batched over chunks of frames, with the mirror padding fused into the first
+// load and the (Re - Im) crop-scatter into the last store.
 #include "device_interp.h"
 #include "esrrf_internal.h"
 #include "fht_internal.h"
@@ -19,49 +22,14 @@
 namespace esrrf {
 
 // ---------------------------------------------------------------------------------------
-// Mirror padding to the N x N square (FourierInterpolation.java:118-172); the identity
-// for power-of-two squares.
+// Radix-2 Stockham FFT (forward, e^{-i}) of G signals of length N held in LDS: A holds the
+// input, B is scratch; returns the buffer holding the result.  tw[k] = e^{-2 pi i k / N}.
 // ---------------------------------------------------------------------------------------
-__global__ void k_fht_pad(const float *__restrict__ src, float2 *__restrict__ dst, int w, int h,
-                          int N, int scaleW, int scaleH, int xROI, int yROI, long src_stride,
-                          long dst_stride)
+template <int G>
+__device__ __forceinline__ float2 *stockham(float2 *A, float2 *B, int N, int log2N,
+                                            const float2 *__restrict__ tw)
 {
-    const int f = blockIdx.y;
-    const long n = (long)N * N;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (long)gridDim.x * blockDim.x) {
-        const int yy = (int)(e / N), xx = (int)(e - (long)yy * N);
-        const int X = xx + xROI, Y = yy + yROI;
-        const int sW = X / w, i = X - sW * w;
-        const int sH = Y / h, j = Y - sH * h;
-        const int p = ((sW - (scaleW - 1) / 2) % 2 == 0) ? i : w - 1 - i;
-        const int q = ((sH - (scaleH - 1) / 2) % 2 == 0) ? j : h - 1 - j;
-        dst[dst_stride * f + e] = make_float2(src[src_stride * f + (long)q * w + p], 0.0f);
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// Batched radix-2 Stockham FFT (forward, e^{-i}).  Signal g of batch b, element k is at
-// base[b*bstride + g*gstride + k*kstride].  A block handles G consecutive signals.
-// ---------------------------------------------------------------------------------------
-template <int G, bool ROWS>
-__global__ void __launch_bounds__(256) k_fft(float2 *__restrict__ base, int N, int log2N,
-                                             long gstride, long kstride, long bstride,
-                                             const float2 *__restrict__ tw)
-{
-    extern __shared__ float2 sm[];
-    float2 *A = sm, *B = sm + G * N;
-    const int g0 = blockIdx.x * G;
-    float2 *data = base + bstride * blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int total = G * N;
-    // coalesced staging: along a row consecutive threads take consecutive elements, along
-    // columns consecutive signals
-    for (int e = tid; e < total; e += nthr) {
-        const int g = ROWS ? e / N : e % G, k = ROWS ? e % N : e / G;
-        A[g * N + k] = data[(long)(g0 + g) * gstride + (long)k * kstride];
-    }
-    __syncthreads();
     const int half = N >> 1;
     for (int L = 1, s = 0; s < log2N; L <<= 1, s++) {
         const int twstep = half / L;   // e^{-2 pi i k/(2L)} = tw[k * N/(2L)]
@@ -81,52 +49,124 @@ __global__ void __launch_bounds__(256) k_fft(float2 *__restrict__ base, int N, i
         A = B;
         B = t;
     }
-    for (int e = tid; e < total; e += nthr) {
-        const int g = ROWS ? e / N : e % G, k = ROWS ? e % N : e / G;
-        data[(long)(g0 + g) * gstride + (long)k * kstride] = A[g * N + k];
+    return A;
+}
+
+// twiddle e^{-2 pi i k(q) v / (MN)} of frequency index q (signed frequency k(q), Nyquist on
+// the negative side) from the table twMN[j] = e^{-2 pi i j / (MN)}
+__device__ __forceinline__ float2 twid(const float2 *__restrict__ twMN, int q, int v, int N, int MN)
+{
+    const int k = q < N / 2 ? q : q - N;
+    int j = (k * v) % MN;
+    if (j < 0) j += MN;
+    return twMN[j];
+}
+
+// Step 1: forward FFT along x of G rows of the mirror-padded frame
+// (FourierInterpolation.java:118-172; the identity for power-of-two squares).
+// grid (N/G, frames); F[f][y][kx]
+template <int G>
+__global__ void __launch_bounds__(256) k_fht_fwd_rows(const float *__restrict__ src, long src_stride,
+                                                      float2 *__restrict__ F, FhtPlan p)
+{
+    extern __shared__ float2 sm[];
+    const int N = p.N;
+    float2 *A = sm, *B = sm + G * N;
+    const int y0 = blockIdx.x * G, f = blockIdx.y;
+    const float *S = src + src_stride * f;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+        const int g = e / N, xx = e - g * N;
+        const int X = xx + p.xROIp, Y = y0 + g + p.yROIp;
+        const int sW = X / p.w, i = X - sW * p.w;
+        const int sH = Y / p.h, j = Y - sH * p.h;
+        const int px = ((sW - (p.scaleW - 1) / 2) % 2 == 0) ? i : p.w - 1 - i;
+        const int qy = ((sH - (p.scaleH - 1) / 2) % 2 == 0) ? j : p.h - 1 - j;
+        A[e] = make_float2(S[(long)qy * p.w + px], 0.0f);
+    }
+    __syncthreads();
+    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    float2 *Fo = F + (long)N * N * f + (long)y0 * N;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) Fo[e] = R[e];
+}
+
+// Step 2: forward FFT along y of G columns, in place. grid (N/G, frames)
+template <int G>
+__global__ void __launch_bounds__(256) k_fht_fwd_cols(float2 *__restrict__ F, FhtPlan p)
+{
+    extern __shared__ float2 sm[];
+    const int N = p.N;
+    float2 *A = sm, *B = sm + G * N;
+    const int a0 = blockIdx.x * G;
+    float2 *Ff = F + (long)N * N * blockIdx.y;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+        const int g = e % G, k = e / G;          // consecutive threads: consecutive columns
+        A[g * N + k] = Ff[(long)k * N + a0 + g];
+    }
+    __syncthreads();
+    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+        const int g = e % G, k = e / G;
+        Ff[(long)k * N + a0 + g] = R[g * N + k];
     }
 }
 
-// Forward spectrum -> Hartley transform H = Re F - Im F, then the M^2 twiddled copies
-// Y_bd(ky, kx) = H e^{-2 pi i (k(kx) b + k(ky) d)/(MN)} (batch index = d*M + b).
-__global__ void k_fht_twiddle(const float2 *__restrict__ F, float2 *__restrict__ Y, int N, int M)
+// Step 3: Hartley spectrum H = Re F - Im F of G rows (ky), twiddled by t_b(kx), FFT along
+// kx -> R[f][b][ky][a].  grid (N/G, frames * M), b = blockIdx.y % M
+template <int G>
+__global__ void __launch_bounds__(256) k_fht_inv_rows(const float2 *__restrict__ F,
+                                                      float2 *__restrict__ Rb, FhtPlan p)
 {
-    const long n = (long)N * N;
-    const int ph = blockIdx.y;
-    const int d = ph / M, b = ph - d * M;
-    const double scale = -2.0 * 3.14159265358979323846 / ((double)M * N);
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (long)gridDim.x * blockDim.x) {
-        const int ky = (int)(e / N), kx = (int)(e - (long)ky * N);
-        const int sx = kx < N / 2 ? kx : kx - N;
-        const int sy = ky < N / 2 ? ky : ky - N;
-        const float2 f = F[e];
-        const float hv = f.x - f.y;
-        // phase reduced modulo MN in integers, then one sincos in double for accuracy
-        const long MN = (long)M * N;
-        long ip = ((long)sx * b + (long)sy * d) % MN;
-        if (ip < 0) ip += MN;
-        double s, c;
-        sincos(scale * (double)ip, &s, &c);
-        Y[(long)ph * n + e] = make_float2((float)(hv * c), (float)(hv * s));
+    extern __shared__ float2 sm[];
+    const int N = p.N, M = p.M, MN = p.M * p.N;
+    float2 *A = sm, *B = sm + G * N;
+    const int y0 = blockIdx.x * G;
+    const int f = blockIdx.y / M, b = blockIdx.y - f * M;
+    const float2 *Fr = F + (long)N * N * f + (long)y0 * N;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+        const int kx = e % N;
+        const float2 v = Fr[e];
+        const float hv = v.x - v.y;
+        const float2 t = twid(p.twMN, kx, b, N, MN);
+        A[e] = make_float2(hv * t.x, hv * t.y);
     }
+    __syncthreads();
+    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    float2 *Ro = Rb + ((long)f * M + b) * N * N + (long)y0 * N;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) Ro[e] = R[e];
 }
 
-// (Re - Im) / (MN)^2 of phase (b, d) at (a, c) -> magnified pixel (M a + b - xROI,
-// M c + d - yROI) of the cropped frame.
-__global__ void k_fht_scatter(const float2 *__restrict__ Z, float *__restrict__ out, int N,
-                              int M, int wM, int hM, int xROI, int yROI, float inv)
+// Step 4: column a of R_b twiddled by t_d(ky), FFT along ky: element c is the inverse-DHT
+// sample at (M c + d, M a + b) of the MN x MN grid; (Re - Im)/(MN)^2 cropped into the
+// magnified frame.  grid (N/G, frames * M * M), (b, d) from blockIdx.y
+template <int G>
+__global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__ Rb,
+                                                      float *__restrict__ out, long out_stride,
+                                                      FhtPlan p, float inv)
 {
-    const long n = (long)N * N;
-    const int ph = blockIdx.y;
-    const int d = ph / M, b = ph - d * M;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(e / N), a = (int)(e - (long)c * N);
-        const int X = M * a + b - xROI, Y = M * c + d - yROI;
+    extern __shared__ float2 sm[];
+    const int N = p.N, M = p.M, MN = p.M * p.N;
+    const int wM = p.M * p.w, hM = p.M * p.h;
+    float2 *A = sm, *B = sm + G * N;
+    const int a0 = blockIdx.x * G;
+    const int MM = M * M;
+    const int f = blockIdx.y / MM, bd = blockIdx.y - f * MM;
+    const int b = bd / M, d = bd - b * M;
+    const float2 *Rc = Rb + ((long)f * M + b) * N * N;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+        const int g = e % G, k = e / G;
+        const float2 v = Rc[(long)k * N + a0 + g];
+        const float2 t = twid(p.twMN, k, d, N, MN);
+        A[g * N + k] = make_float2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
+    }
+    __syncthreads();
+    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    float *O = out + out_stride * f;
+    for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+        const int g = e % G, c = e / G;
+        const int X = M * (a0 + g) + b - p.xROI, Y = M * c + d - p.yROI;
         if (X < 0 || X >= wM || Y < 0 || Y >= hM) continue;
-        const float2 z = Z[(long)ph * n + e];
-        out[(long)Y * wM + X] = (z.x - z.y) * inv;
+        const float2 z = R[g * N + c];
+        O[(long)Y * wM + X] = (z.x - z.y) * inv;
     }
 }
 
@@ -136,48 +176,44 @@ static int grid_of(long n)
     return (int)(g > 4096 ? 4096 : (g < 1 ? 1 : g));
 }
 
-template <bool ROWS>
-static void fft_pass(float2 *data, int N, int log2N, int batches, const float2 *tw, hipStream_t s)
+template <int G>
+static void upsample_g(const FhtPlan &p, const float *frames, float *out, int nf, hipStream_t s)
 {
-    const long n = (long)N * N;
-    const long gs = ROWS ? N : 1, ks = ROWS ? 1 : N;
-    // G signals per block (G <= N so the grid N / G is whole) with 2*G*N complex values of
-    // LDS <= 64 KiB
-    const int G = std::max(1, std::min(std::min(16, N), 4096 / N));
+    const int N = p.N, MM = p.M * p.M;
     const size_t lds = 2 * sizeof(float2) * (size_t)G * N;
-    const dim3 grid(N / G, batches);
-    switch (G) {
-    case 16: hipLaunchKernelGGL((k_fft<16, ROWS>), grid, dim3(256), lds, s, data, N, log2N, gs, ks, n, tw); break;
-    case 8: hipLaunchKernelGGL((k_fft<8, ROWS>), grid, dim3(256), lds, s, data, N, log2N, gs, ks, n, tw); break;
-    case 4: hipLaunchKernelGGL((k_fft<4, ROWS>), grid, dim3(256), lds, s, data, N, log2N, gs, ks, n, tw); break;
-    case 2: hipLaunchKernelGGL((k_fft<2, ROWS>), grid, dim3(256), lds, s, data, N, log2N, gs, ks, n, tw); break;
-    default: hipLaunchKernelGGL((k_fft<1, ROWS>), grid, dim3(256), lds, s, data, N, log2N, gs, ks, n, tw); break;
+    const long npx = (long)p.w * p.h, nM = (long)MM * npx;
+    const float inv = (float)(1.0 / ((double)p.M * p.N * (double)p.M * p.N));
+    for (int f0 = 0; f0 < nf; f0 += p.chunk) {
+        const int k = std::min(p.chunk, nf - f0);
+        hipLaunchKernelGGL((k_fht_fwd_rows<G>), dim3(N / G, k), dim3(256), lds, s,
+                           frames + npx * f0, npx, p.F, p);
+        hipLaunchKernelGGL((k_fht_fwd_cols<G>), dim3(N / G, k), dim3(256), lds, s, p.F, p);
+        hipLaunchKernelGGL((k_fht_inv_rows<G>), dim3(N / G, k * p.M), dim3(256), lds, s, p.F,
+                           p.Y, p);
+        hipLaunchKernelGGL((k_fht_inv_cols<G>), dim3(N / G, k * MM), dim3(256), lds, s, p.Y,
+                           out + nM * f0, nM, p, inv);
     }
 }
 
-// 2-D FFT of `batches` consecutive N x N complex arrays (N a power of two, 2..4096).
-hipError_t fft2_batched(float2 *data, int N, int log2N, int batches, const float2 *tw,
-                        hipStream_t s)
+int fht_group(int N)
 {
-    fft_pass<true>(data, N, log2N, batches, tw, s);
-    fft_pass<false>(data, N, log2N, batches, tw, s);
-    return hipGetLastError();
+    // G signals per block (G <= N so the grid N / G is whole) with 2*G*N complex values of
+    // LDS <= 64 KiB
+    return std::max(1, std::min(std::min(16, N), 4096 / N));
 }
 
-hipError_t launch_fht_upsample(const FhtPlan &p, const float *frame, float *out, hipStream_t s)
+// Upsample nf consecutive raw frames (float, w x h each) into nf magnified frames.
+hipError_t launch_fht_upsample(const FhtPlan &p, const float *frames, float *out, int nf,
+                               hipStream_t s)
 {
-    const long n = (long)p.N * p.N;
-    hipLaunchKernelGGL(k_fht_pad, dim3(grid_of(n), 1), dim3(256), 0, s, frame, p.F, p.w, p.h,
-                       p.N, p.scaleW, p.scaleH, p.xROIp, p.yROIp, 0L, 0L);
-    hipError_t st = fft2_batched(p.F, p.N, p.log2N, 1, p.tw, s);
-    if (st != hipSuccess) return st;
-    const int MM = p.M * p.M;
-    hipLaunchKernelGGL(k_fht_twiddle, dim3(grid_of(n), MM), dim3(256), 0, s, p.F, p.Y, p.N, p.M);
-    st = fft2_batched(p.Y, p.N, p.log2N, MM, p.tw, s);
-    if (st != hipSuccess) return st;
-    const float inv = (float)(1.0 / ((double)p.M * p.N * (double)p.M * p.N));
-    hipLaunchKernelGGL(k_fht_scatter, dim3(grid_of(n), MM), dim3(256), 0, s, p.Y, out, p.N, p.M,
-                       p.M * p.w, p.M * p.h, p.xROI, p.yROI, inv);
+    if (nf <= 0) return hipSuccess;
+    switch (fht_group(p.N)) {
+    case 16: upsample_g<16>(p, frames, out, nf, s); break;
+    case 8: upsample_g<8>(p, frames, out, nf, s); break;
+    case 4: upsample_g<4>(p, frames, out, nf, s); break;
+    case 2: upsample_g<2>(p, frames, out, nf, s); break;
+    default: upsample_g<1>(p, frames, out, nf, s); break;
+    }
     return hipGetLastError();
 }
 

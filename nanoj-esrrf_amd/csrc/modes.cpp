@@ -133,20 +133,31 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
             p.yROIp = (sH * c.h - N) / 2;
             p.xROI = (c.M * N - c.M * c.w) / 2;
             p.yROI = (c.M * N - c.M * c.h) / 2;
-            std::vector<float2> tw(std::max(N / 2, 1));
+            // twiddles e^{-2 pi i k/N} (k < N/2) then e^{-2 pi i j/(MN)} (j < MN), in double
+            const int MN = c.M * N;
+            std::vector<float2> tw(N / 2 + MN);
             for (int k = 0; k < N / 2; k++) {
                 const double ang = -2.0 * M_PI * k / N;
                 tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
             }
+            for (int j = 0; j < MN; j++) {
+                const double ang = -2.0 * M_PI * j / MN;
+                tw[N / 2 + j] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+            }
+            // frames per batch: spectra of (M + 1) N x N complex per frame within 1 GiB
+            const size_t per = (size_t)(c.M + 1) * N * N * sizeof(float2);
+            p.chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)c.nF, (1ull << 30) / per));
+            p.chunk = std::min(p.chunk, 65535 / (c.M * c.M));   // grid.y = chunk * M^2
             float2 *dtw = nullptr;
-            if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N) ||
-                !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * c.M) ||
+            if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N * p.chunk) ||
+                !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * p.chunk) ||
                 !dmalloc(&dtw, sizeof(float2) * tw.size()))
                 return err(ms, ESRRF_ERR_NOMEM, "FHT spectra: hipMalloc failed");
             hipError_t st = hipMemcpy(dtw, tw.data(), sizeof(float2) * tw.size(),
                                       hipMemcpyHostToDevice);
             if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT twiddles", st);
             p.tw = dtw;
+            p.twMN = dtw + N / 2;
         }
         build_fht_taps(ms, c.radius, c.tSS, c.tSO);
         if (!dmalloc(&ms.d_taps, sizeof(Tap) * ms.h_taps.size()) ||
@@ -186,10 +197,9 @@ int mode_prepare(int mode, const ModeCtx &c, ModeState &ms, const void *src, int
         (void)hipEventRecord(e0, c.s);
     }
     if (c.M > 1) {
-        for (int f = 0; f < n; f++) {
-            st = launch_fht_upsample(ms.plan, ms.d_rawf + npx * f, c.d_px + nM * f, c.s);
-            if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT upsampling", st);
-        }
+        st = launch_fht_upsample(ms.plan, ms.d_rawf, c.d_px, n, c.s);
+        if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT upsampling", st);
+        (void)nM;
     } else {
         st = hipMemcpyAsync(c.d_px, ms.d_rawf, sizeof(float) * npx * n, hipMemcpyDeviceToDevice,
                             c.s);
