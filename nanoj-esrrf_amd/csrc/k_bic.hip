@@ -250,6 +250,7 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
     // U buffers: two when the next frame is staged in registers, one otherwise (then the
     // block refills it between two barriers)
     constexpr int NBUF = STAGE_U > 0 ? 2 : 1;
+    constexpr int PH_UNROLL = (PPW <= 4 && MODE != 2) ? PPW : 1;
     float *lR = reinterpret_cast<float *>(lU + NBUF * usz);
 
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -431,7 +432,9 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
         const Phase *PHf = PHS + zf;
         const Tap *TAPf = taps + zf;
 
-#pragma unroll 1
+        // Unrolled, the per-phase state (s_avg[p] ...) has static register indices; rolled,
+        // every access becomes a chain of selects. Unroll where the registers fit.
+#pragma unroll PH_UNROLL
         for (int p = 0; p < PPW; p++) {
             const int ph = wave + p * a.nw;
             if (ph < MM) {
