@@ -59,6 +59,18 @@ static bool tlds_fits(const BicTables &t, bool adj)
            48 * 1024;
 }
 
+// LDS of the frame-pair kernel for this geometry within the 64 KiB a block may take by
+// default
+static bool pair_fits(const BicTables &t)
+{
+    BicRgcArgs a{};
+    a.PP = t.geo.PP;
+    a.PH = t.geo.PH;
+    a.TW = t.geo.TW;
+    a.max_class_taps = t.geo.max_class_taps;
+    return bic_pair_lds_bytes(a) <= 64 * 1024;
+}
+
 BicShape bic_default_shape(const BicTables &t, int M, bool adj)
 {
     const int MM = M * M;
@@ -68,6 +80,8 @@ BicShape bic_default_shape(const BicTables &t, int M, bool adj)
     s.nw = nw;
     s.ppw = (MM + nw - 1) / nw;
     s.taps_in_lds = tlds_fits(t, adj) ? 1 : 0;
+    // frame pairs: one tap read and one address add per two samples
+    if (s.taps_in_lds && !adj && s.ppw <= 4 && pair_fits(t)) s.fp = 2;
     return s;
 }
 
@@ -88,6 +102,10 @@ std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj)
             s.ppw = ppw;
             s.taps_in_lds = tl;
             out.push_back(s);
+            if (tl == 1 && !adj && ppw <= 4 && pair_fits(t)) {
+                s.fp = 2;
+                out.push_back(s);
+            }
         }
     }
     return out;
@@ -123,6 +141,7 @@ BicRgcArgs bic_args(const BicTables &t, const BicShape &sh, const float2 *U, con
     a.nw = sh.nw;
     a.stage = 0;
     a.taps_in_lds = sh.taps_in_lds;
+    a.frames_per_pass = sh.fp;
     a.max_class_taps = t.geo.max_class_taps;
     a.tac_prev_scaled = 1;
     return a;
@@ -145,6 +164,7 @@ hipError_t bic_autotune(const BicTables &t, BicRgcArgs a, int M, int reps, hipSt
     for (BicShape c : bic_candidate_shapes(t, M, a.mode == 2)) {
         a.nw = c.nw;
         a.taps_in_lds = c.taps_in_lds;
+        a.frames_per_pass = c.fp;
         e = launch_bic_rgc(a, c.ppw, s);   // warm-up
         if (e == hipSuccess) e = hipEventRecord(e0, s);
         for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_bic_rgc(a, c.ppw, s);

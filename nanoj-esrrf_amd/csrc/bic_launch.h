@@ -16,6 +16,7 @@ namespace esrrf {
 struct BicShape {
     int nw = 1, ppw = 1;
     int taps_in_lds = 0;
+    int fp = 1;           // frames per pass: 2 = frame-pair kernel (taps in LDS, ppw <= 4)
     float ms = 0.0f;      // measured time of the autotuner's sample launch (0: not tuned)
 };
 
@@ -37,9 +38,11 @@ void bic_tables_free(BicTables &t);
 
 // Default shape: waves per block a multiple of 4 (a block's waves are dealt round-robin
 // to the CU's four SIMDs), about 4 phases per wave, at most 16 waves and 8 phases per wave;
-// tap tables in LDS when a class's tables stay within 48 KiB (adj: with dW/dtSS).
+// tap tables in LDS when a class's tables stay within 48 KiB (adj: with dW/dtSS); frame
+// pairs wherever that kernel applies.
 BicShape bic_default_shape(const BicTables &t, int M, bool adj = false);
-// All shapes the kernel supports for M (the autotuner's candidates).
+// All shapes the kernel supports for M (the autotuner's candidates), frame-pair shapes
+// included where their LDS tiles and tables fit in 64 KiB (not for the adjoint).
 std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj = false);
 
 // Arguments common to every launch over unshifted frames (out, prev, frames, cumulant

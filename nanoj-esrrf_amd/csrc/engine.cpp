@@ -492,6 +492,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
                 }
             }
             if (const char *ov = getenv("ESRRF_BIC_TLDS")) e->shape.taps_in_lds = atoi(ov) != 0;
+            if (const char *ov = getenv("ESRRF_BIC_FP")) e->shape.fp = atoi(ov) == 2 ? 2 : 1;
             e->stage = 0;
             if (const char *ov = getenv("ESRRF_BIC_STAGE")) e->stage = atoi(ov) != 0;
         }

@@ -60,6 +60,7 @@ struct BicRgcArgs {
     int stage;             // 1: stage the next frame's tiles in registers
     int taps_in_lds;       // 1: stage each block's tap tables in LDS
     int max_class_taps;    // largest tap-table size of one x/y class (LDS sizing)
+    int frames_per_pass;   // 2: frame-pair kernel (modes 0, 1 with taps in LDS), else 1
     // kernel variant: 0 = accumulate the four cumulants (liveSRRF.cl:363-393);
     // 1 = emit the normalised CGLH (before the sensitivity power) and the intensity of
     //     every pixel and frame, phase-major [frame][M*M][h][w], into emitC / emitV;
@@ -92,6 +93,7 @@ hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hip
 hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h, int f,
                            hipStream_t s);
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s);
+size_t bic_pair_lds_bytes(const BicRgcArgs &a);   // LDS of the frame-pair kernel
 hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
 hipError_t launch_var(float *out, long nvox, hipStream_t s);
 hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,

@@ -601,6 +601,274 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Frame-pair variant of the tiled RGC (modes 0 and 1, tap tables in LDS).  The block holds
+// two consecutive frames: the U tile interleaved as float4 (Ux, Uy of frame f, Ux, Uy of
+// frame f+1) and the raw tiles as two planes RAW_SZ apart.  One uniform tap read, one
+// address add and one 16-byte sample read then serve both frames, and there is one pair of
+// barriers per two frames.  The cumulants are updated frame by frame in order, so results
+// are those of k_bic_rgc bit for bit.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, float &in,
+                                         float &prv, float C0, float v, bool first,
+                                         const BicRgcArgs &a)
+{
+    // temporal accumulation (liveSRRF.cl:363-393)
+    const float C = apply_sens(C0, a.sens, a.sens_int);
+    const bool intw = a.int_weighting != 0;
+    const float t = intw ? v * C : C;
+    const float e2 = intw ? (t * v) * C : C * C;
+    if (first) {
+        avg = t * a.rn;
+        e2s = e2 * a.rn;
+        tac = 0.0f;
+        in = v * a.rn;
+    } else {
+        avg = avg + t * a.rn;
+        e2s = e2s + e2 * a.rn;
+        tac = tac + (t * prv) * a.rn1;
+        in = in + v * a.rn;
+    }
+    prv = a.tac_prev_scaled ? t * a.rn1 : t;
+}
+
+template <int PPW, int MODE>
+__global__ void __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
+                                                   const Tap *__restrict__ taps,
+                                                   const Phase *__restrict__ phases,
+                                                   const int *__restrict__ tapk)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4 *lU = reinterpret_cast<float4 *>(smem);
+    const int usz = 4 * a.PP * a.PH;
+    float *lR = reinterpret_cast<float *>(lU + usz);   // frame f, then frame f+1 at +RAW_SZ
+    Tap *lT = reinterpret_cast<Tap *>(lR + 2 * RAW_SZ);
+    int *lPlan = reinterpret_cast<int *>(lT + a.max_class_taps + TAP_SPARE);
+    constexpr int PH_UNROLL = PPW <= 4 ? PPW : 1;
+
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lx = lane & 7, ly = lane >> 3;
+    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int X0 = tX << 3, Y0 = tY << 3;
+    const int X = X0 + lx, Y = Y0 + ly;
+    const int M = a.M, MM = M * M;
+    const int w = a.w, h = a.h;
+    const Phase *PHS = phases + (axis_class(Y0, a.xs_log2) * a.ncls_x +
+                                   axis_class(X0, a.xs_log2)) * MM;
+    const bool interior = (2 * X0 + a.R2lo >= 1) && (2 * (X0 + 7) + 1 + a.R2hi <= 2 * w - 1) &&
+                          (2 * Y0 + a.R2lo >= 1) && (2 * (Y0 + 7) + 1 + a.R2hi <= 2 * h - 1);
+    const bool inimg = (X < w) && (Y < h);
+    const long whM = (long)a.wM * a.hM;
+
+    float s_avg[PPW], s_e2[PPW], s_tac[PPW], s_int[PPW], s_prv[PPW];
+    float invw[PPW];
+    int ibase[PPW];
+#pragma unroll
+    for (int p = 0; p < PPW; p++) {
+        const int ph = wave + p * a.nw;
+        invw[p] = 0.0f;
+        ibase[p] = -1;
+        s_avg[p] = s_e2[p] = s_tac[p] = s_int[p] = s_prv[p] = 0.0f;
+        if (ph < MM) {
+            const Phase P = PHS[ph];
+            float ws = P.wsum;
+            if (!interior) {
+                // border: sum only the taps inside the 0 < v < size gates, reference order
+                ws = 0.0f;
+                for (int t = 0; t < P.ntaps; t++) {
+                    const int k = tapk[P.first_tap + t];
+                    const int kx = 2 * X + (k & 255) - 128;
+                    const int ky = 2 * Y + ((k >> 8) & 255) - 128;
+                    if (kx >= 1 && kx <= 2 * w - 1 && ky >= 1 && ky <= 2 * h - 1)
+                        ws += taps[P.first_tap + t].W;
+                }
+            }
+            invw[p] = 1.0f / ws;
+            const int u0 = X + P.du, v0 = Y + P.dv;
+            if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2)
+                ibase[p] = (v0 - 1 - (Y0 - 2)) * RAW_P + (u0 - 1 - (X0 - 2));
+            if (MODE == 0 && !a.first_init && inimg) {
+                const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
+                s_avg[p] = a.out[pix];
+                s_e2[p] = a.out[pix + whM];
+                s_tac[p] = a.out[pix + 2 * whM];
+                s_int[p] = a.out[pix + 3 * whM];
+                s_prv[p] = a.prev[pix];
+            }
+        }
+    }
+
+    const int wI = 2 * w, hI = 2 * h;
+    const long whI = (long)wI * hI, wh = (long)w * h;
+    const int kxo = 2 * X0 - a.G0, kyo = 2 * Y0 - a.G0;
+    const int planesz = a.PP * a.PH;
+    const int base = ly * a.PP + lx;
+    const int pw = a.TW >> 1;
+    const int nU = 4 * a.PH * pw;
+
+    // fill plan of the U tile (plane-major, as in k_bic_rgc) and the class's tap tables
+    for (int e = tid; e < nU; e += nthr) {
+        const int pl = e / (a.PH * pw);
+        const int r = (e - pl * a.PH * pw) / pw;
+        const int c = e - pl * a.PH * pw - r * pw;
+        const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+        lPlan[e] = (pl * planesz + r * a.PP + c) | (gy << 15) | (gx << 23);
+    }
+    const int tcls0 = PHS[0].first_tap;
+    {
+        const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
+        for (int e = tid; e < tcls1 - tcls0 + TAP_SPARE; e += nthr) lT[e] = taps[tcls0 + e];
+    }
+    __syncthreads();
+
+    // frames f and f+1 (f+1 only when `two`) into the LDS tiles
+    auto fill = [&](int f, bool two) {
+        const float2 *Ua = a.U + whI * f;
+        for (int e = tid; e < nU; e += nthr) {
+            const int pk = lPlan[e];
+            const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
+            const int kx = kxo + gx, ky = kyo + gy;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI) {
+                const long g = (long)ky * wI + kx;
+                const float2 ua = Ua[g];
+                const float2 ub = two ? Ua[g + whI] : make_float2(0.0f, 0.0f);
+                v = make_float4(ua.x, ua.y, ub.x, ub.y);
+            }
+            lU[pk & 0x7fff] = v;
+        }
+        const float *Pa = a.px + wh * f;
+        for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
+            const int r = e / RAW_T, c = e - r * RAW_T;
+            const long g = (long)min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+            lR[r * RAW_P + c] = Pa[g];
+            lR[RAW_SZ + r * RAW_P + c] = two ? Pa[g + wh] : 0.0f;
+        }
+    };
+
+    fill(a.frame0, a.nframes > 1);
+    __syncthreads();
+
+    for (int fi = 0; fi < a.nframes; fi += 2) {
+        const bool two = fi + 1 < a.nframes;
+        const bool first = (fi == 0) && a.first_init;
+        // (a.zero is 0 at run time; it keeps the per-phase table reads inside the frame loop)
+        const int zf = a.zero * fi;
+        const Phase *PHf = PHS + zf;
+
+#pragma unroll PH_UNROLL
+        for (int p = 0; p < PPW; p++) {
+            const int ph = wave + p * a.nw;
+            if (ph < MM) {
+                const Phase P = PHf[ph];
+                float acc0 = 0.0f, acc1 = 0.0f;
+                const Tap *Tl = lT + (P.first_tap - tcls0);
+                for (int t = 0; t < P.ntaps; t += 4) {
+                    const Tap c0 = Tl[t], c1 = Tl[t + 1], c2 = Tl[t + 2], c3 = Tl[t + 3];
+                    const float4 u0 = lU[base + c0.off];
+                    const float4 u1 = lU[base + c1.off];
+                    const float4 u2 = lU[base + c2.off];
+                    const float4 u3 = lU[base + c3.off];
+                    acc0 = tap_acc(make_float2(u0.x, u0.y), c0, acc0);
+                    acc1 = tap_acc(make_float2(u0.z, u0.w), c0, acc1);
+                    acc0 = tap_acc(make_float2(u1.x, u1.y), c1, acc0);
+                    acc1 = tap_acc(make_float2(u1.z, u1.w), c1, acc1);
+                    acc0 = tap_acc(make_float2(u2.x, u2.y), c2, acc0);
+                    acc1 = tap_acc(make_float2(u2.z, u2.w), c2, acc1);
+                    acc0 = tap_acc(make_float2(u3.x, u3.y), c3, acc0);
+                    acc1 = tap_acc(make_float2(u3.z, u3.w), c3, acc1);
+                }
+                const float C0a = (P.wsum + acc0) * invw[p];
+                const float C0b = (P.wsum + acc1) * invw[p];
+
+                // interpolated intensity at (xM/M - 0.5, yM/M - 0.5) (liveSRRF.cl:361)
+                float va, vb;
+                if (ibase[p] >= 0) {
+                    const float *row = lR + ibase[p];
+                    float qa = 0.0f, qb = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        float pa = 0.0f, pb = 0.0f;
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            pa = pa + row[j * RAW_P + i] * P.wx[i];
+                            pb = pb + row[RAW_SZ + j * RAW_P + i] * P.wx[i];
+                        }
+                        qa = qa + pa * P.wy[j];
+                        qb = qb + pb * P.wy[j];
+                    }
+                    va = qa;
+                    vb = qb;
+                } else {
+                    const int xM = (X + zf) * M + P.sx, yM = (Y + zf) * M + P.sy;
+                    va = border_intensity(lR, X0, Y0, w, h, M, xM, yM);
+                    vb = border_intensity(lR + RAW_SZ, X0, Y0, w, h, M, xM, yM);
+                }
+
+                if (MODE == 1) {
+                    if (inimg) {
+                        const long e = (((long)(a.frame0 + fi) * MM + ph) * h + Y) * w + X;
+                        a.emitC[e] = C0a;
+                        a.emitV[e] = va;
+                        if (two) {
+                            a.emitC[e + (long)MM * wh] = C0b;
+                            a.emitV[e + (long)MM * wh] = vb;
+                        }
+                    }
+                    continue;
+                }
+                cumulate(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0a, va, first, a);
+                if (two)
+                    cumulate(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0b, vb, false, a);
+            }
+        }
+        if (fi + 2 < a.nframes) {
+            __syncthreads();   // every wave is done with this pair's tiles
+            fill(a.frame0 + fi + 2, fi + 3 < a.nframes);
+            __syncthreads();
+        }
+    }
+
+    if (MODE == 0 && inimg) {
+#pragma unroll
+        for (int p = 0; p < PPW; p++) {
+            const int ph = wave + p * a.nw;
+            if (ph >= MM) continue;
+            const Phase P = PHS[ph];
+            const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
+            a.out[pix] = s_avg[p];
+            a.out[pix + whM] = s_e2[p];
+            a.out[pix + 2 * whM] = s_tac[p];
+            a.out[pix + 3 * whM] = s_int[p];
+            a.prev[pix] = s_prv[p];
+        }
+    }
+}
+
+size_t bic_pair_lds_bytes(const BicRgcArgs &a)
+{
+    return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float) * 2 * RAW_SZ +
+           sizeof(Tap) * ((size_t)a.max_class_taps + TAP_SPARE) +
+           sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
+}
+
+template <int PPW, int MODE>
+static hipError_t launch_pair_m(const BicRgcArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE>), dim3(a.tiles_x * a.tiles_y), dim3(64 * a.nw),
+                       bic_pair_lds_bytes(a), s, a, a.taps, a.phases, a.tapk);
+    return hipGetLastError();
+}
+
+template <int PPW>
+static hipError_t launch_pair(const BicRgcArgs &a, hipStream_t s)
+{
+    return a.mode == 1 ? launch_pair_m<PPW, 1>(a, s) : launch_pair_m<PPW, 0>(a, s);
+}
+
 template <int PPW, bool TLDS, int MODE>
 static hipError_t launch_ppw_m(const BicRgcArgs &a, hipStream_t s)
 {
@@ -646,6 +914,15 @@ static hipError_t launch_ppw(const BicRgcArgs &a, hipStream_t s)
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s)
 {
     if (a.nframes <= 0) return hipSuccess;
+    if (a.frames_per_pass == 2 && a.mode != 2 && a.taps_in_lds && ppw <= 4 &&
+        bic_pair_lds_bytes(a) <= 64 * 1024) {
+        switch (ppw) {
+        case 1: return launch_pair<1>(a, s);
+        case 2: return launch_pair<2>(a, s);
+        case 3: return launch_pair<3>(a, s);
+        default: return launch_pair<4>(a, s);
+        }
+    }
     switch (ppw) {
     case 1: return launch_ppw<1>(a, s);
     case 2: return launch_ppw<2>(a, s);

@@ -166,6 +166,17 @@ int enqueue_run(esrrf_sweep *s, int dtype)
     return ESRRF_OK;
 }
 
+// One shape serves every fwhm of a magnification: keep the LDS tap tables and the frame
+// pairs only where every fwhm's tables fit them.
+static void clamp_shape(esrrf_sweep *s, size_t im)
+{
+    for (size_t fi = 0; fi < s->fwhm.size(); fi++) {
+        const BicShape d = bic_default_shape(s->tabs[im][fi], s->Ms[im]);
+        if (!d.taps_in_lds) s->shape[im].taps_in_lds = 0;
+        if (d.fp != 2 || !s->shape[im].taps_in_lds) s->shape[im].fp = 1;
+    }
+}
+
 int tune(esrrf_sweep *s)
 {
     // per magnification: the emit launch of the first fwhm over up to 32 frames
@@ -186,6 +197,7 @@ int tune(esrrf_sweep *s)
         BicShape best;
         SCHK(s, bic_autotune(t, a, M, 2, s->stream, &best, nullptr));
         s->shape[im] = best;
+        clamp_shape(s, im);
     }
     return ESRRF_OK;
 }
@@ -316,9 +328,7 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
             if (!s->tabs[im][fi].ok) return bad(ESRRF_ERR_UNSUPPORTED);
         }
         s->shape[im] = bic_default_shape(s->tabs[im][0], M);
-        // one shape per magnification: the largest fwhm decides whether the tables fit LDS
-        for (size_t fi = 1; fi < s->fwhm.size(); fi++)
-            if (!bic_default_shape(s->tabs[im][fi], M).taps_in_lds) s->shape[im].taps_in_lds = 0;
+        clamp_shape(s, im);
         if (!alloc((void **)&s->d_out[im], sizeof(float) * nc * 4 * wh * M * M) ||
             !alloc((void **)&s->d_mp[im], sizeof(float) * nc * 3 * M * M))
             return bad(ESRRF_ERR_NOMEM);

@@ -79,6 +79,24 @@ def test_bic_configs_parity(pkg, O, M, fwhm, S, w, h):
                   f"M{M} fwhm{fwhm} S{S} {w}x{h}")
 
 
+@pytest.mark.parametrize("M,w,h,loads", [
+    (5, 40, 32, [(0, 5), (5, 9)]), (4, 27, 19, [(0, 7)]), (2, 24, 24, [(0, 1), (1, 4)]),
+    (7, 16, 24, [(0, 6)]),
+])
+def test_bic_frame_pair_kernel_matches_single_frame(pkg, monkeypatch, M, w, h, loads):
+    """The frame-pair kernel (two frames per LDS pass, the default) reproduces the
+    single-frame kernel bit for bit: odd tails, one-frame loads and continuation across
+    loads included."""
+    n = loads[-1][1]
+    st = stack(n, w, h, seed=40 + M)
+    res = {}
+    for fp in ("1", "2"):
+        monkeypatch.setenv("ESRRF_BIC_FP", fp)
+        res[fp], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
+                                   nFor=n, loads=loads)
+    assert np.array_equal(res["1"], res["2"])
+
+
 def test_bic_no_intensity_weighting(pkg, O):
     st = stack(6, 24, 20, seed=5)
     got, _, _ = run_engine(pkg, st, 4, 1.5, 2, intW=False)
