@@ -64,6 +64,43 @@ __device__ __forceinline__ float interp_tile(const float *t, int pitch, int ox, 
     return q;
 }
 
+// interp_tile on a plane whose pixels are every S-th float of the tile (one plane of S
+// interleaved planes), same operations.
+template <int S>
+__device__ __forceinline__ float interp_tile_s(const float *t, int pitch, int ox, int oy, int w,
+                                               int h, float x, float y)
+{
+    const int u0 = (int)floorf(x);
+    const int v0 = (int)floorf(y);
+    float q = 0.0f;
+    if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+#pragma unroll
+        for (int j = 0; j <= 3; j++) {
+            const int v = v0 - 1 + j;
+            float p = 0.0f;
+#pragma unroll
+            for (int i = 0; i <= 3; i++) {
+                const int u = u0 - 1 + i;
+                p = p + t[S * ((v - oy) * pitch + (u - ox))] * cubic_w(x - (float)u);
+            }
+            q = q + p * cubic_w(y - (float)v);
+        }
+    } else {
+        const int xbase = (int)fminf((float)w - 2.0f, fmaxf(x, 0.0f));
+        const int ybase = (int)fminf((float)h - 2.0f, fmaxf(y, 0.0f));
+        const float xf = x - (float)xbase;
+        const float yf = y - (float)ybase;
+        const float ll = t[S * ((ybase - oy) * pitch + (xbase - ox))];
+        const float lr = t[S * ((ybase - oy) * pitch + (xbase + 1 - ox))];
+        const float ur = t[S * ((ybase + 1 - oy) * pitch + (xbase + 1 - ox))];
+        const float ul = t[S * ((ybase + 1 - oy) * pitch + (xbase - ox))];
+        const float ua = ul + xf * (ur - ul);
+        const float la = ll + xf * (lr - ll);
+        q = la + yf * (ua - la);
+    }
+    return q;
+}
+
 __device__ __forceinline__ float interp_global(const float *__restrict__ a, int w, int h,
                                                float x, float y)
 {

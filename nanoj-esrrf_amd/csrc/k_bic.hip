@@ -609,15 +609,15 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
 // barriers per two frames.  The cumulants are updated frame by frame in order, so results
 // are those of k_bic_rgc bit for bit.
 // ---------------------------------------------------------------------------------------
+template <bool INTW>
 __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, float &in,
                                          float &prv, float C0, float v, bool first,
                                          const BicRgcArgs &a)
 {
-    // temporal accumulation (liveSRRF.cl:363-393)
+    // temporal accumulation (liveSRRF.cl:363-393); Prev holds t/(n-1) (tac_prev_scaled)
     const float C = apply_sens(C0, a.sens, a.sens_int);
-    const bool intw = a.int_weighting != 0;
-    const float t = intw ? v * C : C;
-    const float e2 = intw ? (t * v) * C : C * C;
+    const float t = INTW ? v * C : C;
+    const float e2 = INTW ? (t * v) * C : C * C;
     if (first) {
         avg = t * a.rn;
         e2s = e2 * a.rn;
@@ -629,12 +629,12 @@ __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, flo
         tac = tac + (t * prv) * a.rn1;
         in = in + v * a.rn;
     }
-    prv = a.tac_prev_scaled ? t * a.rn1 : t;
+    prv = t * a.rn1;
 }
 
-template <int PPW, int MODE>
+template <int PPW, int MODE, bool INTW>
 __global__ void __launch_bounds__(1024)
-__attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
+__attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
                                                    const Tap *__restrict__ taps,
                                                    const Phase *__restrict__ phases,
                                                    const int *__restrict__ tapk)
@@ -642,8 +642,8 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
     extern __shared__ __align__(16) unsigned char smem[];
     float4 *lU = reinterpret_cast<float4 *>(smem);
     const int usz = 4 * a.PP * a.PH;
-    float *lR = reinterpret_cast<float *>(lU + usz);   // frame f, then frame f+1 at +RAW_SZ
-    Tap *lT = reinterpret_cast<Tap *>(lR + 2 * RAW_SZ);
+    float2 *lR = reinterpret_cast<float2 *>(lU + usz);   // raw tiles of f and f+1, interleaved
+    Tap *lT = reinterpret_cast<Tap *>(lR + RAW_SZ);
     int *lPlan = reinterpret_cast<int *>(lT + a.max_class_taps + TAP_SPARE);
     constexpr int PH_UNROLL = PPW <= 4 ? PPW : 1;
 
@@ -744,8 +744,7 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
         for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
             const int r = e / RAW_T, c = e - r * RAW_T;
             const long g = (long)min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
-            lR[r * RAW_P + c] = Pa[g];
-            lR[RAW_SZ + r * RAW_P + c] = two ? Pa[g + wh] : 0.0f;
+            lR[r * RAW_P + c] = make_float2(Pa[g], two ? Pa[g + wh] : 0.0f);
         }
     };
 
@@ -766,6 +765,9 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
                 const Phase P = PHf[ph];
                 float acc0 = 0.0f, acc1 = 0.0f;
                 const Tap *Tl = lT + (P.first_tap - tcls0);
+                // (one chunk of 4 taps per iteration: two in flight exceed the register
+                // budget of 5 waves per SIMD)
+#pragma unroll 1
                 for (int t = 0; t < P.ntaps; t += 4) {
                     const Tap c0 = Tl[t], c1 = Tl[t + 1], c2 = Tl[t + 2], c3 = Tl[t + 3];
                     const float4 u0 = lU[base + c0.off];
@@ -787,15 +789,16 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
                 // interpolated intensity at (xM/M - 0.5, yM/M - 0.5) (liveSRRF.cl:361)
                 float va, vb;
                 if (ibase[p] >= 0) {
-                    const float *row = lR + ibase[p];
+                    const float2 *row = lR + ibase[p];
                     float qa = 0.0f, qb = 0.0f;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         float pa = 0.0f, pb = 0.0f;
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            pa = pa + row[j * RAW_P + i] * P.wx[i];
-                            pb = pb + row[RAW_SZ + j * RAW_P + i] * P.wx[i];
+                            const float2 r2 = row[j * RAW_P + i];
+                            pa = pa + r2.x * P.wx[i];
+                            pb = pb + r2.y * P.wx[i];
                         }
                         qa = qa + pa * P.wy[j];
                         qb = qb + pb * P.wy[j];
@@ -803,9 +806,13 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
                     va = qa;
                     vb = qb;
                 } else {
+                    // border pixel: the reference's bilinear-extrapolation regime
                     const int xM = (X + zf) * M + P.sx, yM = (Y + zf) * M + P.sy;
-                    va = border_intensity(lR, X0, Y0, w, h, M, xM, yM);
-                    vb = border_intensity(lR + RAW_SZ, X0, Y0, w, h, M, xM, yM);
+                    const float xf = (float)xM / (float)M - 0.5f;
+                    const float yf = (float)yM / (float)M - 0.5f;
+                    const float *lRf = reinterpret_cast<const float *>(lR);
+                    va = interp_tile_s<2>(lRf, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
+                    vb = interp_tile_s<2>(lRf + 1, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
                 }
 
                 if (MODE == 1) {
@@ -820,9 +827,11 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
                     }
                     continue;
                 }
-                cumulate(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0a, va, first, a);
+                cumulate<INTW>(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0a, va, first,
+                               a);
                 if (two)
-                    cumulate(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0b, vb, false, a);
+                    cumulate<INTW>(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0b, vb,
+                                   false, a);
             }
         }
         if (fi + 2 < a.nframes) {
@@ -850,23 +859,25 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
 
 size_t bic_pair_lds_bytes(const BicRgcArgs &a)
 {
-    return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float) * 2 * RAW_SZ +
+    return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float2) * RAW_SZ +
            sizeof(Tap) * ((size_t)a.max_class_taps + TAP_SPARE) +
            sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
 }
 
-template <int PPW, int MODE>
+template <int PPW, int MODE, bool INTW>
 static hipError_t launch_pair_m(const BicRgcArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE>), dim3(a.tiles_x * a.tiles_y), dim3(64 * a.nw),
-                       bic_pair_lds_bytes(a), s, a, a.taps, a.phases, a.tapk);
+    hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW>), dim3(a.tiles_x * a.tiles_y),
+                       dim3(64 * a.nw), bic_pair_lds_bytes(a), s, a, a.taps, a.phases, a.tapk);
     return hipGetLastError();
 }
 
 template <int PPW>
 static hipError_t launch_pair(const BicRgcArgs &a, hipStream_t s)
 {
-    return a.mode == 1 ? launch_pair_m<PPW, 1>(a, s) : launch_pair_m<PPW, 0>(a, s);
+    if (a.mode == 1) return launch_pair_m<PPW, 1, false>(a, s);
+    return a.int_weighting ? launch_pair_m<PPW, 0, true>(a, s)
+                           : launch_pair_m<PPW, 0, false>(a, s);
 }
 
 template <int PPW, bool TLDS, int MODE>
@@ -915,7 +926,7 @@ hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s)
 {
     if (a.nframes <= 0) return hipSuccess;
     if (a.frames_per_pass == 2 && a.mode != 2 && a.taps_in_lds && ppw <= 4 &&
-        bic_pair_lds_bytes(a) <= 64 * 1024) {
+        a.tac_prev_scaled && bic_pair_lds_bytes(a) <= 64 * 1024) {
         switch (ppw) {
         case 1: return launch_pair<1>(a, s);
         case 2: return launch_pair<2>(a, s);
