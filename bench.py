@@ -94,8 +94,8 @@ def cpu_baseline(stack, nframes):
 
 def load_traffic():
     """HBM bytes per RGC launch from the committed rocprofv3 PMC summary (or None): the
-    line `hbm_bytes_per_launch: <int>` of profiles/k_bic_rgc_256x256x1000_m5_traffic.md."""
-    p = os.path.join(REPO, "profiles", "k_bic_rgc_256x256x1000_m5_traffic.md")
+    line `hbm_bytes_per_launch: <int>` of profiles/k_bic_rgc2_256x256x1000_m5_traffic.md."""
+    p = os.path.join(REPO, "profiles", "k_bic_rgc2_256x256x1000_m5_traffic.md")
     try:
         with open(p) as f:
             for line in f:
@@ -138,7 +138,9 @@ def bench_sweep(pkg, torch, stack):
         sw.run_device(d_frames.data_ptr(), C.ESRRF_U16, n)
     torch.cuda.synchronize()
     fused_s = (time.perf_counter() - t0) / reps
-    shapes = {m: sw.shape(i) for i, m in enumerate(mags)}
+    shapes = {f"M{m} fwhm{fw}": [sw.shape(i, j)[k] for k in ("nw", "ppw", "taps_in_lds",
+                                                           "frames_per_pass")]
+              for i, m in enumerate(mags) for j, fw in enumerate(fwhms)}
     sw.close()
     # looped plain: a fresh engine per configuration, as the reference re-initialises
     d_out = torch.empty(4 * W * H * 25, dtype=torch.float32, device=dev)
@@ -165,8 +167,7 @@ def bench_sweep(pkg, torch, stack):
             "fused_configs_per_s": round(nconf / fused_s, 2),
             "looped_plain_configs_per_s": round(nconf / looped_s, 2),
             "speedup": round(looped_s / fused_s, 2),
-            "autotuned_shapes": {str(k): [v["nw"], v["ppw"], v["taps_in_lds"]]
-                                 for k, v in shapes.items()},
+            "autotuned_shapes_nw_ppw_tlds_fpp": shapes,
             "workload": "500x256x256 u16; M {2,4,5} x fwhm {1..3 step .5} x S {1..4} x "
                         "first {100,250,500} frames; intensity weighting + MP on"}
 
@@ -327,10 +328,17 @@ def main():
         nw_, ppw_, tl_, ms_ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         eng.check(L.esrrf_autotune(h, 2, ctypes.byref(nw_), ctypes.byref(ppw_),
                                    ctypes.byref(tl_), ctypes.byref(ms_)))
-        tuned = {"waves_per_block": nw_.value, "phases_per_wave": ppw_.value,
-                 "taps_in_lds": tl_.value, "tuning_launch_ms": round(ms_.value, 3)}
+        tuned = {"tuning_launch_ms": round(ms_.value, 3)}
         step()
         torch.cuda.synchronize()
+    shape = None
+    if not fht:
+        sh = [ctypes.c_int() for _ in range(4)]
+        eng.check(L.esrrf_launch_shape(h, *[ctypes.byref(x) for x in sh]))
+        shape = {"waves_per_block": sh[0].value, "phases_per_wave": sh[1].value,
+                 "taps_in_lds": sh[2].value, "frames_per_pass": sh[3].value}
+        if tuned:
+            shape.update(tuned)
     eng.check(L.esrrf_profile_enable(h, 1))
     t0 = time.perf_counter()
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -376,7 +384,9 @@ def main():
                                   else "BIC"),
                    "frames": nf, "width": W, "height": H, "magnification": M,
                    "fwhm": FWHM, "sensitivity": SENS},
-        "roofline": {"bound": "valu", "kernel": "k_fht_rgc" if fht else "k_bic_rgc",
+        "roofline": {"bound": "valu",
+                     "kernel": "k_fht_rgc" if fht else
+                     ("k_bic_rgc2" if shape and shape["frames_per_pass"] == 2 else "k_bic_rgc"),
                      "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": None if fht else load_traffic(),
@@ -387,7 +397,7 @@ def main():
                      if not fht else None,
                      "upsample_ms_per_load": round(up_ms.value / max(up_n.value, 1), 3)
                      if fht else None,
-                     "launch_shape": tuned},
+                     "launch_shape": shape},
         "wall_s": round(wall, 3),
     }
     eng.close()

@@ -135,6 +135,12 @@ int esrrf_plan_load(const int *plan6, int r, int l, int *first_index, int *n_fra
  * (waves per block x tap tables in LDS or not) over the frames of the current load, `reps`
  * launches each, into scratch outputs, and keeps the fastest for this engine.  Reports it. */
 int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_lds, float *ms);
+/* Launch shape the engine's tiled 2-D kernel uses now (no counterpart in the reference):
+ * waves per block, sub-pixel phases per wave, tap tables in LDS, and frames per pass
+ * (2: two frames share each tap read).  ESRRF_ERR_STATE when the engine runs no tiled
+ * kernel (FHT, 3-D, or a geometry outside the tiled kernel's range). */
+int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
+                       int *frames_per_pass);
 /* ---- Parameter sweep (gui/ParametersSweep_.java:328-519) --------------------------------
  * Every (magnification, frame count, sensitivity, fwhm) configuration of a sweep over the
  * first nf frames of one stack, 2-D BIC, no drift; for each the four channels that
@@ -172,8 +178,10 @@ int esrrf_sweep_run_device(esrrf_sweep *s, const void *d_frames, int dtype, int 
 int esrrf_sweep_read(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm, float *out);
 int esrrf_sweep_read_device(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm,
                             float *d_out);
-/* Launch shape used for a magnification (autotuned or default). */
-int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int *nw, int *ppw, int *taps_in_lds, float *ms);
+/* Launch shape used for a (magnification, fwhm) (autotuned or default): waves per block,
+ * phases per wave, tap tables in LDS, frames per pass, autotuner time (0: not tuned). */
+int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int i_fwhm, int *nw, int *ppw, int *taps_in_lds,
+                      int *frames_per_pass, float *ms);
 const char *esrrf_sweep_last_error(esrrf_sweep *s);
 
 /* ---- SQUIRREL RSE and its gradient (ErrorMapLiveSRRF.java:81-246) ------------------------

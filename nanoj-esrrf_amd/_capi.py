@@ -28,7 +28,7 @@ EXPORTS = [
     "esrrf_read_frame_counters", "esrrf_get_dims", "esrrf_finish", "esrrf_set_stream",
     "esrrf_profile_enable", "esrrf_profile_read", "esrrf_bic_constants",
     "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
-    "esrrf_last_error", "esrrf_version", "esrrf_autotune",
+    "esrrf_last_error", "esrrf_version", "esrrf_autotune", "esrrf_launch_shape",
     "esrrf_sweep_create", "esrrf_sweep_destroy", "esrrf_sweep_set_stream", "esrrf_sweep_run",
     "esrrf_sweep_run_device", "esrrf_sweep_read", "esrrf_sweep_read_device",
     "esrrf_sweep_shape", "esrrf_sweep_last_error",
@@ -126,6 +126,7 @@ def lib():
         "esrrf_last_error": [vp],
         "esrrf_version": [],
         "esrrf_autotune": [vp, c_int, ip, ip, ip, fp],
+        "esrrf_launch_shape": [vp, ip, ip, ip, ip],
         "esrrf_sweep_create": [ctypes.POINTER(SweepParams), ctypes.POINTER(vp)],
         "esrrf_sweep_destroy": [vp],
         "esrrf_sweep_set_stream": [vp, vp],
@@ -133,7 +134,7 @@ def lib():
         "esrrf_sweep_run_device": [vp, vp, c_int, c_int],
         "esrrf_sweep_read": [vp, c_int, c_int, c_int, c_int, fp],
         "esrrf_sweep_read_device": [vp, c_int, c_int, c_int, c_int, vp],
-        "esrrf_sweep_shape": [vp, c_int, ip, ip, ip, fp],
+        "esrrf_sweep_shape": [vp, c_int, c_int, ip, ip, ip, ip, fp],
         "esrrf_sweep_last_error": [vp],
         "esrrf_rse_create": [ctypes.POINTER(RseParams), ctypes.POINTER(vp)],
         "esrrf_rse_destroy": [vp],

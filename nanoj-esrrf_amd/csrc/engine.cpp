@@ -773,6 +773,20 @@ int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_ld
     return ESRRF_OK;
 }
 
+int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
+                       int *frames_per_pass)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->mode != MODE_BIC || !e->tiled)
+        return fail(e, ESRRF_ERR_STATE, "the engine runs no tiled 2-D kernel");
+    if (nw) *nw = e->shape.nw;
+    if (ppw) *ppw = e->shape.ppw;
+    if (taps_in_lds) *taps_in_lds = e->shape.taps_in_lds;
+    if (frames_per_pass) *frames_per_pass = e->shape.fp;
+    return ESRRF_OK;
+}
+
 int esrrf_plan_rolling(int n_slices, int n_frame_for_srrf_user, int frame_gap_user, int rolling,
                        int max_frames_on_gpu, int *out6)
 {

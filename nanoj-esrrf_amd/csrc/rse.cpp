@@ -41,7 +41,8 @@ struct esrrf_rse {
     int loaded = 0;
     BicTables tabs;
     float tabs_fwhm = -1.0f;
-    BicShape shape;
+    BicShape shape;          // adjoint launches (mode 2)
+    BicShape shape_value;    // value-only launches (mode 0)
     RsePlan plan{};
     // (fwhm, S) of the current SR image in plan.SR (and of Dt/DS when cur_grad)
     float cur_fwhm = -1.0f, cur_sens = -1.0f;
@@ -92,9 +93,11 @@ int prepare_sr(esrrf_rse *r, float fwhm, float sens, bool grad)
         RCHK(r, bic_tables_create(r->w, r->h, r->M, fwhm, r->tabs));
         if (!r->tabs.ok) return rfail(r, ESRRF_ERR_UNSUPPORTED, "fwhm %g: window too large", fwhm);
         r->tabs_fwhm = fwhm;
-        r->shape = bic_default_shape(r->tabs, r->M, true);   // fits both variants
+        r->shape = bic_default_shape(r->tabs, r->M, true);        // adjoint variant
+        r->shape_value = bic_default_shape(r->tabs, r->M, false);  // value only
     }
-    BicRgcArgs a = bic_args(r->tabs, r->shape, r->d_U, r->d_px, r->w, r->h, r->M);
+    const BicShape &sh = grad ? r->shape : r->shape_value;
+    BicRgcArgs a = bic_args(r->tabs, sh, r->d_U, r->d_px, r->w, r->h, r->M);
     a.mode = grad ? 2 : 0;
     a.out = r->d_out4;
     a.prev = r->d_prev;
@@ -106,7 +109,7 @@ int prepare_sr(esrrf_rse *r, float fwhm, float sens, bool grad)
     a.sens = sens;
     a.sens_int = sens_exact_int(sens);
     a.int_weighting = r->intw;
-    RCHK(r, launch_bic_rgc(a, r->shape.ppw, r->stream));
+    RCHK(r, launch_bic_rgc(a, sh.ppw, r->stream));
     RCHK(r, launch_rse_prep(r->plan, r->d_out4, r->stream));
     r->cur_fwhm = fwhm;
     r->cur_sens = sens;

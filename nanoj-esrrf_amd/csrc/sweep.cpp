@@ -12,7 +12,7 @@
 //     sensitivity) configuration;
 //   * VAR and MP correction finish each configuration (liveSRRF.cl:427-505).
 // The whole device sequence can be captured once in a hipGraph and replayed, and the RGC
-// launch shape of each magnification can be picked by the autotuner.
+// launch shape of each (magnification, fwhm) can be picked by the autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,7 +47,7 @@ struct esrrf_sweep {
     float *d_C = nullptr, *d_V = nullptr;   // emitted CGLH / intensity, largest M
     float *d_dummy = nullptr;      // out/prev of the emit launches (never written)
     std::vector<std::vector<BicTables>> tabs;   // [iM][iF]
-    std::vector<BicShape> shape;   // [iM]
+    std::vector<std::vector<BicShape>> shape;   // [iM][iFwhm]
     std::vector<float *> d_out;    // [iM] configurations [nf][S][fwhm][4][hM][wM]
     std::vector<float *> d_mp;     // [iM] MP maps [nf][S][fwhm][3][M][M]
     float *d_tmp = nullptr;        // read-back staging
@@ -111,7 +111,7 @@ int enqueue_run(esrrf_sweep *s, int dtype)
         const long whM = wh * M * M;
         for (int fi = 0; fi < nF; fi++) {
             const BicTables &t = s->tabs[im][fi];
-            BicRgcArgs a = bic_args(t, s->shape[im], s->d_U, s->d_px, s->w, s->h, M);
+            BicRgcArgs a = bic_args(t, s->shape[im][fi], s->d_U, s->d_px, s->w, s->h, M);
             a.mode = 1;
             a.emitC = s->d_C;
             a.emitV = s->d_V;
@@ -125,7 +125,7 @@ int enqueue_run(esrrf_sweep *s, int dtype)
             a.sens = 1.0f;
             a.sens_int = 1;
             a.int_weighting = s->int_weighting;
-            SCHK(s, launch_bic_rgc(a, s->shape[im].ppw, st));
+            SCHK(s, launch_bic_rgc(a, s->shape[im][fi].ppw, st));
             SweepAccArgs c{};
             c.C = s->d_C;
             c.V = s->d_V;
@@ -166,38 +166,28 @@ int enqueue_run(esrrf_sweep *s, int dtype)
     return ESRRF_OK;
 }
 
-// One shape serves every fwhm of a magnification: keep the LDS tap tables and the frame
-// pairs only where every fwhm's tables fit them.
-static void clamp_shape(esrrf_sweep *s, size_t im)
-{
-    for (size_t fi = 0; fi < s->fwhm.size(); fi++) {
-        const BicShape d = bic_default_shape(s->tabs[im][fi], s->Ms[im]);
-        if (!d.taps_in_lds) s->shape[im].taps_in_lds = 0;
-        if (d.fp != 2 || !s->shape[im].taps_in_lds) s->shape[im].fp = 1;
-    }
-}
-
 int tune(esrrf_sweep *s)
 {
-    // per magnification: the emit launch of the first fwhm over up to 32 frames
+    // per (magnification, fwhm): the emit launch over up to 32 frames
     for (size_t im = 0; im < s->Ms.size(); im++) {
         const int M = s->Ms[im];
-        const BicTables &t = s->tabs[im][0];
-        BicRgcArgs a = bic_args(t, s->shape[im], s->d_U, s->d_px, s->w, s->h, M);
-        a.mode = 1;
-        a.emitC = s->d_C;
-        a.emitV = s->d_V;
-        a.out = s->d_dummy;
-        a.prev = s->d_dummy;
-        a.nframes = std::min(s->nfmax, 32);
-        a.first_init = 1;
-        a.rn = a.rn1 = a.sens = 1.0f;
-        a.sens_int = 1;
-        a.int_weighting = s->int_weighting;
-        BicShape best;
-        SCHK(s, bic_autotune(t, a, M, 2, s->stream, &best, nullptr));
-        s->shape[im] = best;
-        clamp_shape(s, im);
+        for (size_t fi = 0; fi < s->fwhm.size(); fi++) {
+            const BicTables &t = s->tabs[im][fi];
+            BicRgcArgs a = bic_args(t, s->shape[im][fi], s->d_U, s->d_px, s->w, s->h, M);
+            a.mode = 1;
+            a.emitC = s->d_C;
+            a.emitV = s->d_V;
+            a.out = s->d_dummy;
+            a.prev = s->d_dummy;
+            a.nframes = std::min(s->nfmax, 32);
+            a.first_init = 1;
+            a.rn = a.rn1 = a.sens = 1.0f;
+            a.sens_int = 1;
+            a.int_weighting = s->int_weighting;
+            BicShape best;
+            SCHK(s, bic_autotune(t, a, M, 2, s->stream, &best, nullptr));
+            s->shape[im][fi] = best;
+        }
     }
     return ESRRF_OK;
 }
@@ -327,8 +317,9 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
                 return bad(ESRRF_ERR_HIP);
             if (!s->tabs[im][fi].ok) return bad(ESRRF_ERR_UNSUPPORTED);
         }
-        s->shape[im] = bic_default_shape(s->tabs[im][0], M);
-        clamp_shape(s, im);
+        s->shape[im].resize(s->fwhm.size());
+        for (size_t fi = 0; fi < s->fwhm.size(); fi++)
+            s->shape[im][fi] = bic_default_shape(s->tabs[im][fi], M);
         if (!alloc((void **)&s->d_out[im], sizeof(float) * nc * 4 * wh * M * M) ||
             !alloc((void **)&s->d_mp[im], sizeof(float) * nc * 3 * M * M))
             return bad(ESRRF_ERR_NOMEM);
@@ -408,14 +399,18 @@ int esrrf_sweep_read_device(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int
     return ESRRF_OK;
 }
 
-int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int *nw, int *ppw, int *taps_in_lds, float *ms)
+int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int i_fwhm, int *nw, int *ppw, int *taps_in_lds,
+                      int *frames_per_pass, float *ms)
 {
-    if (!s || i_mag < 0 || i_mag >= (int)s->Ms.size()) return ESRRF_ERR_INVALID;
+    if (!s || i_mag < 0 || i_mag >= (int)s->Ms.size() || i_fwhm < 0 ||
+        i_fwhm >= (int)s->fwhm.size())
+        return ESRRF_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
-    const BicShape &b = s->shape[i_mag];
+    const BicShape &b = s->shape[i_mag][i_fwhm];
     if (nw) *nw = b.nw;
     if (ppw) *ppw = b.ppw;
     if (taps_in_lds) *taps_in_lds = b.taps_in_lds;
+    if (frames_per_pass) *frames_per_pass = b.fp;
     if (ms) *ms = b.ms;
     return ESRRF_OK;
 }

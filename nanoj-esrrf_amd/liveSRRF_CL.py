@@ -252,6 +252,15 @@ class LiveSRRF_CL:
                                    ctypes.byref(tl), ctypes.byref(ms)))
         return dict(nw=nw.value, ppw=ppw.value, taps_in_lds=tl.value, ms=ms.value)
 
+    def launchShape(self):
+        """Launch shape of the tiled 2-D kernel in use (esrrf_launch_shape; no counterpart
+        in the reference)."""
+        e = self._e()
+        v = [ctypes.c_int() for _ in range(4)]
+        e.check(e.L.esrrf_launch_shape(e.h, *[ctypes.byref(x) for x in v]))
+        return dict(nw=v[0].value, ppw=v[1].value, taps_in_lds=v[2].value,
+                    frames_per_pass=v[3].value)
+
     def frameCounters(self):
         """nCurrentFrame[0..1] (exact bookkeeping hook)."""
         return self._e().counters()

@@ -87,12 +87,15 @@ class ParametersSweep:
         self._check(self.L.esrrf_sweep_read(self.h, i_mag, i_nf, i_sens, i_fwhm, C.f32ptr(out)))
         return out
 
-    def shape(self, i_mag):
-        nw, ppw, tl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    def shape(self, i_mag, i_fwhm=0):
+        """Launch shape of the RGC emit launch of one (magnification, fwhm)."""
+        nw, ppw, tl, fpp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_float()
-        self._check(self.L.esrrf_sweep_shape(self.h, i_mag, ctypes.byref(nw), ctypes.byref(ppw),
-                                             ctypes.byref(tl), ctypes.byref(ms)))
-        return dict(nw=nw.value, ppw=ppw.value, taps_in_lds=tl.value, ms=ms.value)
+        self._check(self.L.esrrf_sweep_shape(self.h, i_mag, i_fwhm, ctypes.byref(nw),
+                                             ctypes.byref(ppw), ctypes.byref(tl),
+                                             ctypes.byref(fpp), ctypes.byref(ms)))
+        return dict(nw=nw.value, ppw=ppw.value, taps_in_lds=tl.value,
+                    frames_per_pass=fpp.value, ms=ms.value)
 
     def close(self):
         if self.h:

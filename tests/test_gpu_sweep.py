@@ -93,8 +93,9 @@ def test_sweep_autotune_picks_a_supported_shape(pkg):
     st = stack(8, 24, 24, seed=5)
     tuned = pkg.ParametersSweep(24, 24, [5], [1.5], [2.0], [8], autotune=True, use_graph=False)
     tuned.run(st)
-    sh = tuned.shape(0)
+    sh = tuned.shape(0, 0)
     assert sh["ms"] > 0 and sh["nw"] in (4, 8, 12, 16) and sh["ppw"] * sh["nw"] >= 25
+    assert sh["frames_per_pass"] in (1, 2)
     base = pkg.ParametersSweep(24, 24, [5], [1.5], [2.0], [8], autotune=False, use_graph=False)
     base.run(st)
     assert_parity(tuned.read(0, 0, 0, 0), base.read(0, 0, 0, 0), "autotuned vs default")
