@@ -28,6 +28,8 @@ struct FhtRgcArgs {
     float rn, rn1, sens;
     int sens_int, int_weighting;
     int R, TW, TH, pitch;     // window half-size, LDS tile extent and pitch
+    int Rhi;                  // window reach on the high side ((int)(radius + 1))
+    int pair;                 // 1: two frames and two pixels per lane where LDS allows
     int ntaps;
     float wsum;
     int tiles_x, tiles_y;

@@ -313,6 +313,29 @@ __device__ __forceinline__ float tap_acc_f(float2 u, const Tap &tp, float acc)
     return fmaf(tp.W, hh, acc);
 }
 
+// Temporal accumulation of one pixel and frame (liveSRRF_noInterpolation.cl:358-372) with
+// every rounding spelled out, so that the kernels below agree bit for bit whatever the
+// compiler contracts.
+__device__ __forceinline__ void cumulate_f(float &s0, float &s1, float &s2, float &s3, float &s4,
+                                           float C0, float v, bool first, const FhtRgcArgs &a)
+{
+    const float C = apply_sens_f(C0, a.sens, a.sens_int);
+    const float t = a.int_weighting ? __fmul_rn(v, C) : C;
+    const float e2 = a.int_weighting ? __fmul_rn(__fmul_rn(t, v), C) : __fmul_rn(C, C);
+    if (first) {
+        s0 = __fmul_rn(t, a.rn);
+        s1 = __fmul_rn(e2, a.rn);
+        s2 = 0.0f;
+        s3 = __fmul_rn(v, a.rn);
+    } else {
+        s0 = fmaf(t, a.rn, s0);
+        s1 = fmaf(e2, a.rn, s1);
+        s2 = fmaf(__fmul_rn(t, s4), a.rn1, s2);
+        s3 = fmaf(v, a.rn, s3);
+    }
+    s4 = t;   // single 1/(n-1) TAC2 (liveSRRF_noInterpolation.cl:369-372)
+}
+
 __global__ void __launch_bounds__(256) k_fht_rgc(const FhtRgcArgs a,
                                                  const Tap *__restrict__ taps,
                                                  const int *__restrict__ tapk)
@@ -377,24 +400,9 @@ __global__ void __launch_bounds__(256) k_fht_rgc(const FhtRgcArgs a,
             acc = tap_acc_f(u2, c2, acc);
             acc = tap_acc_f(u3, c3, acc);
         }
-        float C = (a.wsum + acc) * invw;
-        C = apply_sens_f(C, a.sens, a.sens_int);
         // unshifted frames: the intensity is the magnified pixel itself (:356)
         const float v = inimg ? a.px[wh * f + pix] : 0.0f;
-        const float t = a.int_weighting ? v * C : C;
-        const float e2 = a.int_weighting ? (t * v) * C : C * C;
-        if (fi == 0 && a.first_init) {
-            s0 = t * a.rn;
-            s1 = e2 * a.rn;
-            s2 = 0.0f;
-            s3 = v * a.rn;
-        } else {
-            s0 = s0 + t * a.rn;
-            s1 = s1 + e2 * a.rn;
-            s2 = s2 + (t * s4) * a.rn1;
-            s3 = s3 + v * a.rn;
-        }
-        s4 = t;   // single 1/(n-1) TAC2 (liveSRRF_noInterpolation.cl:369-372)
+        cumulate_f(s0, s1, s2, s3, s4, (a.wsum + acc) * invw, v, fi == 0 && a.first_init, a);
     }
     if (inimg) {
         a.out[pix] = s0;
@@ -410,9 +418,177 @@ size_t fht_rgc_lds_bytes(int TH, int pitch, int ntaps)
     return sizeof(float2) * (size_t)TH * pitch + sizeof(Tap) * (size_t)(ntaps + 4);
 }
 
+// ---------------------------------------------------------------------------------------
+// Batched variant: each lane takes two pixels (x and x+8) of two consecutive frames.  The
+// block (8 waves) covers 32 x 32 pixels; the U tile of the frame pair is interleaved as
+// float4 (Ux, Uy of f, Ux, Uy of f+1) with a pitch of 8 (mod 16) float4, so one uniform
+// tap read and one address add serve four samples and both 16-byte reads of a tap sit at
+// compile-time offsets.  Per pixel the frames are accumulated in order with the same
+// operations as k_fht_rgc: the results are equal bit for bit.
+// ---------------------------------------------------------------------------------------
+constexpr int FT2 = 32;
+
+__host__ __device__ __forceinline__ int fht2_pitch(int TW2)
+{
+    int p = 8;
+    while (p < TW2) p += 16;
+    return p;
+}
+
+size_t fht_rgc2_lds_bytes(const FhtRgcArgs &a)
+{
+    const int TW2 = FT2 + a.R + a.Rhi;
+    return sizeof(float4) * (size_t)TW2 * fht2_pitch(TW2) + sizeof(Tap) * (size_t)(a.ntaps + 4);
+}
+
+__global__ void __launch_bounds__(512) k_fht_rgc2(const FhtRgcArgs a,
+                                                  const Tap *__restrict__ taps,
+                                                  const int *__restrict__ tapk)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int TW2 = FT2 + a.R + a.Rhi;
+    const int P2 = fht2_pitch(TW2);
+    float4 *lU = reinterpret_cast<float4 *>(smem);
+    Tap *lT = reinterpret_cast<Tap *>(lU + TW2 * P2);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lx = (lane & 7) + 16 * (wave & 1), ly = (lane >> 3) + 8 * (wave >> 1);
+    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int X0 = tX * FT2, Y0 = tY * FT2;
+    const int xa = X0 + lx, xb = xa + 8, yM = Y0 + ly;
+    const int W = a.W, H = a.H;
+    const bool ina = xa < W && yM < H, inb = xb < W && yM < H;
+    const long wh = (long)W * H;
+    // tap table with offsets for this tile's pitch (dx, dy from tapk)
+    for (int e = tid; e < a.ntaps + 4; e += 512) {
+        Tap t = taps[e];
+        const int k = tapk[e];
+        t.off = (((k >> 8) & 255) - 128) * P2 + ((k & 255) - 128);
+        lT[e] = t;
+    }
+    const bool interior = X0 - a.R >= 0 && X0 + FT2 - 1 + a.Rhi <= W - 1 && Y0 - a.R >= 0 &&
+                          Y0 + FT2 - 1 + a.Rhi <= H - 1;
+    float wsa = a.wsum, wsb = a.wsum;
+    if (!interior) {
+        wsa = 0.0f;
+        wsb = 0.0f;
+        for (int t = 0; t < a.ntaps; t++) {
+            const int k = tapk[t];
+            const int dn = (k & 255) - 128, m = yM + ((k >> 8) & 255) - 128;
+            const bool my = m >= 0 && m <= H - 1;
+            if (my && xa + dn >= 0 && xa + dn <= W - 1) wsa += taps[t].W;
+            if (my && xb + dn >= 0 && xb + dn <= W - 1) wsb += taps[t].W;
+        }
+    }
+    const float invwa = 1.0f / wsa, invwb = 1.0f / wsb;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
+    float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f, b4 = 0.0f;
+    const long pixa = (long)yM * W + xa, pixb = pixa + 8;
+    if (!a.first_init) {
+        if (ina) {
+            a0 = a.out[pixa];
+            a1 = a.out[pixa + wh];
+            a2 = a.out[pixa + 2 * wh];
+            a3 = a.out[pixa + 3 * wh];
+            a4 = a.prev[pixa];
+        }
+        if (inb) {
+            b0 = a.out[pixb];
+            b1 = a.out[pixb + wh];
+            b2 = a.out[pixb + 2 * wh];
+            b3 = a.out[pixb + 3 * wh];
+            b4 = a.prev[pixb];
+        }
+    }
+    const int base = (ly + a.R) * P2 + (lx + a.R);
+    const int nU = TW2 * TW2;
+    for (int fi = 0; fi < a.nframes; fi += 2) {
+        const int f = a.frame0 + fi;
+        const bool two = fi + 1 < a.nframes;
+        __syncthreads();
+        const float2 *Uf = a.U + wh * f;
+        for (int e = tid; e < nU; e += 512) {
+            const int r = e / TW2, c = e - r * TW2;
+            const int n = X0 - a.R + c, m = Y0 - a.R + r;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (n >= 0 && n < W && m >= 0 && m < H) {
+                const long g = (long)m * W + n;
+                const float2 u0 = Uf[g];
+                const float2 u1 = two ? Uf[g + wh] : make_float2(0.0f, 0.0f);
+                v = make_float4(u0.x, u0.y, u1.x, u1.y);
+            }
+            lU[r * P2 + c] = v;
+        }
+        __syncthreads();
+        float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+        for (int t = 0; t < a.ntaps; t += 4) {
+            const Tap c0 = lT[t], c1 = lT[t + 1], c2 = lT[t + 2], c3 = lT[t + 3];
+            const float4 *q0 = lU + base + c0.off;
+            const float4 *q1 = lU + base + c1.off;
+            const float4 *q2 = lU + base + c2.off;
+            const float4 *q3 = lU + base + c3.off;
+            const float4 u0 = q0[0], v0 = q0[8];
+            const float4 u1 = q1[0], v1 = q1[8];
+            const float4 u2 = q2[0], v2 = q2[8];
+            const float4 u3 = q3[0], v3 = q3[8];
+            acc0 = tap_acc_f(make_float2(u0.x, u0.y), c0, acc0);
+            acc1 = tap_acc_f(make_float2(u0.z, u0.w), c0, acc1);
+            acc2 = tap_acc_f(make_float2(v0.x, v0.y), c0, acc2);
+            acc3 = tap_acc_f(make_float2(v0.z, v0.w), c0, acc3);
+            acc0 = tap_acc_f(make_float2(u1.x, u1.y), c1, acc0);
+            acc1 = tap_acc_f(make_float2(u1.z, u1.w), c1, acc1);
+            acc2 = tap_acc_f(make_float2(v1.x, v1.y), c1, acc2);
+            acc3 = tap_acc_f(make_float2(v1.z, v1.w), c1, acc3);
+            acc0 = tap_acc_f(make_float2(u2.x, u2.y), c2, acc0);
+            acc1 = tap_acc_f(make_float2(u2.z, u2.w), c2, acc1);
+            acc2 = tap_acc_f(make_float2(v2.x, v2.y), c2, acc2);
+            acc3 = tap_acc_f(make_float2(v2.z, v2.w), c2, acc3);
+            acc0 = tap_acc_f(make_float2(u3.x, u3.y), c3, acc0);
+            acc1 = tap_acc_f(make_float2(u3.z, u3.w), c3, acc1);
+            acc2 = tap_acc_f(make_float2(v3.x, v3.y), c3, acc2);
+            acc3 = tap_acc_f(make_float2(v3.z, v3.w), c3, acc3);
+        }
+        // unshifted frames: the intensity is the magnified pixel itself (:356)
+        const float *Pf = a.px + wh * f;
+        const float va0 = ina ? Pf[pixa] : 0.0f, vb0 = inb ? Pf[pixb] : 0.0f;
+        const float va1 = (ina && two) ? Pf[pixa + wh] : 0.0f;
+        const float vb1 = (inb && two) ? Pf[pixb + wh] : 0.0f;
+        const bool first = fi == 0 && a.first_init;
+        cumulate_f(a0, a1, a2, a3, a4, (a.wsum + acc0) * invwa, va0, first, a);
+        cumulate_f(b0, b1, b2, b3, b4, (a.wsum + acc2) * invwb, vb0, first, a);
+        if (two) {
+            cumulate_f(a0, a1, a2, a3, a4, (a.wsum + acc1) * invwa, va1, false, a);
+            cumulate_f(b0, b1, b2, b3, b4, (a.wsum + acc3) * invwb, vb1, false, a);
+        }
+    }
+    if (ina) {
+        a.out[pixa] = a0;
+        a.out[pixa + wh] = a1;
+        a.out[pixa + 2 * wh] = a2;
+        a.out[pixa + 3 * wh] = a3;
+        a.prev[pixa] = a4;
+    }
+    if (inb) {
+        a.out[pixb] = b0;
+        a.out[pixb + wh] = b1;
+        a.out[pixb + 2 * wh] = b2;
+        a.out[pixb + 3 * wh] = b3;
+        a.prev[pixb] = b4;
+    }
+}
+
 hipError_t launch_fht_rgc(const FhtRgcArgs &a, hipStream_t s)
 {
     if (a.nframes <= 0) return hipSuccess;
+    if (a.pair && fht_rgc2_lds_bytes(a) <= 65536) {
+        const int tx = (a.W + FT2 - 1) / FT2, ty = (a.H + FT2 - 1) / FT2;
+        FhtRgcArgs b = a;
+        b.tiles_x = tx;
+        b.tiles_y = ty;
+        hipLaunchKernelGGL(k_fht_rgc2, dim3(tx * ty), dim3(512), fht_rgc2_lds_bytes(a), s, b,
+                           a.taps, a.tapk);
+        return hipGetLastError();
+    }
     const size_t lds = fht_rgc_lds_bytes(a.TH, a.pitch, a.ntaps);
     hipLaunchKernelGGL(k_fht_rgc, dim3(a.tiles_x * a.tiles_y), dim3(256), lds, s, a, a.taps,
                        a.tapk);

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -56,6 +57,7 @@ void build_fht_taps(ModeState &ms, float radius, float tSS, float tSO)
     int pitch = 8;
     while (pitch < TW) pitch += 32;
     ms.fht_R = R;
+    ms.fht_Rhi = Rhi;
     ms.fht_TW = TW;
     ms.fht_pitch = pitch;
     ms.h_taps.clear();
@@ -246,6 +248,9 @@ int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
         a.sens_int = r.sens_int;
         a.int_weighting = r.int_weighting;
         a.R = ms.fht_R;
+        a.Rhi = ms.fht_Rhi;
+        a.pair = 1;
+        if (const char *ov = getenv("ESRRF_FHT_PAIR")) a.pair = atoi(ov) != 0;
         a.TW = ms.fht_TW;
         a.TH = ms.fht_TW;
         a.pitch = ms.fht_pitch;

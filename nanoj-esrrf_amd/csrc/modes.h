@@ -28,7 +28,7 @@ struct ModeState {
     int *d_tapk = nullptr;
     int ntaps = 0;
     float tap_wsum = 0.0f;
-    int fht_R = 0, fht_TW = 0, fht_pitch = 0;
+    int fht_R = 0, fht_Rhi = 0, fht_TW = 0, fht_pitch = 0;
     // 3-D
     float *d_st = nullptr;        // [3][nP] -theta (rad), shiftX, shiftY (LiveSRRF_CL.java:440-445)
     int *d_cell = nullptr;        // [2][nP] montage cell origins

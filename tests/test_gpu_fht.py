@@ -80,6 +80,31 @@ def test_fht_configs_parity(pkg, O, M, fwhm, S, w, h):
         assert np.abs(g[c] - og[c]).max() <= 2e-5 * scale, f"gradient {c} differs"
 
 
+@pytest.mark.parametrize("M,w,h,loads", [
+    (5, 32, 20, [(0, 5), (5, 9)]), (3, 21, 17, [(0, 7)]), (4, 30, 24, [(0, 1), (1, 4)]),
+])
+def test_fht_batched_rgc_matches_single_kernel(pkg, monkeypatch, M, w, h, loads):
+    """The batched FHT RGC kernel (two pixels and two frames per lane, the default)
+    reproduces the one-pixel one-frame kernel bit for bit: odd tails, one-frame loads,
+    continuation across loads and ragged tile edges included."""
+    n = loads[-1][1]
+    st = stack(n, w, h, seed=60 + M)
+    res = {}
+    for pair in ("0", "1"):
+        monkeypatch.setenv("ESRRF_FHT_PAIR", pair)
+        e = pkg.LiveSRRF_CL()
+        e.checkDevices()
+        e.initialise(w, h, M, 1.5, 2, max(b - a for a, b in loads), n, 20000,
+                     e.allCLdevices[0], True, True, True, None, 0, 1)
+        e.resetFramePosition()
+        e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
+        for a, b in loads:
+            e.calculateSRRF(st[a:b])
+        res[pair] = e.readSRRFbuffer()
+        e.release()
+    assert np.array_equal(res["0"], res["1"])
+
+
 def test_fht_no_intensity_weighting(pkg, O):
     st = stack(5, 20, 20, seed=17)
     got, _, _ = run_engine(pkg, st, 3, 1.5, 1, intW=False)
