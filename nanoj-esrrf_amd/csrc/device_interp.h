@@ -44,9 +44,9 @@ __device__ __forceinline__ float interp_tile(const float *t, int pitch, int ox, 
 #pragma unroll
             for (int i = 0; i <= 3; i++) {
                 const int u = u0 - 1 + i;
-                p = p + t[(v - oy) * pitch + (u - ox)] * cubic_w(x - (float)u);
+                p = fmaf(t[(v - oy) * pitch + (u - ox)], cubic_w(x - (float)u), p);
             }
-            q = q + p * cubic_w(y - (float)v);
+            q = fmaf(p, cubic_w(y - (float)v), q);
         }
     } else {
         const int xbase = (int)fminf((float)w - 2.0f, fmaxf(x, 0.0f));
@@ -57,9 +57,9 @@ __device__ __forceinline__ float interp_tile(const float *t, int pitch, int ox, 
         const float lr = t[(ybase - oy) * pitch + (xbase + 1 - ox)];
         const float ur = t[(ybase + 1 - oy) * pitch + (xbase + 1 - ox)];
         const float ul = t[(ybase + 1 - oy) * pitch + (xbase - ox)];
-        const float ua = ul + xf * (ur - ul);
-        const float la = ll + xf * (lr - ll);
-        q = la + yf * (ua - la);
+        const float ua = fmaf(xf, ur - ul, ul);
+        const float la = fmaf(xf, lr - ll, ll);
+        q = fmaf(yf, ua - la, la);
     }
     return q;
 }
@@ -81,9 +81,9 @@ __device__ __forceinline__ float interp_tile_s(const float *t, int pitch, int ox
 #pragma unroll
             for (int i = 0; i <= 3; i++) {
                 const int u = u0 - 1 + i;
-                p = p + t[S * ((v - oy) * pitch + (u - ox))] * cubic_w(x - (float)u);
+                p = fmaf(t[S * ((v - oy) * pitch + (u - ox))], cubic_w(x - (float)u), p);
             }
-            q = q + p * cubic_w(y - (float)v);
+            q = fmaf(p, cubic_w(y - (float)v), q);
         }
     } else {
         const int xbase = (int)fminf((float)w - 2.0f, fmaxf(x, 0.0f));
@@ -94,9 +94,9 @@ __device__ __forceinline__ float interp_tile_s(const float *t, int pitch, int ox
         const float lr = t[S * ((ybase - oy) * pitch + (xbase + 1 - ox))];
         const float ur = t[S * ((ybase + 1 - oy) * pitch + (xbase + 1 - ox))];
         const float ul = t[S * ((ybase + 1 - oy) * pitch + (xbase - ox))];
-        const float ua = ul + xf * (ur - ul);
-        const float la = ll + xf * (lr - ll);
-        q = la + yf * (ua - la);
+        const float ua = fmaf(xf, ur - ul, ul);
+        const float la = fmaf(xf, lr - ll, ll);
+        q = fmaf(yf, ua - la, la);
     }
     return q;
 }

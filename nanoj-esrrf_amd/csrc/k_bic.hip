@@ -509,11 +509,11 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         float pp = 0.0f;
-                        pp = pp + row[j * RAW_P + 0] * P.wx[0];
-                        pp = pp + row[j * RAW_P + 1] * P.wx[1];
-                        pp = pp + row[j * RAW_P + 2] * P.wx[2];
-                        pp = pp + row[j * RAW_P + 3] * P.wx[3];
-                        q = q + pp * P.wy[j];
+                        pp = fmaf(row[j * RAW_P + 0], P.wx[0], pp);
+                        pp = fmaf(row[j * RAW_P + 1], P.wx[1], pp);
+                        pp = fmaf(row[j * RAW_P + 2], P.wx[2], pp);
+                        pp = fmaf(row[j * RAW_P + 3], P.wx[3], pp);
+                        q = fmaf(pp, P.wy[j], q);
                     }
                     v = q;
                 } else {
@@ -547,21 +547,22 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
                     }
                     continue;
                 }
-                // temporal accumulation (liveSRRF.cl:363-393)
-                const float t = intw ? v * C : C;
-                const float e2 = intw ? (t * v) * C : C * C;
+                // temporal accumulation (liveSRRF.cl:363-393), every rounding spelled out
+                // (the frame-pair kernel's cumulate() does the same operations)
+                const float t = intw ? __fmul_rn(v, C) : C;
+                const float e2 = intw ? __fmul_rn(__fmul_rn(t, v), C) : __fmul_rn(C, C);
                 if (first) {
-                    s_avg[p] = t * rn;
-                    s_e2[p] = e2 * rn;
+                    s_avg[p] = __fmul_rn(t, rn);
+                    s_e2[p] = __fmul_rn(e2, rn);
                     s_tac[p] = 0.0f;
-                    s_int[p] = v * rn;
+                    s_int[p] = __fmul_rn(v, rn);
                 } else {
-                    s_avg[p] = s_avg[p] + t * rn;
-                    s_e2[p] = s_e2[p] + e2 * rn;
-                    s_tac[p] = s_tac[p] + (t * s_prv[p]) * rn1;
-                    s_int[p] = s_int[p] + v * rn;
+                    s_avg[p] = fmaf(t, rn, s_avg[p]);
+                    s_e2[p] = fmaf(e2, rn, s_e2[p]);
+                    s_tac[p] = fmaf(__fmul_rn(t, s_prv[p]), rn1, s_tac[p]);
+                    s_int[p] = fmaf(v, rn, s_int[p]);
                 }
-                s_prv[p] = prev_scaled ? t * rn1 : t;
+                s_prv[p] = prev_scaled ? __fmul_rn(t, rn1) : t;
             }
         }
         if (NBUF == 2) {
@@ -616,20 +617,20 @@ __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, flo
 {
     // temporal accumulation (liveSRRF.cl:363-393); Prev holds t/(n-1) (tac_prev_scaled)
     const float C = apply_sens(C0, a.sens, a.sens_int);
-    const float t = INTW ? v * C : C;
-    const float e2 = INTW ? (t * v) * C : C * C;
+    const float t = INTW ? __fmul_rn(v, C) : C;
+    const float e2 = INTW ? __fmul_rn(__fmul_rn(t, v), C) : __fmul_rn(C, C);
     if (first) {
-        avg = t * a.rn;
-        e2s = e2 * a.rn;
+        avg = __fmul_rn(t, a.rn);
+        e2s = __fmul_rn(e2, a.rn);
         tac = 0.0f;
-        in = v * a.rn;
+        in = __fmul_rn(v, a.rn);
     } else {
-        avg = avg + t * a.rn;
-        e2s = e2s + e2 * a.rn;
-        tac = tac + (t * prv) * a.rn1;
-        in = in + v * a.rn;
+        avg = fmaf(t, a.rn, avg);
+        e2s = fmaf(e2, a.rn, e2s);
+        tac = fmaf(__fmul_rn(t, prv), a.rn1, tac);
+        in = fmaf(v, a.rn, in);
     }
-    prv = t * a.rn1;
+    prv = __fmul_rn(t, a.rn1);
 }
 
 template <int PPW, int MODE, bool INTW>
@@ -797,11 +798,11 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
                             const float2 r2 = row[j * RAW_P + i];
-                            pa = pa + r2.x * P.wx[i];
-                            pb = pb + r2.y * P.wx[i];
+                            pa = fmaf(r2.x, P.wx[i], pa);
+                            pb = fmaf(r2.y, P.wx[i], pb);
                         }
-                        qa = qa + pa * P.wy[j];
-                        qb = qb + pb * P.wy[j];
+                        qa = fmaf(pa, P.wy[j], qa);
+                        qb = fmaf(pb, P.wy[j], qb);
                     }
                     va = qa;
                     vb = qb;
