@@ -15,6 +15,8 @@
 //   k3d_rgc       3-D radial gradient convergence + axial cubic + temporal cumulants
 //                 (live3DSRRF.cl:325-498), cumulants in registers across frames.  Frames
 //                 without drift share one geometry, evaluated once per chunk of FC frames.
+#include <cstdlib>
+
 #include "device_interp.h"
 #include "esrrf_internal.h"
 #include "mode3d_internal.h"
@@ -312,6 +314,162 @@ __global__ void __launch_bounds__(256) k3d_rgc(const Rgc3dArgs a)
     a.prev[vox] = s4;
 }
 
+// ---------------------------------------------------------------------------------------
+// Frames without drift: the geometry of a voxel (sample points, distances, weights, unit
+// directions e = d/|d|) is the same for every frame, so it is computed once per chunk of FC
+// frames and each frame pays only the sample itself.  With U a unit vector or zero,
+//     (1 - |e x U|) [e.U < 0]  =  1 + med3(-1, -|e x U|, -1 - e.U)
+// (|e x U|^2 + (e.U)^2 = 1, so -1 - e.U >= -|e x U| exactly when e.U <= 0; U = 0 gives 0
+// as the reference's GdotR < 0 test does), and CGLH = (sum W + sum W h) / sum W.  The window
+// loops visit, per sample row, only the x range the tSO gate can pass, and every frame's
+// load shares one 32-bit offset from a per-frame uniform base.
+// ---------------------------------------------------------------------------------------
+template <int FC>
+__global__ void __launch_bounds__(256) k3d_rgc2(const Rgc3dArgs a)
+{
+    const int xM = blockIdx.x * 64 + threadIdx.x;
+    const int yM = blockIdx.y * 4 + threadIdx.y;
+    const int zM = blockIdx.z;
+    if (xM >= a.wM || yM >= a.hM) return;
+    const int w = a.w, h = a.h, nP = a.nP, M = a.M;
+    const long whM = (long)a.wM * a.hM;
+    const long whdM = whM * nP * M;
+    const long vox = whM * zM + (long)yM * a.wM + xM;
+    const long whI = 4L * w * h;
+    const int wI = 2 * w;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+    if (a.count0 > 0) {
+        s0 = a.out[vox];
+        s1 = a.out[vox + whdM];
+        s2 = a.out[vox + 2 * whdM];
+        s3 = a.out[vox + 3 * whdM];
+        s4 = a.prev[vox];
+    }
+    const float zc = (float)((zM + 0.5) / M);
+    int z0;
+    const float zcof = zc - 0.5f;
+    if (zcof < 0.0f) z0 = 0;
+    else if (zcof >= (float)(nP - 1)) z0 = nP - 2;
+    else z0 = (int)floorf(zcof);
+    const int zm1 = max(z0 - 1, 0), zp2 = min(z0 + 1, nP - 1);
+    const float z01 = zc - ((float)z0 + 0.5f);
+    const int jlo = -(int)(2.0f * a.radius), jhi = (int)(2.0f * a.radius + 1.0f);
+    const int klo = -(int)a.radius, khi = (int)a.radius + 1;
+    const float nFf = (float)a.nFor, nF1 = (float)(a.nFor - 1);
+    // sample-point geometry (liveSRRF 3-D: xc = xM/M + 0.5/M with zero drift)
+    const float xc = (float)((xM + 0.5) / M + 0.0);
+    const float yc = (float)((yM + 0.5) / M + 0.0);
+    const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc), zi = (int)zc;
+    const float tSO = a.tSO, tSO2 = tSO * tSO;
+    const size_t fbytes = (size_t)nP * whI * sizeof(float4);   // one frame of U
+
+    for (int f0 = 0; f0 < a.nframes; f0 += FC) {
+        const int nf = min(FC, a.nframes - f0);
+        const int c00 = a.count0 + f0;
+        const char *Ub = reinterpret_cast<const char *>(a.U + (long)(a.frame0 + f0) * nP * whI);
+        float acc[FC];
+#pragma unroll
+        for (int q = 0; q < FC; q++) acc[q] = 0.0f;
+        float sumW = 0.0f;
+        for (int j = jlo; j <= jhi; j++) {
+            const int byj = by + j;
+            const float vy = (float)byj / 2.0f;
+            if (!(vy > 0.0f && vy < (float)h)) continue;
+            const int m = byj - 1;                     // (int)(2 (vy - 0.5)), vy >= 0.5
+            const float dy = vy - yc;
+            const float dy2 = __fmul_rn(dy, dy);
+            for (int k = klo; k <= khi; k++) {
+                const float vz = (float)zi + (float)k;
+                if (!(vz > 0.0f && vz < (float)nP)) continue;
+                const float dz = vz - zc;
+                const float dz2 = __fmul_rn(dz, dz);
+                // x range of the tSO gate, one extra sample each side for rounding
+                const float rem = tSO2 - dy2 - dz2;
+                if (rem < -1e-3f * tSO2) continue;
+                const float rx = sqrtf(fmaxf(rem, 0.0f));
+                const int ilo = max(jlo, (int)floorf(2.0f * (xc - rx)) - bx - 1);
+                const int ihi = min(jhi, (int)ceilf(2.0f * (xc + rx)) - bx + 1);
+                const long rowoff = whI * (int)vz + (long)m * wI;
+                for (int i = ilo; i <= ihi; i++) {
+                    const int bxi = bx + i;
+                    const float vx = (float)bxi / 2.0f;
+                    if (!(vx > 0.0f && vx < (float)w)) continue;
+                    const float dx = vx - xc;
+                    float d2 = __fmul_rn(dx, dx);
+                    d2 = __fadd_rn(d2, dy2);
+                    d2 = __fadd_rn(d2, dz2);
+                    const float d = sqrtf(d2);
+                    if (!(d != 0.0f && d <= tSO)) continue;
+                    float dw = d * expf(-(d * d) / a.tSS);
+                    dw = dw * dw * dw * dw;
+                    sumW += dw;
+                    const float invd = __builtin_amdgcn_rcpf(d);
+                    const float ex = dx * invd, ey = dy * invd, ez = dz * invd;
+                    const unsigned off = (unsigned)(rowoff + (bxi - 1)) * (unsigned)sizeof(float4);
+#pragma unroll
+                    for (int q = 0; q < FC; q++) {
+                        if (q < nf) {
+                            const float4 u = *reinterpret_cast<const float4 *>(
+                                Ub + (size_t)q * fbytes + off);
+                            float c1 = fmaf(u.x, -ex, -1.0f);
+                            c1 = fmaf(u.y, -ey, c1);
+                            c1 = fmaf(u.z, -ez, c1);
+                            const float X = fmaf(u.y, ez, -u.z * ey);
+                            const float Y = fmaf(u.z, ex, -u.x * ez);
+                            const float Z = fmaf(u.x, ey, -u.y * ex);
+                            const float sn = __builtin_amdgcn_sqrtf(fmaf(X, X, fmaf(Y, Y, Z * Z)));
+                            acc[q] = fmaf(dw, __builtin_amdgcn_fmed3f(-1.0f, -sn, c1), acc[q]);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < FC; q++) {
+            if (q < nf) {
+                const int fi = f0 + q;
+                float C = (sumW + acc[q]) / sumW;
+                C = apply_sens3(C, a.sens, a.sens_int);
+                const float *Lf = a.L + (long)fi * nP * whM + (long)yM * a.wM + xM;
+                const float v_1 = Lf[whM * zm1], v0 = Lf[whM * z0];
+                const float v1 = Lf[whM * (z0 + 1)], v2 = Lf[whM * zp2];
+                // axial cubic (live3DSRRF.cl:448-452), left-to-right as written
+                float aa = __fmul_rn(-0.5f, v_1);
+                aa = __fadd_rn(aa, __fmul_rn(1.5f, v0));
+                aa = __fsub_rn(aa, __fmul_rn(1.5f, v1));
+                aa = __fadd_rn(aa, __fmul_rn(0.5f, v2));
+                float bb = __fsub_rn(v_1, __fmul_rn(2.5f, v0));
+                bb = __fadd_rn(bb, __fmul_rn(2.0f, v1));
+                bb = __fsub_rn(bb, __fmul_rn(0.5f, v2));
+                const float cc = __fadd_rn(__fmul_rn(-0.5f, v_1), __fmul_rn(0.5f, v1));
+                float v = __fmul_rn(__fmul_rn(__fmul_rn(aa, z01), z01), z01);
+                v = __fadd_rn(v, __fmul_rn(__fmul_rn(bb, z01), z01));
+                v = __fadd_rn(v, __fmul_rn(cc, z01));
+                v = __fadd_rn(v, v0);
+                const float t = a.int_weighting ? v * C : C;
+                const float e2 = a.int_weighting ? ((t * v) * C) : C * C;
+                if (c00 + q == 0) {
+                    s0 = t / nFf;
+                    s1 = e2 / nFf;
+                    s2 = 0.0f;
+                    s3 = v / nFf;
+                } else {
+                    s0 = s0 + t / nFf;
+                    s1 = s1 + e2 / nFf;
+                    s2 = s2 + (t * s4) / nF1;
+                    s3 = s3 + v / nFf;
+                }
+                s4 = t;   // single 1/(n-1) in TAC2 (live3DSRRF.cl:472)
+            }
+        }
+    }
+    a.out[vox] = s0;
+    a.out[vox + whdM] = s1;
+    a.out[vox + 2 * whdM] = s2;
+    a.out[vox + 3 * whdM] = s3;
+    a.prev[vox] = s4;
+}
+
 // Gated samples of one frame without drift, summed over all voxels.
 __global__ void __launch_bounds__(256) k3d_count(int w, int h, int nP, int M, float radius,
                                                  float tSO, unsigned long long *total)
@@ -385,6 +543,12 @@ hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long 
     return hipGetLastError();
 }
 
+static int getenv_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
 hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s)
 {
     if (a.nframes <= 0) return hipSuccess;
@@ -392,7 +556,11 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s)
     hipLaunchKernelGGL(k3d_lateral, dim3(grid1(nl)), dim3(256), 0, s, a.px, a.L, a.w, a.h, a.nP,
                        a.M, a.frame0, a.nframes, a.drift, a.count0, a.nFor);
     const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.M);
-    if (drift_zero)
+    // (the fast kernel addresses one frame of U with 32-bit byte offsets)
+    const bool off32 = (double)a.nP * 4.0 * a.w * a.h * sizeof(float4) < 4294967296.0;
+    if (drift_zero && off32 && getenv_int("ESRRF_3D_FAST", 1))
+        hipLaunchKernelGGL(k3d_rgc2<8>, grid, dim3(64, 4), 0, s, a);
+    else if (drift_zero)
         hipLaunchKernelGGL(k3d_rgc<8>, grid, dim3(64, 4), 0, s, a);
     else
         hipLaunchKernelGGL(k3d_rgc<1>, grid, dim3(64, 4), 0, s, a);

@@ -96,6 +96,39 @@ def test_3d_zero_calibration_parity(pkg, O, w, h, nP, M, fwhm, S):
     assert_parity(got, exp, f"3D {w}x{h}x{nP} M{M} fwhm{fwhm} S{S}")
 
 
+@pytest.mark.parametrize("w,h,nP,M,fwhm,loads", [
+    (20, 16, 9, 4, 1.5, [(0, 11)]), (16, 12, 5, 2, 2.0, [(0, 5), (5, 13)]),
+    (12, 14, 3, 3, 1.5, [(0, 1), (1, 9)]),
+])
+def test_3d_fast_kernel_matches_window_kernel(pkg, monkeypatch, w, h, nP, M, fwhm, loads):
+    """The drift-free 3-D kernel (geometry once per chunk of frames, gate folded into med3,
+    x range limited to the tSO sphere) against the window-order kernel on the same data:
+    chunks of 8 frames with ragged tails and continuation across loads.  The two sum the
+    samples in different orders, so they agree to float rounding (1e-5 of each channel's
+    maximum; TAC2 and VAR are products of per-frame values), not bit for bit."""
+    n = loads[-1][1]
+    planes = planes_stack(n, w, h, nP, seed=70 + w + nP)
+    mont = montage(planes, np.arange(nP))
+    cd = calib_dict(*zero_calib(nP))
+    res = {}
+    for fast in ("0", "1"):
+        monkeypatch.setenv("ESRRF_3D_FAST", fast)
+        e = pkg.LiveSRRF_CL()
+        e.checkDevices()
+        e.initialise(3 * w, 3 * h, M, fwhm, 2, max(b - a for a, b in loads), n, 20000,
+                     e.allCLdevices[0], True, True, False, cd, 0, 1)
+        e.resetFramePosition()
+        e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
+        for a, b in loads:
+            e.calculateSRRF(mont[a:b])
+        res[fast] = np.asarray(e.readSRRFbuffer()).reshape(4, -1)
+        e.release()
+    for c in range(4):
+        scale = np.abs(res["0"][c]).max()
+        err = np.abs(res["1"][c] - res["0"][c]).max()
+        assert err <= 1e-5 * scale, f"channel {c}: {err:.3g} vs scale {scale:.3g}"
+
+
 def test_3d_calibrated_montage_parity(pkg, O):
     """Rotation, sub-pixel shifts, permuted ROI cells, intensity scaling, background, and a
     montage whose width and height are not multiples of 3."""
