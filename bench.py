@@ -258,8 +258,9 @@ def bench_3d(a, pkg, torch):
     ms_step = ev0.elapsed_time(ev1) / a.steps
     rgc_ms, rgc_n = ctypes.c_double(), ctypes.c_long()
     eng.check(L.esrrf_profile_read(h, 0, ctypes.byref(rgc_ms), ctypes.byref(rgc_n)))
-    eng.close()
     rgc_avg = rgc_ms.value / max(rgc_n.value, 1)
+    kname = eng.rgc_kernel()
+    eng.close()
     flops = 24.0 * nact.value * nf
     return {"metric": "eSRRF 3-D output Mvoxel/s (x4 lateral and axial, 512 frames)",
             "value": round(nvox * nf / (ms_step / 1e3) / 1e6, 1), "unit": "Mvoxel/s",
@@ -267,7 +268,7 @@ def bench_3d(a, pkg, torch):
             "dtype": "f32", "data": "synthetic (seeded 3-D blinking filaments, uint16)",
             "config": {"workload": "3-D eSRRF: 512 frames x 9 planes x 128x128 uint16 (3x3 "
                                    "montage 384x384) -> 36 x 512 x 512, M=4, fwhm 1.5, S=2, MP on"},
-            "roofline": {"bound": "valu", "kernel": "k3d_rgc", "unit": "TFLOP/s",
+            "roofline": {"bound": "valu", "kernel": kname, "unit": "TFLOP/s",
                          "achieved": round(flops / (rgc_avg / 1e3) / 1e12, 2),
                          "peak": PEAK_FP32_TFLOPS,
                          "frac": round(flops / (rgc_avg / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 4),
@@ -357,6 +358,7 @@ def main():
     up_ms, up_n = ctypes.c_double(), ctypes.c_long()
     eng.check(L.esrrf_profile_read(h, 2, ctypes.byref(up_ms), ctypes.byref(up_n)))
     eng.check(L.esrrf_profile_enable(h, 0))
+    kname = eng.rgc_kernel()
 
     ms_step = ms_total / a.steps
     outpx = W * M * H * M
@@ -385,8 +387,7 @@ def main():
                    "frames": nf, "width": W, "height": H, "magnification": M,
                    "fwhm": FWHM, "sensitivity": SENS},
         "roofline": {"bound": "valu",
-                     "kernel": "k_fht_rgc" if fht else
-                     ("k_bic_rgc2" if shape and shape["frames_per_pass"] == 2 else "k_bic_rgc"),
+                     "kernel": kname,
                      "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": None if fht else load_traffic(),

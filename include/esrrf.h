@@ -141,6 +141,10 @@ int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_ld
  * kernel (FHT, 3-D, or a geometry outside the tiled kernel's range). */
 int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
                        int *frames_per_pass);
+/* Name of the RGC kernel the engine's last calculate launched, e.g. "k_bic_rgc2",
+ * "k_fht_rgc2", "k3d_rgc3" ("" before the first launch).  No counterpart in the reference
+ * (its single OpenCL kernel per mode needs none); for profiling and tests. */
+int esrrf_rgc_kernel(esrrf_engine *e, char *buf, int len);
 /* ---- Parameter sweep (gui/ParametersSweep_.java:328-519) --------------------------------
  * Every (magnification, frame count, sensitivity, fwhm) configuration of a sweep over the
  * first nf frames of one stack, 2-D BIC, no drift; for each the four channels that

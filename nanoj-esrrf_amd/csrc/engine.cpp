@@ -77,6 +77,7 @@ struct esrrf_engine {
     bool has_mp = false;
     bool profiling = false;
     ProfSlot prof[3];
+    const char *rgc_kernel = "";   // RGC kernel of the last launch (BIC mode)
 };
 
 namespace {
@@ -197,7 +198,7 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
         a.int_weighting = e->p.int_weighting ? 1 : 0;
         a.stage = e->stage;
         ProfScope ps(e, 0);
-        HIPCHK(e, launch_bic_rgc(a, e->shape.ppw, e->stream));
+        HIPCHK(e, launch_bic_rgc(a, e->shape.ppw, e->stream, &e->rgc_kernel));
         return ESRRF_OK;
     }
     GenericRgcArgs g{};
@@ -226,6 +227,7 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
     g.tSO = e->tSO;
     ProfScope ps(e, 0);
     HIPCHK(e, launch_generic_rgc(g, e->stream));
+    e->rgc_kernel = "k_generic_rgc";
     return ESRRF_OK;
 }
 
@@ -784,6 +786,15 @@ int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
     if (ppw) *ppw = e->shape.ppw;
     if (taps_in_lds) *taps_in_lds = e->shape.taps_in_lds;
     if (frames_per_pass) *frames_per_pass = e->shape.fp;
+    return ESRRF_OK;
+}
+
+int esrrf_rgc_kernel(esrrf_engine *e, char *buf, int len)
+{
+    if (!e || !buf || len < 1) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const char *k = e->mode == MODE_BIC ? e->rgc_kernel : e->ms.rgc_kernel;
+    snprintf(buf, (size_t)len, "%s", k);
     return ESRRF_OK;
 }
 

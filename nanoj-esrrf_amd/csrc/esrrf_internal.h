@@ -87,12 +87,43 @@ struct GenericRgcArgs {
     float radius, tSS, tSO;
 };
 
+// Geometry of one (y-class, x-class, magnified z-slice, lateral sub-pixel phase) of the
+// tiled 3-D RGC kernel.  Its taps are listed in live3DSRRF.cl's j, i, k loop order.
+struct Phase3 {
+    int first_tap;  // index into tapv / tapo / tapk
+    int ntaps;      // a multiple of 4 (zero-weight padding at the end)
+    float wsum;     // sum of W over the phase's taps, in the reference's order
+    int zlo, nz;    // planes [zlo, zlo + nz) of the z-slice's LDS tile
+    int sx, sy;     // lateral sub-pixel phase (xM mod M, yM mod M)
+    int pad;
+};
+
+// Tap tables and LDS tile layout of the tiled 3-D RGC kernel (built once per engine).
+//   tapv : (ex, ey, ez, W) unit direction from the voxel centre to the sample point and
+//          its distance weight (d exp(-d^2 / tSS))^4 (live3DSRRF.cl:383-384)
+//   tapo : float4 index of the sample's unit gradient in the LDS tile, relative to the
+//          lane's base index 2 ly P + lx
+//   tapk : (kx_rel + 128) | (ky_rel + 128) << 8, the sample's 2x-grid offset from (2X, 2Y)
+struct Rgc3Tables {
+    const float4 *tapv;
+    const int *tapo;
+    const int *tapk;
+    const Phase3 *phases;   // [ncls_y][ncls_x][nP M][M M]
+    int R2lo, R2hi, G0;
+    int TW, TH;             // tile extent on the 2x grid (columns, rows)
+    int PW, P, PZ, NZ;      // half-row width, row pitch, plane pitch (float4), planes
+    int xs_log2, ncls_x, ncls_y, nZM;
+    int gtaps;              // LDS tap slots per block (largest group of 4 phases + spares)
+};
+
 // Host helpers implemented in the .hip files (all launches on `s`).
 hipError_t launch_convert_frames(const void *src, int dtype, float *dst, long n, hipStream_t s);
 hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s);
 hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h, int f,
                            hipStream_t s);
-hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s);
+// *kname (when given) receives the name of the kernel launched.
+hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s,
+                          const char **kname = nullptr);
 size_t bic_pair_lds_bytes(const BicRgcArgs &a);   // LDS of the frame-pair kernel
 hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
 hipError_t launch_var(float *out, long nvox, hipStream_t s);

@@ -53,7 +53,7 @@ hipError_t launch_fht_upsample(const FhtPlan &p, const float *frames, float *out
 hipError_t launch_fht_prep(const float *px, float2 *U, int W, int H, int nF, hipStream_t s);
 hipError_t launch_fht_grad(const float *px, float *gx, float *gy, int W, int H, hipStream_t s);
 size_t fht_rgc_lds_bytes(int TH, int pitch, int ntaps);
-hipError_t launch_fht_rgc(const FhtRgcArgs &a, hipStream_t s);
+hipError_t launch_fht_rgc(const FhtRgcArgs &a, hipStream_t s, const char **kname = nullptr);
 hipError_t launch_fht_generic(const FhtGenericArgs &a, hipStream_t s);
 
 }  // namespace esrrf

@@ -221,3 +221,154 @@ long long count_active_samples_bic(int w, int h, int M, float radius, float tSO)
 }
 
 }  // namespace esrrf
+
+namespace esrrf {
+
+// ---------------------------------------------------------------------------------------
+// Tiled 3-D RGC tables.  For a voxel (xM, yM, zM) without drift the sample points of
+// live3DSRRF.cl:366-387 are vx = (floor(2 xc) + i)/2, vy = (floor(2 yc) + j)/2 and
+// vz = floor(zc) + k; (dx, dy) depend on the lateral phase and, through the rounding of
+// xc = (float)((xM + 0.5)/M), on the binade of X = xM / M, exactly as in the 2-D tables
+// (for M a power of two xc is exact and one class serves every X); dz depends on zM alone.
+// The LDS tile of a block holds the unit gradients of planes [zlo, zlo + nz) over the 2x-
+// grid rectangle its 8x8 macro-pixels reach, each row split into its even and odd columns
+// so that the stride-2 sample pattern of a wave reads consecutive float4s.
+// ---------------------------------------------------------------------------------------
+bool build_3d_geometry(int w, int h, int nP, int M, float radius, float tSS, float tSO,
+                       Geometry3 &g)
+{
+    g.R2lo = -(int)(2.0f * radius);
+    g.R2hi = (int)(2.0f * radius + 1.0f);
+    const int klo = -(int)radius, khi = (int)radius + 1;
+    if (M < 1 || nP < 2 || g.R2hi - g.R2lo > 60 || khi - klo > 40) return false;
+    g.G0 = 2 * ((-g.R2lo + 1) / 2);
+    // lane columns 2 lx + kx_rel + G0 with kx_rel in [R2lo, R2hi + 1], lx in [0, 8)
+    g.TW = g.TH = 16 + g.R2hi + g.G0;
+    g.PW = (g.TW + 1) / 2;
+    // row pitch = 4 (mod 8) float4s: rows 2 ly apart land in opposite 128-byte bank halves
+    g.P = 2 * g.PW;
+    while (g.P % 8 != 4) g.P++;
+    g.PZ = g.TH * g.P;
+    int xs = 8;
+    while (xs < 1 - g.R2lo) xs *= 2;
+    int lg = 0;
+    while ((1 << lg) < xs) lg++;
+    g.xs_log2 = lg;
+    const bool pow2 = (M & (M - 1)) == 0;
+    g.ncls_x = pow2 ? 1 : class_of(w > 0 ? w - 1 : 0, lg) + 1;
+    g.ncls_y = pow2 ? 1 : class_of(h > 0 ? h - 1 : 0, lg) + 1;
+    g.nZM = nP * M;
+    const int MM = M * M;
+
+    // planes each z-slice can reach (|dz| <= tSO, 0 < vz < nP)
+    std::vector<int> zlo(g.nZM), nz(g.nZM);
+    g.NZ = 0;
+    for (int zM = 0; zM < g.nZM; zM++) {
+        const float zc = (float)((zM + 0.5) / M);
+        int lo = 1 << 30, hi = -1;
+        for (int k = klo; k <= khi; k++) {
+            const float vz = (float)((int)zc) + (float)k;
+            if (!(vz > 0.0f && vz < (float)nP)) continue;
+            if (!(std::fabs(vz - zc) <= tSO + 1e-5f)) continue;   // margin: d may round below |dz|
+            lo = std::min(lo, (int)vz);
+            hi = std::max(hi, (int)vz);
+        }
+        zlo[zM] = hi >= lo ? lo : 1;
+        nz[zM] = hi >= lo ? hi - lo + 1 : 0;
+        g.NZ = std::max(g.NZ, nz[zM]);
+    }
+
+    g.tapv.clear();
+    g.tapo.clear();
+    g.tapk.clear();
+    g.phases.assign((size_t)g.ncls_y * g.ncls_x * g.nZM * MM, Phase3{});
+    g.max_group_taps = 0;
+    int max_off = 0;
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int cy = 0; cy < g.ncls_y; cy++) {
+        for (int cx = 0; cx < g.ncls_x; cx++) {
+            const int Xr = pow2 ? 8 : rep_of(cx, lg), Yr = pow2 ? 8 : rep_of(cy, lg);
+            for (int zM = 0; zM < g.nZM; zM++) {
+                const float zc = (float)((zM + 0.5) / M);
+                const int zi = (int)zc;
+                Phase3 *PB = &g.phases[(((size_t)cy * g.ncls_x + cx) * g.nZM + zM) * MM];
+                for (int sy = 0; sy < M; sy++) {
+                    for (int sx = 0; sx < M; sx++) {
+                        Phase3 &P = PB[sy * M + sx];
+                        const int xM = Xr * M + sx, yM = Yr * M + sy;
+                        const float xc = (float)((xM + 0.5) / M);
+                        const float yc = (float)((yM + 0.5) / M);
+                        const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc);
+                        P.first_tap = (int)g.tapv.size();
+                        float wsum = 0.0f;
+                        for (int j = g.R2lo; j <= g.R2hi; j++) {
+                            const float vy = ((float)by + (float)j) / 2.0f;
+                            for (int i = g.R2lo; i <= g.R2hi; i++) {
+                                const float vx = ((float)bx + (float)i) / 2.0f;
+                                for (int k = klo; k <= khi; k++) {
+                                    const float vz = (float)zi + (float)k;
+                                    if (!(vz > 0.0f && vz < (float)nP)) continue;
+                                    volatile float dx = vx - xc;
+                                    volatile float dy = vy - yc;
+                                    volatile float dz = vz - zc;
+                                    volatile float d2 = dx * dx;
+                                    volatile float t2 = dy * dy;
+                                    d2 = d2 + t2;
+                                    t2 = dz * dz;
+                                    d2 = d2 + t2;
+                                    const float d = std::sqrt((float)d2);
+                                    if (!(d != 0.0f && d <= tSO)) continue;
+                                    const int kz = (int)vz - zlo[zM];
+                                    if (kz < 0 || kz >= nz[zM]) return false;
+                                    // distanceWeight (live3DSRRF.cl:383-384), left to right
+                                    volatile float e = -(d * d);
+                                    e = e / tSS;
+                                    volatile float w1 = d * std::exp((float)e);
+                                    volatile float W = w1 * w1;
+                                    W = W * w1;
+                                    W = W * w1;
+                                    wsum = wsum + (float)W;
+                                    const int kxr = bx - 2 * Xr + i, kyr = by - 2 * Yr + j;
+                                    const int gx = kxr + g.G0, gy = kyr + g.G0;
+                                    const int off = kz * g.PZ + gy * g.P + (gx & 1) * g.PW + (gx >> 1);
+                                    max_off = std::max(max_off, off);
+                                    g.tapv.push_back(make_float4((float)dx / d, (float)dy / d,
+                                                                 (float)dz / d, (float)W));
+                                    g.tapo.push_back(off);
+                                    g.tapk.push_back((kxr + 128) | ((kyr + 128) << 8));
+                                }
+                            }
+                        }
+                        while (((int)g.tapv.size() - P.first_tap) % 4 != 0) {
+                            g.tapv.push_back(zero4);
+                            g.tapo.push_back(0);
+                            g.tapk.push_back(128 | (128 << 8));
+                        }
+                        P.ntaps = (int)g.tapv.size() - P.first_tap;
+                        P.wsum = wsum;
+                        P.zlo = zlo[zM];
+                        P.nz = nz[zM];
+                        P.sx = sx;
+                        P.sy = sy;
+                    }
+                }
+                for (int g0 = 0; g0 < MM; g0 += 4) {
+                    const int gl = std::min(MM, g0 + 4) - 1;
+                    g.max_group_taps = std::max(
+                        g.max_group_taps, PB[gl].first_tap + PB[gl].ntaps - PB[g0].first_tap);
+                }
+            }
+        }
+    }
+    for (int k = 0; k < TAP_SPARE; k++) {
+        g.tapv.push_back(zero4);
+        g.tapo.push_back(0);
+        g.tapk.push_back(128 | (128 << 8));
+    }
+    // every lane's reads (base 2 ly P + lx, lx, ly < 8) stay inside the NZ-plane tile
+    if (max_off + 14 * g.P + 7 >= g.NZ * g.PZ) return false;
+    if ((int)(2 * g.PW) > g.P) return false;
+    return true;
+}
+
+}  // namespace esrrf

@@ -25,3 +25,23 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
 long long count_active_samples_bic(int w, int h, int M, float radius, float tSO);
 
 }  // namespace esrrf
+
+namespace esrrf {
+
+// Host copy of the tiled 3-D RGC kernel's tables (Rgc3Tables without device pointers).
+struct Geometry3 {
+    int R2lo = 0, R2hi = 0, G0 = 0;
+    int TW = 0, TH = 0, PW = 0, P = 0, PZ = 0, NZ = 0;
+    int xs_log2 = 3, ncls_x = 1, ncls_y = 1, nZM = 0;
+    int max_group_taps = 0;   // taps of the largest group of 4 consecutive phases
+    std::vector<float4> tapv;
+    std::vector<int> tapo, tapk;
+    std::vector<Phase3> phases;
+};
+
+// Tables for frames without drift; false when the window or tile exceeds the kernel's
+// limits (the per-voxel kernel is used then).
+bool build_3d_geometry(int w, int h, int nP, int M, float radius, float tSS, float tSO,
+                       Geometry3 &g);
+
+}  // namespace esrrf

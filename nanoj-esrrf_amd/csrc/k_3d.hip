@@ -470,6 +470,244 @@ __global__ void __launch_bounds__(256) k3d_rgc2(const Rgc3dArgs a)
     a.prev[vox] = s4;
 }
 
+// ---------------------------------------------------------------------------------------
+// Tiled 3-D RGC for frames without drift.  A block of 4 waves covers an 8x8 macro-pixel tile
+// of one magnified z-slice; wave w takes lateral phase 4 blockIdx.y + w with one lane per
+// macro-pixel, so every tap (LDS offset, unit direction e, weight W) is wave-uniform and is
+// read from the block's LDS copy of its phases' tables.  Each frame's unit gradients over the
+// planes the slice reaches are staged in LDS (rows split into even and odd columns, so a
+// wave's stride-2 reads are consecutive float4s); the next frame's are loaded into registers
+// while the current one is evaluated.  Per sample, with U the unit gradient (or 0):
+//     c = U.e,  p = U - c e,  |e x U| = |p|,  h = med3(-1, -|p|, -1 - c)
+// (h = -|e x U| when c < 0, else -1; U = 0 gives -1), and
+//     CGLH = (sum W + sum W h) / sum_gated W
+// equals the reference's sum over GdotR < 0 of W (1 - |e x G| / |G|) over its weight sum
+// (live3DSRRF.cl:366-411).
+// ---------------------------------------------------------------------------------------
+constexpr int K3_FE = 16;   // tile elements each thread loads per frame (4096 per block)
+
+__device__ __forceinline__ int axis_class3(int X0, int xs_log2)
+{
+    if (X0 < (1 << xs_log2)) return 0;
+    return 32 - __clz(X0) - xs_log2;   // floor(log2 X0) - xs_log2 + 1
+}
+
+__device__ __forceinline__ float tap3_acc(const float4 u, const float4 q, float acc)
+{
+    float c = u.x * q.x;
+    c = fmaf(u.y, q.y, c);
+    c = fmaf(u.z, q.z, c);
+    const float px = fmaf(-c, q.x, u.x);
+    const float py = fmaf(-c, q.y, u.y);
+    const float pz = fmaf(-c, q.z, u.z);
+    float n2 = px * px;
+    n2 = fmaf(py, py, n2);
+    n2 = fmaf(pz, pz, n2);
+    const float sn = __builtin_amdgcn_sqrtf(n2);
+    return fmaf(q.w, __builtin_amdgcn_fmed3f(-1.0f, -sn, -1.0f - c), acc);
+}
+
+__global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tables t)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4 *lU = reinterpret_cast<float4 *>(smem);
+    float4 *lTv = lU + t.NZ * t.PZ;
+    int *lTo = reinterpret_cast<int *>(lTv + t.gtaps);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lx = lane & 7, ly = lane >> 3;
+    const int w = a.w, h = a.h, nP = a.nP, M = a.M, MM = M * M;
+    const int tiles_x = (w + 7) >> 3;
+    const int tX = blockIdx.x % tiles_x, tY = blockIdx.x / tiles_x;
+    const int X0 = tX << 3, Y0 = tY << 3, X = X0 + lx, Y = Y0 + ly;
+    const int zM = blockIdx.z;
+    const int ph0 = blockIdx.y * 4;
+    const int phN = min(MM, ph0 + 4);
+    const int ph = ph0 + wave;
+    const bool active = ph < phN;
+    int cls = 0;
+    if (t.ncls_x > 1 || t.ncls_y > 1)
+        cls = axis_class3(Y0, t.xs_log2) * t.ncls_x + axis_class3(X0, t.xs_log2);
+    const Phase3 *PB = t.phases + ((long)cls * t.nZM + zM) * MM;
+
+    // the tap tables of the block's phases (plus the spares after them)
+    const int tg0 = PB[ph0].first_tap;
+    const int tg1 = PB[phN - 1].first_tap + PB[phN - 1].ntaps;
+    for (int e = tid; e < tg1 - tg0 + TAP_SPARE; e += 256) {
+        lTv[e] = t.tapv[tg0 + e];
+        lTo[e] = t.tapo[tg0 + e];
+    }
+
+    // fill plan: tile element e -> (LDS index, offset of its unit gradient in one frame;
+    // -1 outside the 0 < vx < w, 0 < vy < h gates, where the tile holds 0)
+    const int zlo = PB[ph0].zlo, nz = PB[ph0].nz;
+    const int wI = 2 * w, hI = 2 * h, whI = wI * hI;
+    const int per_plane = t.TH * t.TW;
+    const int nel = nz * per_plane;
+    int gofs[K3_FE], lidx[K3_FE];
+#pragma unroll
+    for (int q = 0; q < K3_FE; q++) {
+        const int e = tid + q * 256;
+        gofs[q] = -1;
+        lidx[q] = -1;
+        if (e < nel) {
+            const int kz = e / per_plane;
+            const int r = e - kz * per_plane;
+            const int gy = r / t.TW, gx = r - gy * t.TW;
+            const int bxi = 2 * X0 - t.G0 + gx, byj = 2 * Y0 - t.G0 + gy;
+            lidx[q] = kz * t.PZ + gy * t.P + (gx & 1) * t.PW + (gx >> 1);
+            if (bxi >= 1 && bxi <= wI - 1 && byj >= 1 && byj <= hI - 1)
+                gofs[q] = (zlo + kz) * whI + (byj - 1) * wI + (bxi - 1);
+        }
+    }
+
+    // the lane's voxel, its gated weight sum, and its cumulants
+    int ft = 0, nt = 0, sx = 0, sy = 0;
+    float wsum = 0.0f, wsg = 0.0f;
+    if (active) {
+        const Phase3 P = PB[ph];
+        ft = P.first_tap - tg0;
+        nt = P.ntaps;
+        wsum = wsg = P.wsum;
+        sx = P.sx;
+        sy = P.sy;
+        const bool interior = (2 * X0 + t.R2lo >= 1) && (2 * (X0 + 7) + 1 + t.R2hi <= wI - 1) &&
+                              (2 * Y0 + t.R2lo >= 1) && (2 * (Y0 + 7) + 1 + t.R2hi <= hI - 1);
+        if (!interior) {
+            // border: only the samples inside the gates count, in the reference's order
+            wsg = 0.0f;
+            for (int k = 0; k < nt; k++) {
+                const int kk = t.tapk[P.first_tap + k];
+                const int kx = 2 * X + (kk & 255) - 128, ky = 2 * Y + ((kk >> 8) & 255) - 128;
+                if (kx >= 1 && kx <= wI - 1 && ky >= 1 && ky <= hI - 1)
+                    wsg += t.tapv[P.first_tap + k].w;
+            }
+        }
+    }
+    const bool inimg = active && X < w && Y < h;
+    const int xM = X * M + sx, yM = Y * M + sy;
+    const long whM = (long)a.wM * a.hM;
+    const long whdM = whM * nP * M;
+    const long vox = whM * zM + (long)yM * a.wM + xM;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+    if (a.count0 > 0 && inimg) {
+        s0 = a.out[vox];
+        s1 = a.out[vox + whdM];
+        s2 = a.out[vox + 2 * whdM];
+        s3 = a.out[vox + 3 * whdM];
+        s4 = a.prev[vox];
+    }
+    // axial cubic planes (live3DSRRF.cl:418-452)
+    const float zc = (float)((zM + 0.5) / M);
+    int z0;
+    const float zcof = zc - 0.5f;
+    if (zcof < 0.0f) z0 = 0;
+    else if (zcof >= (float)(nP - 1)) z0 = nP - 2;
+    else z0 = (int)floorf(zcof);
+    const int zm1 = max(z0 - 1, 0), zp2 = min(z0 + 1, nP - 1);
+    const float z01 = zc - ((float)z0 + 0.5f);
+    const float nFf = (float)a.nFor, nF1 = (float)(a.nFor - 1);
+
+    const long fstride = (long)nP * whI;
+    {
+        const float4 *Uf = a.U + (long)a.frame0 * fstride;
+#pragma unroll
+        for (int q = 0; q < K3_FE; q++)
+            if (lidx[q] >= 0)
+                lU[lidx[q]] = gofs[q] >= 0 ? Uf[gofs[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
+
+    const int base = 2 * ly * t.P + lx;
+    const float4 *Tv = lTv + ft;
+    const int *To = lTo + ft;
+    for (int fi = 0; fi < a.nframes; fi++) {
+        const bool more = fi + 1 < a.nframes;
+        // lateral intensities of this frame first, then the next frame's tile: the tile
+        // loads stay in flight through the tap loop
+        float v_1 = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+        if (inimg) {
+            const float *Lf = a.L + (long)fi * nP * whM + (long)yM * a.wM + xM;
+            v_1 = Lf[whM * zm1];
+            v0 = Lf[whM * z0];
+            v1 = Lf[whM * (z0 + 1)];
+            v2 = Lf[whM * zp2];
+        }
+        float4 st[K3_FE];
+        if (more) {
+            const float4 *Un = a.U + (long)(a.frame0 + fi + 1) * fstride;
+#pragma unroll
+            for (int q = 0; q < K3_FE; q++)
+                st[q] = gofs[q] >= 0 ? Un[gofs[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (active) {
+            float acc = 0.0f;
+#pragma unroll 1
+            for (int k = 0; k < nt; k += 4) {
+                const int4 o = *reinterpret_cast<const int4 *>(To + k);
+                const float4 q0 = Tv[k], q1 = Tv[k + 1], q2 = Tv[k + 2], q3 = Tv[k + 3];
+                const float4 u0 = lU[base + o.x], u1 = lU[base + o.y];
+                const float4 u2 = lU[base + o.z], u3 = lU[base + o.w];
+                acc = tap3_acc(u0, q0, acc);
+                acc = tap3_acc(u1, q1, acc);
+                acc = tap3_acc(u2, q2, acc);
+                acc = tap3_acc(u3, q3, acc);
+            }
+            if (inimg) {
+                float C = (wsum + acc) / wsg;
+                C = apply_sens3(C, a.sens, a.sens_int);
+                // axial cubic (live3DSRRF.cl:448-452), left-to-right as written
+                float aa = __fmul_rn(-0.5f, v_1);
+                aa = __fadd_rn(aa, __fmul_rn(1.5f, v0));
+                aa = __fsub_rn(aa, __fmul_rn(1.5f, v1));
+                aa = __fadd_rn(aa, __fmul_rn(0.5f, v2));
+                float bb = __fsub_rn(v_1, __fmul_rn(2.5f, v0));
+                bb = __fadd_rn(bb, __fmul_rn(2.0f, v1));
+                bb = __fsub_rn(bb, __fmul_rn(0.5f, v2));
+                const float cc = __fadd_rn(__fmul_rn(-0.5f, v_1), __fmul_rn(0.5f, v1));
+                float v = __fmul_rn(__fmul_rn(__fmul_rn(aa, z01), z01), z01);
+                v = __fadd_rn(v, __fmul_rn(__fmul_rn(bb, z01), z01));
+                v = __fadd_rn(v, __fmul_rn(cc, z01));
+                v = __fadd_rn(v, v0);
+                const float tt = a.int_weighting ? v * C : C;
+                const float e2 = a.int_weighting ? ((tt * v) * C) : C * C;
+                if (a.count0 + fi == 0) {
+                    s0 = tt / nFf;
+                    s1 = e2 / nFf;
+                    s2 = 0.0f;
+                    s3 = v / nFf;
+                } else {
+                    s0 = s0 + tt / nFf;
+                    s1 = s1 + e2 / nFf;
+                    s2 = s2 + (tt * s4) / nF1;
+                    s3 = s3 + v / nFf;
+                }
+                s4 = tt;   // single 1/(n-1) in TAC2 (live3DSRRF.cl:472)
+            }
+        }
+        if (more) {
+            __syncthreads();   // every wave is done with this frame's tile
+#pragma unroll
+            for (int q = 0; q < K3_FE; q++)
+                if (lidx[q] >= 0) lU[lidx[q]] = st[q];
+            __syncthreads();
+        }
+    }
+    if (inimg) {
+        a.out[vox] = s0;
+        a.out[vox + whdM] = s1;
+        a.out[vox + 2 * whdM] = s2;
+        a.out[vox + 3 * whdM] = s3;
+        a.prev[vox] = s4;
+    }
+}
+
+size_t rgc3_lds_bytes(const Rgc3Tables &t)
+{
+    return sizeof(float4) * ((size_t)t.NZ * t.PZ + t.gtaps) + sizeof(int) * (size_t)t.gtaps;
+}
+
 // Gated samples of one frame without drift, summed over all voxels.
 __global__ void __launch_bounds__(256) k3d_count(int w, int h, int nP, int M, float radius,
                                                  float tSO, unsigned long long *total)
@@ -549,7 +787,8 @@ static int getenv_int(const char *name, int dflt)
     return v ? atoi(v) : dflt;
 }
 
-hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s)
+hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
+                        size_t lds_limit, hipStream_t s, const char **kname)
 {
     if (a.nframes <= 0) return hipSuccess;
     const long nl = (long)a.nframes * a.nP * a.wM * a.hM;
@@ -558,12 +797,24 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s)
     const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.M);
     // (the fast kernel addresses one frame of U with 32-bit byte offsets)
     const bool off32 = (double)a.nP * 4.0 * a.w * a.h * sizeof(float4) < 4294967296.0;
-    if (drift_zero && off32 && getenv_int("ESRRF_3D_FAST", 1))
+    const int kern = getenv_int("ESRRF_3D_KERNEL", 3);
+    // the tiled kernel addresses one frame of U with 32-bit element offsets
+    const bool off31 = (double)a.nP * 4.0 * a.w * a.h < 2147483648.0;
+    if (drift_zero && t && kern == 3 && off31 && rgc3_lds_bytes(*t) <= lds_limit) {
+        const int tiles = ((a.w + 7) >> 3) * ((a.h + 7) >> 3);
+        const dim3 g3(tiles, (a.M * a.M + 3) / 4, a.nP * a.M);
+        hipLaunchKernelGGL(k3d_rgc3, g3, dim3(256), rgc3_lds_bytes(*t), s, a, *t);
+        if (kname) *kname = "k3d_rgc3";
+    } else if (drift_zero && off32 && kern >= 2) {
         hipLaunchKernelGGL(k3d_rgc2<8>, grid, dim3(64, 4), 0, s, a);
-    else if (drift_zero)
+        if (kname) *kname = "k3d_rgc2";
+    } else if (drift_zero) {
         hipLaunchKernelGGL(k3d_rgc<8>, grid, dim3(64, 4), 0, s, a);
-    else
+        if (kname) *kname = "k3d_rgc";
+    } else {
         hipLaunchKernelGGL(k3d_rgc<1>, grid, dim3(64, 4), 0, s, a);
+        if (kname) *kname = "k3d_rgc";
+    }
     return hipGetLastError();
 }
 

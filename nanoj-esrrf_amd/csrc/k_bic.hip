@@ -923,11 +923,13 @@ static hipError_t launch_ppw(const BicRgcArgs &a, hipStream_t s)
     return a.taps_in_lds ? launch_ppw_t<PPW, true>(a, s) : launch_ppw_t<PPW, false>(a, s);
 }
 
-hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s)
+hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s, const char **kname)
 {
     if (a.nframes <= 0) return hipSuccess;
-    if (a.frames_per_pass == 2 && a.mode != 2 && a.taps_in_lds && ppw <= 4 &&
-        a.tac_prev_scaled && bic_pair_lds_bytes(a) <= 64 * 1024) {
+    const bool pair = a.frames_per_pass == 2 && a.mode != 2 && a.taps_in_lds && ppw <= 4 &&
+                      a.tac_prev_scaled && bic_pair_lds_bytes(a) <= 64 * 1024;
+    if (kname) *kname = pair ? "k_bic_rgc2" : "k_bic_rgc";
+    if (pair) {
         switch (ppw) {
         case 1: return launch_pair<1>(a, s);
         case 2: return launch_pair<2>(a, s);

@@ -577,10 +577,12 @@ __global__ void __launch_bounds__(512) k_fht_rgc2(const FhtRgcArgs a,
     }
 }
 
-hipError_t launch_fht_rgc(const FhtRgcArgs &a, hipStream_t s)
+hipError_t launch_fht_rgc(const FhtRgcArgs &a, hipStream_t s, const char **kname)
 {
     if (a.nframes <= 0) return hipSuccess;
-    if (a.pair && fht_rgc2_lds_bytes(a) <= 65536) {
+    const bool pair = a.pair && fht_rgc2_lds_bytes(a) <= 65536;
+    if (kname) *kname = pair ? "k_fht_rgc2" : "k_fht_rgc";
+    if (pair) {
         const int tx = (a.W + FT2 - 1) / FT2, ty = (a.H + FT2 - 1) / FT2;
         FhtRgcArgs b = a;
         b.tiles_x = tx;

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <vector>
 
+#include "geometry.h"
 #include "mode3d_internal.h"
 
 namespace esrrf {
@@ -46,6 +47,57 @@ Prep3dArgs prep_args(const ModeCtx &c, const ModeState &ms)
     p.g = ms.d_g3;
     p.U = ms.d_u3;
     return p;
+}
+
+// Tap tables of the tiled RGC kernel (frames without drift).  Where the window, the tile
+// or the table size exceed the kernel's limits the per-voxel kernels serve every frame.
+int build_tiled3(const ModeCreate &c, ModeState &ms)
+{
+    ms.tiled3 = false;
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return err3(ms, ESRRF_ERR_HIP, "3-D: device LDS limit query failed");
+    ms.lds_limit = (size_t)lds;
+    Geometry3 g;
+    if (!build_3d_geometry(c.w, c.h, c.nP, c.M, c.radius, c.tSS, c.tSO, g)) return ESRRF_OK;
+    const size_t ntap = g.tapv.size();
+    if (ntap > ((size_t)1 << 28) || g.NZ * g.TH * g.TW > 256 * 16) return ESRRF_OK;
+    if (!alloc(&ms.d_tapv3, sizeof(float4) * ntap) || !alloc(&ms.d_tapo3, sizeof(int) * ntap) ||
+        !alloc(&ms.d_tapk3, sizeof(int) * ntap) ||
+        !alloc(&ms.d_ph3, sizeof(Phase3) * g.phases.size()))
+        return err3(ms, ESRRF_ERR_NOMEM, "3-D tap tables: hipMalloc failed");
+    hipError_t e = hipMemcpy(ms.d_tapv3, g.tapv.data(), sizeof(float4) * ntap,
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(ms.d_tapo3, g.tapo.data(), sizeof(int) * ntap, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(ms.d_tapk3, g.tapk.data(), sizeof(int) * ntap, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(ms.d_ph3, g.phases.data(), sizeof(Phase3) * g.phases.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D tap tables upload", e);
+    Rgc3Tables &t = ms.t3;
+    t.tapv = ms.d_tapv3;
+    t.tapo = ms.d_tapo3;
+    t.tapk = ms.d_tapk3;
+    t.phases = ms.d_ph3;
+    t.R2lo = g.R2lo;
+    t.R2hi = g.R2hi;
+    t.G0 = g.G0;
+    t.TW = g.TW;
+    t.TH = g.TH;
+    t.PW = g.PW;
+    t.P = g.P;
+    t.PZ = g.PZ;
+    t.NZ = g.NZ;
+    t.xs_log2 = g.xs_log2;
+    t.ncls_x = g.ncls_x;
+    t.ncls_y = g.ncls_y;
+    t.nZM = g.nZM;
+    t.gtaps = g.max_group_taps + TAP_SPARE;
+    ms.tiled3 = true;
+    return ESRRF_OK;
 }
 
 }  // namespace
@@ -110,7 +162,7 @@ int mode3d_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U
     ms.active = (long long)cnt;
     ms.wM = w * M;
     ms.hM = h * M;
-    return ESRRF_OK;
+    return build_tiled3(c, ms);
 }
 
 int mode3d_prepare(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n)
@@ -147,7 +199,8 @@ int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
     a.radius = r.radius;
     a.tSS = r.tSS;
     a.tSO = r.tSO;
-    hipError_t e = launch3d_rgc(a, r.drift_zero, c.s);
+    hipError_t e = launch3d_rgc(a, r.drift_zero, ms.tiled3 ? &ms.t3 : nullptr, ms.lds_limit,
+                                c.s, &ms.rgc_kernel);
     if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D RGC launch", e);
     return ESRRF_OK;
 }

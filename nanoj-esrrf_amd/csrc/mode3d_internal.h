@@ -35,7 +35,11 @@ hipError_t launch3d_prepare(const void *src, int dtype, const Prep3dArgs &p, int
                             hipStream_t s);
 hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long gn,
                          hipStream_t s);
-hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, hipStream_t s);
+// The tiled kernel (k3d_rgc3) runs for frames without drift when `t` is given and its LDS
+// fits `lds_limit`; *kname receives the name of the kernel launched.
+hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
+                        size_t lds_limit, hipStream_t s, const char **kname);
+size_t rgc3_lds_bytes(const Rgc3Tables &t);
 hipError_t launch3d_count(int w, int h, int nP, int M, float radius, float tSO,
                           unsigned long long *d_total, hipStream_t s);
 

@@ -258,7 +258,7 @@ int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
         a.wsum = ms.tap_wsum;
         a.tiles_x = (c.wM + 15) / 16;
         a.tiles_y = (c.hM + 15) / 16;
-        st = launch_fht_rgc(a, c.s);
+        st = launch_fht_rgc(a, c.s, &ms.rgc_kernel);
     } else {
         FhtGenericArgs g{};
         g.U = c.d_U;
@@ -281,6 +281,7 @@ int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
         g.tSS = r.tSS;
         g.tSO = r.tSO;
         st = launch_fht_generic(g, c.s);
+        ms.rgc_kernel = "k_fht_generic";
     }
     if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT RGC launch", st);
     return ESRRF_OK;
@@ -308,7 +309,8 @@ void free_mode_state(ModeState &ms)
 {
     void *ptrs[] = {ms.d_rawf, ms.plan.F,   ms.plan.Y, (void *)ms.plan.tw, ms.d_taps,
                     ms.d_tapk, ms.d_st,     ms.d_cell, ms.d_icf,           ms.d_planes,
-                    ms.d_g3,   ms.d_u3,     ms.d_lat};
+                    ms.d_g3,   ms.d_u3,     ms.d_lat,  ms.d_tapv3,        ms.d_tapo3,
+                    ms.d_tapk3, ms.d_ph3};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ms.d_rawf = nullptr;
@@ -321,6 +323,10 @@ void free_mode_state(ModeState &ms)
     ms.d_icf = nullptr;
     ms.d_planes = ms.d_g3 = ms.d_lat = nullptr;
     ms.d_u3 = nullptr;
+    ms.d_tapv3 = nullptr;
+    ms.d_tapo3 = ms.d_tapk3 = nullptr;
+    ms.d_ph3 = nullptr;
+    ms.tiled3 = false;
 }
 
 }  // namespace esrrf

@@ -39,6 +39,15 @@ struct ModeState {
     float4 *d_u3 = nullptr;       // unit tap gradients [nF*nP][2h][2w]
     float *d_lat = nullptr;       // lateral intensities [nF][nP][hM][wM]
     int n_loaded = 0;             // frames of the current load (gradient plane stride)
+    // tiled 3-D RGC tables (frames without drift)
+    float4 *d_tapv3 = nullptr;
+    int *d_tapo3 = nullptr;
+    int *d_tapk3 = nullptr;
+    Phase3 *d_ph3 = nullptr;
+    Rgc3Tables t3{};
+    bool tiled3 = false;
+    size_t lds_limit = 0;         // dynamic LDS one block may use on the device
+    const char *rgc_kernel = "";  // RGC kernel of the last launch
 };
 
 struct ModeCtx {

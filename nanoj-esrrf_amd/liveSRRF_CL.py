@@ -70,6 +70,12 @@ class Engine:
         self.check(self.L.esrrf_read_frame_counters(self.h, out))
         return [out[0], out[1]]
 
+    def rgc_kernel(self) -> str:
+        """Name of the RGC kernel the last calculate launched."""
+        buf = ctypes.create_string_buffer(64)
+        self.check(self.L.esrrf_rgc_kernel(self.h, buf, 64))
+        return buf.value.decode()
+
 
 class LiveSRRF_CL:
     """Mirror of LiveSRRF_CL (LiveSRRF_CL.java:29-1017)."""

@@ -98,21 +98,24 @@ def test_3d_zero_calibration_parity(pkg, O, w, h, nP, M, fwhm, S):
 
 @pytest.mark.parametrize("w,h,nP,M,fwhm,loads", [
     (20, 16, 9, 4, 1.5, [(0, 11)]), (16, 12, 5, 2, 2.0, [(0, 5), (5, 13)]),
-    (12, 14, 3, 3, 1.5, [(0, 1), (1, 9)]),
+    (12, 14, 3, 3, 1.5, [(0, 1), (1, 9)]), (13, 9, 4, 4, 1.0, [(0, 3), (3, 4), (4, 7)]),
+    (40, 36, 4, 3, 2.0, [(0, 6)]), (24, 24, 5, 5, 2.5, [(0, 2), (2, 5)]),
 ])
-def test_3d_fast_kernel_matches_window_kernel(pkg, monkeypatch, w, h, nP, M, fwhm, loads):
-    """The drift-free 3-D kernel (geometry once per chunk of frames, gate folded into med3,
-    x range limited to the tSO sphere) against the window-order kernel on the same data:
-    chunks of 8 frames with ragged tails and continuation across loads.  The two sum the
-    samples in different orders, so they agree to float rounding (1e-5 of each channel's
-    maximum; TAC2 and VAR are products of per-frame values), not bit for bit."""
+def test_3d_fast_kernels_match_window_kernel(pkg, monkeypatch, w, h, nP, M, fwhm, loads):
+    """The drift-free 3-D kernels against the window-order kernel (k3d_rgc) on the same data:
+    k3d_rgc2 (per-voxel geometry once per chunk of 8 frames) and the tiled k3d_rgc3
+    (wave-uniform tap tables per x/y class, z-slice and phase; LDS-staged gradient tiles;
+    partial tiles, border classes, M = 2..5, one-frame and ragged loads continued across
+    launches).  They sum the samples in different orders, so they agree to float rounding,
+    not bit for bit: within the parity tolerance |a - b| <= 1e-5 + 1e-4 |b| on each
+    channel divided by its maximum (TAC2 and VAR are products of per-frame values)."""
     n = loads[-1][1]
     planes = planes_stack(n, w, h, nP, seed=70 + w + nP)
     mont = montage(planes, np.arange(nP))
     cd = calib_dict(*zero_calib(nP))
     res = {}
-    for fast in ("0", "1"):
-        monkeypatch.setenv("ESRRF_3D_FAST", fast)
+    for kern, name in (("1", "k3d_rgc"), ("2", "k3d_rgc2"), ("3", "k3d_rgc3")):
+        monkeypatch.setenv("ESRRF_3D_KERNEL", kern)
         e = pkg.LiveSRRF_CL()
         e.checkDevices()
         e.initialise(3 * w, 3 * h, M, fwhm, 2, max(b - a for a, b in loads), n, 20000,
@@ -121,12 +124,16 @@ def test_3d_fast_kernel_matches_window_kernel(pkg, monkeypatch, w, h, nP, M, fwh
         e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
         for a, b in loads:
             e.calculateSRRF(mont[a:b])
-        res[fast] = np.asarray(e.readSRRFbuffer()).reshape(4, -1)
+        assert e._e().rgc_kernel() == name
+        res[kern] = np.asarray(e.readSRRFbuffer()).reshape(4, -1)
         e.release()
-    for c in range(4):
-        scale = np.abs(res["0"][c]).max()
-        err = np.abs(res["1"][c] - res["0"][c]).max()
-        assert err <= 1e-5 * scale, f"channel {c}: {err:.3g} vs scale {scale:.3g}"
+    for kern in ("2", "3"):
+        for c in range(4):
+            scale = max(float(np.abs(res["1"][c]).max()), 1e-30)
+            b = res["1"][c] / scale
+            excess = np.abs(res[kern][c] / scale - b) - (1e-5 + 1e-4 * np.abs(b))
+            assert excess.max() <= 0, \
+                f"kernel {kern} channel {c}: excess {excess.max():.3g} (scale {scale:.3g})"
 
 
 def test_3d_calibrated_montage_parity(pkg, O):
