@@ -226,6 +226,62 @@ int esrrf_rse_descend(esrrf_rse *r, const float *x0, int steps, const float *lr,
 int esrrf_rse_counters(esrrf_rse *r, long long *evaluations, long long *rgc_passes);
 const char *esrrf_rse_last_error(esrrf_rse *r);
 
+/* ---- SQUIRREL error map of any (reference, SR) image pair (ErrorMapLiveSRRF.java:50-193) --
+ * optimise(impRef, impSR) then calculateErrorMap() on device images of (width*M) x (height*M)
+ * pixels: crop crop_size*M SR pixels per side when crop_border (:57-64, :88-99), find the RSF
+ * sigma by Brent on [0, 4/2.35482*M] (:119-127; or take fixed_sigma*M, :53,:116-117), fit
+ * alpha, beta by least squares of ref on (blur(SR), blur(1)) (:145-152, :201-233), and form
+ * RSC = blur(alpha SR + beta), RSE = sqrt(mean((RSC - ref)^2)) (globalRMSE), RSP = Pearson
+ * correlation of RSC and ref (globalPPMCC) and the error map |ref - RSC| (:160-190).  NaN
+ * pixels of the inputs read as 0 (readSRRFbuffer hands over NaN -> 0 images).  The blur is a
+ * separable Gaussian truncated at ceil(5 sigma_max) with clamp-to-edge borders (stand-in for
+ * ImageJ's blurGaussian), and sums run in double. */
+typedef struct esrrf_squirrel esrrf_squirrel;
+typedef struct esrrf_squirrel_params {
+    int width, height;        /* raw frame size (imp of ErrorMapLiveSRRF's constructor)      */
+    int magnification;        /* 1..16                                                      */
+    int crop_border;          /* 1: crop crop_size raw pixels per side (cropBorder)          */
+    int crop_size;            /* cropSize (3 in the sweep, gui/ParametersSweep_.java:114)    */
+    int fix_sigma;            /* 1: RSF sigma = fixed_sigma * M (fixSigma, fixedSigma)       */
+    float fixed_sigma;        /* raw pixels                                                 */
+    int device;               /* -1: fastest                                                */
+} esrrf_squirrel_params;
+typedef struct esrrf_squirrel_result {
+    double sigma;             /* RSF sigma in SR pixels (sigma_linear)                      */
+    double alpha, beta;
+    double rse;               /* globalRMSE                                                 */
+    double rsp;               /* globalPPMCC                                                */
+    int overblur;             /* 1: sigma within 1e-4 of its bound (OVERBLUR_MESSAGE, :135) */
+    int evaluations;          /* objective evaluations of the sigma search                  */
+} esrrf_squirrel_result;
+int esrrf_squirrel_create(const esrrf_squirrel_params *p, esrrf_squirrel **out);
+int esrrf_squirrel_destroy(esrrf_squirrel *q);
+int esrrf_squirrel_set_stream(esrrf_squirrel *q, void *hip_stream);
+/* Cropped size of the maps (Wc x Hc). */
+int esrrf_squirrel_dims(esrrf_squirrel *q, int *width_c, int *height_c);
+/* ref, sr: [height*M][width*M] device images; error_map, rsc: [Hc][Wc] device outputs or
+ * NULL.  Blocking (the sigma search reads back a few doubles per evaluation). */
+int esrrf_squirrel_run_device(esrrf_squirrel *q, const float *d_ref, const float *d_sr,
+                              esrrf_squirrel_result *res, float *d_error_map, float *d_rsc);
+/* Same from and to host memory. */
+int esrrf_squirrel_run(esrrf_squirrel *q, const float *ref, const float *sr,
+                       esrrf_squirrel_result *res, float *error_map, float *rsc);
+const char *esrrf_squirrel_last_error(esrrf_squirrel *q);
+/* The sweep's error mapping (gui/ParametersSweep_.java:465-517): for every configuration of
+ * the last esrrf_sweep_run, SR = channel (0 AVG, 1 VAR, 2 TAC2) and ref = INT of that
+ * configuration, cropped by crop_size raw pixels.  rse, rsp, sigma (each may be NULL) receive
+ * [n_magnifications][n_n_frames][n_sensitivity][n_fwhm] values (the reference's RSE and RSP
+ * sweep maps are the [n_sensitivity][n_fwhm] slices, :518-530). */
+int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sigma,
+                         float fixed_sigma, double *rse, double *rsp, double *sigma);
+/* QnR map (gui/GetQnRmap_.java:179-234): each FRC resolution value r becomes
+ * L(r) = 1/(exp(-k (r - x0)) + 1), the logistic through (res_min, 0.925) and (res_max, 0.075)
+ * (:179-203); then QnR = 2 L RSP / (L + RSP) (:227-234).  With auto_min_max the bounds are the
+ * smallest and largest finite FRC values (:86-90).  n values; host arrays.  The best value and
+ * its index (first maximum) go to *best and *best_index (each may be NULL). */
+int esrrf_qnr_map(const float *frc, const float *rsp, long n, int auto_min_max, float res_min,
+                  float res_max, float *qnr, float *best, long *best_index);
+
 const char *esrrf_last_error(esrrf_engine *e);
 const char *esrrf_version(void);
 

@@ -35,6 +35,9 @@ EXPORTS = [
     "esrrf_sweep_shape", "esrrf_sweep_last_error",
     "esrrf_rse_create", "esrrf_rse_destroy", "esrrf_rse_load", "esrrf_rse_value_and_grad",
     "esrrf_rse_best_sigma", "esrrf_rse_descend", "esrrf_rse_counters", "esrrf_rse_last_error",
+    "esrrf_squirrel_create", "esrrf_squirrel_destroy", "esrrf_squirrel_set_stream",
+    "esrrf_squirrel_dims", "esrrf_squirrel_run_device", "esrrf_squirrel_run",
+    "esrrf_squirrel_last_error", "esrrf_sweep_squirrel", "esrrf_qnr_map",
 ]
 
 
@@ -80,6 +83,22 @@ class RseParams(ctypes.Structure):
         ("n_frames", ctypes.c_int), ("crop_size", ctypes.c_int), ("int_weighting", ctypes.c_int),
         ("do_mp_map_correction", ctypes.c_int), ("sigma_max", ctypes.c_float),
         ("device", ctypes.c_int),
+    ]
+
+
+class SquirrelParams(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int), ("magnification", ctypes.c_int),
+        ("crop_border", ctypes.c_int), ("crop_size", ctypes.c_int), ("fix_sigma", ctypes.c_int),
+        ("fixed_sigma", ctypes.c_float), ("device", ctypes.c_int),
+    ]
+
+
+class SquirrelResult(ctypes.Structure):
+    _fields_ = [
+        ("sigma", ctypes.c_double), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+        ("rse", ctypes.c_double), ("rsp", ctypes.c_double), ("overblur", ctypes.c_int),
+        ("evaluations", ctypes.c_int),
     ]
 
 
@@ -147,9 +166,19 @@ def lib():
         "esrrf_rse_counters": [vp, ctypes.POINTER(ctypes.c_longlong),
                                ctypes.POINTER(ctypes.c_longlong)],
         "esrrf_rse_last_error": [vp],
+        "esrrf_squirrel_create": [ctypes.POINTER(SquirrelParams), ctypes.POINTER(vp)],
+        "esrrf_squirrel_destroy": [vp],
+        "esrrf_squirrel_set_stream": [vp, vp],
+        "esrrf_squirrel_dims": [vp, ip, ip],
+        "esrrf_squirrel_run_device": [vp, vp, vp, ctypes.POINTER(SquirrelResult), vp, vp],
+        "esrrf_squirrel_run": [vp, fp, fp, ctypes.POINTER(SquirrelResult), fp, fp],
+        "esrrf_squirrel_last_error": [vp],
+        "esrrf_sweep_squirrel": [vp, c_int, c_int, c_int, c_float, dp, dp, dp],
+        "esrrf_qnr_map": [fp, fp, ctypes.c_long, c_int, c_float, c_float, fp, fp,
+                          ctypes.POINTER(ctypes.c_long)],
     }
     strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error",
-               "esrrf_rse_last_error")
+               "esrrf_rse_last_error", "esrrf_squirrel_last_error")
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args

@@ -147,79 +147,6 @@ int eval_impl(esrrf_rse *r, float fwhm, float sens, float sigma, double *rse, do
     return ESRRF_OK;
 }
 
-// Brent's derivative-free minimisation (R. P. Brent, Algorithms for Minimization without
-// Derivatives, 1973, ch. 5), as commons-math3's BrentOptimizer applies it for the RSF
-// sigma (ErrorMapLiveSRRF.java:119-127: relative 1e-10, absolute 1e-14, 1000 evaluations).
-template <class F>
-int brent(F f, double lo, double hi, double *xbest, double *fbest, int *nev)
-{
-    const double c = 0.5 * (3.0 - std::sqrt(5.0));
-    const double rel = 1e-10, abs_ = 1e-14;
-    double a = lo, b = hi;
-    double x = a + c * (b - a), w = x, v = x;
-    double fx, fw, fv;
-    int rc = f(x, &fx);
-    if (rc) return rc;
-    fw = fv = fx;
-    int n = 1;
-    double d = 0.0, e = 0.0;
-    while (n < 1000) {
-        const double m = 0.5 * (a + b);
-        const double tol = rel * std::fabs(x) + abs_, t2 = 2.0 * tol;
-        if (std::fabs(x - m) <= t2 - 0.5 * (b - a)) break;
-        double p = 0.0, q = 0.0, r = 0.0;
-        if (std::fabs(e) > tol) {
-            r = (x - w) * (fx - fv);
-            q = (x - v) * (fx - fw);
-            p = (x - v) * q - (x - w) * r;
-            q = 2.0 * (q - r);
-            if (q > 0) p = -p;
-            else q = -q;
-            r = e;
-            e = d;
-        }
-        if (std::fabs(p) < std::fabs(0.5 * q * r) && p > q * (a - x) && p < q * (b - x)) {
-            d = p / q;
-            const double u = x + d;
-            if (u - a < t2 || b - u < t2) d = x <= m ? tol : -tol;
-        } else {
-            e = x < m ? b - x : a - x;
-            d = c * e;
-        }
-        const double u = x + (std::fabs(d) >= tol ? d : (d > 0 ? tol : -tol));
-        double fu;
-        rc = f(u, &fu);
-        if (rc) return rc;
-        n++;
-        if (fu <= fx) {
-            if (u < x) b = x;
-            else a = x;
-            v = w;
-            fv = fw;
-            w = x;
-            fw = fx;
-            x = u;
-            fx = fu;
-        } else {
-            if (u < x) a = u;
-            else b = u;
-            if (fu <= fw || w == x) {
-                v = w;
-                fv = fw;
-                w = u;
-                fw = fu;
-            } else if (fu <= fv || v == x || v == w) {
-                v = u;
-                fv = fu;
-            }
-        }
-    }
-    *xbest = x;
-    *fbest = fx;
-    if (nev) *nev = n;
-    return ESRRF_OK;
-}
-
 }  // namespace
 
 int esrrf_rse_create(const esrrf_rse_params *p, esrrf_rse **out)

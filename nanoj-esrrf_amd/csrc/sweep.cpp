@@ -416,3 +416,51 @@ int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int i_fwhm, int *nw, int *ppw, 
 }
 
 const char *esrrf_sweep_last_error(esrrf_sweep *s) { return s ? s->err.c_str() : "null sweep"; }
+
+int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sigma,
+                         float fixed_sigma, double *rse, double *rsp, double *sigma)
+{
+    if (!s || channel < 0 || channel > 2) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->have_result) return sfail(s, ESRRF_ERR_STATE, "no sweep has run");
+    if (crop_size < 0 || 2 * crop_size >= std::min(s->w, s->h))
+        return sfail(s, ESRRF_ERR_INVALID, "crop_size %d outside [0, %d)", crop_size,
+                     (std::min(s->w, s->h) + 1) / 2);
+    SCHK(s, hipSetDevice(s->dev));
+    const long ncfg = (long)s->nfs.size() * s->sens.size() * s->fwhm.size();
+    for (size_t im = 0; im < s->Ms.size(); im++) {
+        const int M = s->Ms[im];
+        esrrf_squirrel_params qp{};
+        qp.width = s->w;
+        qp.height = s->h;
+        qp.magnification = M;
+        qp.crop_border = 1;
+        qp.crop_size = crop_size;
+        qp.fix_sigma = fix_sigma;
+        qp.fixed_sigma = fixed_sigma;
+        qp.device = s->dev;
+        esrrf_squirrel *q = nullptr;
+        int rc = esrrf_squirrel_create(&qp, &q);
+        if (rc) return sfail(s, rc, "SQUIRREL plan for M=%d: error %d", M, rc);
+        esrrf_squirrel_set_stream(q, s->stream);   // ordered after the sweep's launches
+        const long whM = (long)s->w * s->h * M * M;
+        for (long c = 0; c < ncfg && rc == ESRRF_OK; c++) {
+            // configuration c of [nf][S][fwhm]: SR = its channel, ref = its INT (:473)
+            const float *base = s->d_out[im] + c * 4 * whM;
+            esrrf_squirrel_result r{};
+            rc = esrrf_squirrel_run_device(q, base + 3 * whM, base + channel * whM, &r, nullptr,
+                                           nullptr);
+            if (rc) {
+                s->err = std::string("SQUIRREL: ") + esrrf_squirrel_last_error(q);
+                break;
+            }
+            const long o = (long)im * ncfg + c;
+            if (rse) rse[o] = r.rse;
+            if (rsp) rsp[o] = r.rsp;
+            if (sigma) sigma[o] = r.sigma;
+        }
+        esrrf_squirrel_destroy(q);
+        if (rc) return rc;
+    }
+    return ESRRF_OK;
+}

@@ -97,6 +97,19 @@ class ParametersSweep:
         return dict(nw=nw.value, ppw=ppw.value, taps_in_lds=tl.value,
                     frames_per_pass=fpp.value, ms=ms.value)
 
+    def error_maps(self, channel="AVG", crop_size=3, fix_sigma=False, fixed_sigma=0.0):
+        """The sweep's error mapping (gui/ParametersSweep_.java:465-530) for one channel
+        ("AVG", "VAR" or "TAC2") against each configuration's INT image: RSE, RSP and RSF
+        sigma arrays [n_magnifications][n_n_frames][n_sensitivity][n_fwhm]."""
+        ch = {"AVG": 0, "VAR": 1, "TAC2": 2}[channel]
+        shape = (self.mags.size, self.nfs.size, self.sens.size, self.fwhm.size)
+        rse, rsp, sig = (np.empty(shape, np.float64) for _ in range(3))
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(self.L.esrrf_sweep_squirrel(
+            self.h, ch, int(crop_size), 1 if fix_sigma else 0, float(fixed_sigma),
+            rse.ctypes.data_as(dp), rsp.ctypes.data_as(dp), sig.ctypes.data_as(dp)))
+        return dict(rse=rse, rsp=rsp, sigma=sig)
+
     def close(self):
         if self.h:
             self.L.esrrf_sweep_destroy(self.h)

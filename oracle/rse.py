@@ -146,3 +146,70 @@ def brent_min(f, lo: float, hi: float, rel=1e-10, abs_=1e-14, max_eval=1000):
 def best_sigma(ref, sr, sigma_max: float, K: int):
     """ErrorMapLiveSRRF.optimise with fixSigma false: Brent on [0, sigma_max]."""
     return brent_min(lambda s: rse(ref, sr, s, K), 0.0, sigma_max)
+
+
+def finite_or_zero(img) -> np.ndarray:
+    """readSRRFbuffer hands over images with NaN -> 0 (LiveSRRF_CL.java:776-781)."""
+    a = np.asarray(img, np.float64)
+    return np.where(np.isnan(a), 0.0, a)
+
+
+def pearson(x: np.ndarray, y: np.ndarray) -> float:
+    """Pearson correlation (globalPPMCC, ErrorMapLiveSRRF.java:176; NanoJ-Core's
+    ArrayMath.calculatePPMCC(.., true) is not available here: its flag is taken to mean the
+    mean-centred coefficient, parity unpinned)."""
+    x = np.asarray(x, np.float64).ravel()
+    y = np.asarray(y, np.float64).ravel()
+    xc, yc = x - x.mean(), y - y.mean()
+    den = math.sqrt(float((xc * xc).sum() * (yc * yc).sum()))
+    return float((xc * yc).sum() / den) if den > 0 else 0.0
+
+
+def error_map(ref_full, sr_full, M: int, crop_size: int, crop_border: bool = True,
+              fix_sigma: bool = False, fixed_sigma: float = 0.0, K: int | None = None):
+    """ErrorMapLiveSRRF.optimise + calculateErrorMap (:81-190) on full-size SR-space images.
+    Returns dict(sigma, alpha, beta, rse, rsp, emap, rsc, overblur)."""
+    ref = crop(finite_or_zero(ref_full), M, crop_size if crop_border else 0)
+    sr = crop(finite_or_zero(sr_full), M, crop_size if crop_border else 0)
+    sigma_max = float(np.float32(4.0 / 2.35482) * np.float32(M))           # :75
+    fixed = float(np.float32(fixed_sigma) * np.float32(M))                 # :53
+    if K is None:
+        K = default_K(max(sigma_max, fixed if fix_sigma else 0.0))
+    if fix_sigma:
+        sigma = fixed
+    else:
+        sigma, _, _ = best_sigma(ref, sr, sigma_max, K)                    # :119-127
+    sigma = float(np.float32(sigma))                                       # sigma_linear
+    x = blur(sr, sigma, K)
+    a, b = alpha_beta(x, ref)                                              # :145-152
+    a32, b32 = np.float32(a), np.float32(b)
+    rsc = (a32 * x.astype(np.float32) + b32).astype(np.float64)            # :160-168
+    d = rsc - ref
+    return dict(sigma=sigma, alpha=float(a32), beta=float(b32),
+                rse=math.sqrt(float((d * d).mean())),                      # :175
+                rsp=pearson(rsc, ref),                                     # :176
+                emap=np.abs(ref - rsc),                                    # :180-188
+                rsc=rsc, overblur=abs(sigma - sigma_max) < 1e-4)           # :135
+
+
+def qnr_map(frc, rsp, auto_min_max=True, res_min=0.05, res_max=0.2):
+    """gui/GetQnRmap_.java: logistic conversion of the FRC resolution map through
+    (res_min, 0.925) and (res_max, 0.075) (:179-203), then QnR = 2 L RSP / (L + RSP)
+    (:227-234), first maximum (:244-258).  Returns (qnr, best, index)."""
+    frc = np.asarray(frc, np.float32).ravel()
+    rsp = np.asarray(rsp, np.float32).ravel()
+    if auto_min_max:                                                        # :86-90, :290-305
+        f = frc[np.isfinite(frc)]
+        res_min, res_max = float(f.min()), float(f.max())
+    M1, M2 = 0.075, 0.925
+    A1, A2 = math.log((1 - M1) / M1), math.log((1 - M2) / M2)
+    x0 = (A2 * float(res_max) - A1 * float(res_min)) / (A2 - A1)
+    k = 1 / (x0 - float(res_max)) * A1
+    L = (1.0 / (np.exp(-k * (frc.astype(np.float64) - x0)) + 1.0)).astype(np.float32)
+    q = (np.float32(2) * L * rsp) / (L + rsp)
+    q = q.astype(np.float32)
+    best, idx = -np.float32(3.4028235e38), 0
+    for i, v in enumerate(q):
+        if v > best:
+            best, idx = v, i
+    return q, float(best), idx

@@ -88,3 +88,55 @@ def test_brent_finds_the_minimum(f, lo, hi, xmin):
     x, fx, n = R.brent_min(f, lo, hi)
     assert abs(x - xmin) < 1e-6 and n < 200
     assert fx <= f(xmin) + 1e-9
+
+
+def test_pearson_closed_forms():
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(6, 7))
+    assert math.isclose(R.pearson(x, 3.0 * x + 2.0), 1.0, rel_tol=1e-12)
+    assert math.isclose(R.pearson(x, -0.5 * x + 1.0), -1.0, rel_tol=1e-12)
+    y = rng.normal(size=(6, 7))
+    assert math.isclose(R.pearson(x, y), float(np.corrcoef(x.ravel(), y.ravel())[0, 1]),
+                        rel_tol=1e-12)
+
+
+def test_error_map_of_an_exact_blur_model():
+    """ref = alpha blur(SR) + beta exactly at sigma 1.5 (in SR pixels): the sigma search
+    lands on 1.5, alpha/beta are recovered, RSE and the error map vanish, RSP = 1."""
+    rng = np.random.default_rng(6)
+    M, crop = 2, 1
+    sr = rng.random((20, 22)) * 100.0
+    K = R.default_K(float(np.float32(4.0 / 2.35482) * np.float32(M)))
+    ref = 0.8 * R.blur(R.crop(sr, M, crop), 1.5, K) + 12.0
+    ref_full = np.zeros_like(sr)
+    c = crop * M
+    ref_full[c:-c, c:-c] = ref
+    out = R.error_map(ref_full, sr, M, crop)
+    assert abs(out["sigma"] - 1.5) < 1e-4
+    assert abs(out["alpha"] - 0.8) < 1e-4 and abs(out["beta"] - 12.0) < 1e-3
+    assert out["rse"] < 1e-3 and out["emap"].max() < 5e-3 and out["rsp"] > 1 - 1e-9
+
+
+def test_error_map_nan_reads_as_zero_and_fixed_sigma():
+    rng = np.random.default_rng(7)
+    sr, ref = rng.random((12, 12)), rng.random((12, 12))
+    sr_nan = sr.copy()
+    sr_nan[3, 4] = np.nan
+    sr0 = sr.copy()
+    sr0[3, 4] = 0.0
+    a = R.error_map(ref, sr_nan, 2, 1, fix_sigma=True, fixed_sigma=0.6)
+    b = R.error_map(ref, sr0, 2, 1, fix_sigma=True, fixed_sigma=0.6)
+    assert a["sigma"] == b["sigma"] == float(np.float32(np.float32(0.6) * np.float32(2)))
+    assert a["rse"] == b["rse"] and np.array_equal(a["emap"], b["emap"])
+
+
+def test_qnr_logistic_end_points_and_harmonic_mean():
+    """L(res_min) = 0.925 and L(res_max) = 0.075 (the logistic's defining points); QnR is the
+    harmonic mean of L and RSP, so QnR = RSP where L = RSP; the first maximum is reported."""
+    frc = np.array([0.05, 0.2, 0.125, np.nan], np.float32)
+    rsp = np.array([0.925, 0.075, 0.9, 0.5], np.float32)
+    q, best, idx = R.qnr_map(frc, rsp, auto_min_max=True)
+    assert abs(q[0] - 0.925) < 1e-6 and abs(q[1] - 0.075) < 1e-6
+    # L(midpoint) = 0.5 by the symmetry of the two defining points
+    assert abs(q[2] - 2 * 0.5 * 0.9 / 1.4) < 1e-6
+    assert np.isnan(q[3]) and idx == 0 and best == q[0]
