@@ -282,6 +282,44 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
 int esrrf_qnr_map(const float *frc, const float *rsp, long n, int auto_min_max, float res_min,
                   float res_max, float *qnr, float *best, long *best_index);
 
+/* ---- Rolling analysis of a long stack (gui/LiveSRRF_optimised_.java:276-384) ------------
+ * SR image r (0-based) covers frames [r*gap, r*gap + nF) of an n_slices stack, with nF, gap
+ * and the number of SR images from the plugin's plan (:621-627, esrrf_plan_rolling with
+ * rolling on); 2-D BIC, no drift.  Each result equals what initialise / calculateSRRF over
+ * that window / readSRRFbuffer returns (LiveSRRF_CL.java:159,636,749).  Frames stream
+ * through in chunks: one gradient and one RGC pass per frame serve every window containing
+ * it, so overlapping windows (gap < nF) do not repeat the RGC.  When more than about 60
+ * windows overlap one frame (gap < nF/61) esrrf_rolling_begin returns ESRRF_ERR_UNSUPPORTED
+ * and callers loop the engine per window, as the plugin does. */
+typedef struct esrrf_rolling esrrf_rolling;
+typedef struct esrrf_rolling_params {
+    int width, height;        /* raw frame size                                             */
+    int magnification;        /* 1..8                                                       */
+    float fwhm, sensitivity;
+    int n_frame_for_srrf;     /* the user's value: 0 = every frame (:621-623)               */
+    int frame_gap;            /* the user's value: 0 = n_frame_for_srrf (:624-627)          */
+    int int_weighting;
+    int do_mp_map_correction;
+    int chunk_frames;         /* frames per RGC pass, <= 0: 256                             */
+    int device;               /* -1: fastest                                                */
+} esrrf_rolling_params;
+int esrrf_rolling_create(const esrrf_rolling_params *p, esrrf_rolling **out);
+int esrrf_rolling_destroy(esrrf_rolling *r);
+int esrrf_rolling_set_stream(esrrf_rolling *r, void *hip_stream);
+/* Start a run over n_slices frames (u16 or f32) in host memory, which must stay valid until
+ * the run's last esrrf_rolling_next; *n_sr receives the number of SR images (nSRRFframe). */
+int esrrf_rolling_begin(esrrf_rolling *r, const void *frames, int dtype, int n_slices,
+                        int *n_sr);
+/* Same with the stack in device memory. */
+int esrrf_rolling_begin_device(esrrf_rolling *r, const void *d_frames, int dtype, int n_slices,
+                               int *n_sr);
+/* Compute until the next SR image (in order) is complete and copy its [4][hM][wM] (AVG, VAR,
+ * TAC2, INT; NaN -> 0) to host memory; *index receives its number.  Blocking. */
+int esrrf_rolling_next(esrrf_rolling *r, float *out, int *index);
+/* Same into device memory, asynchronous on the rolling stream. */
+int esrrf_rolling_next_device(esrrf_rolling *r, float *d_out, int *index);
+const char *esrrf_rolling_last_error(esrrf_rolling *r);
+
 const char *esrrf_last_error(esrrf_engine *e);
 const char *esrrf_version(void);
 

@@ -38,6 +38,9 @@ EXPORTS = [
     "esrrf_squirrel_create", "esrrf_squirrel_destroy", "esrrf_squirrel_set_stream",
     "esrrf_squirrel_dims", "esrrf_squirrel_run_device", "esrrf_squirrel_run",
     "esrrf_squirrel_last_error", "esrrf_sweep_squirrel", "esrrf_qnr_map",
+    "esrrf_rolling_create", "esrrf_rolling_destroy", "esrrf_rolling_set_stream",
+    "esrrf_rolling_begin", "esrrf_rolling_begin_device", "esrrf_rolling_next",
+    "esrrf_rolling_next_device", "esrrf_rolling_last_error",
 ]
 
 
@@ -99,6 +102,16 @@ class SquirrelResult(ctypes.Structure):
         ("sigma", ctypes.c_double), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
         ("rse", ctypes.c_double), ("rsp", ctypes.c_double), ("overblur", ctypes.c_int),
         ("evaluations", ctypes.c_int),
+    ]
+
+
+class RollingParams(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("height", ctypes.c_int), ("magnification", ctypes.c_int),
+        ("fwhm", ctypes.c_float), ("sensitivity", ctypes.c_float),
+        ("n_frame_for_srrf", ctypes.c_int), ("frame_gap", ctypes.c_int),
+        ("int_weighting", ctypes.c_int), ("do_mp_map_correction", ctypes.c_int),
+        ("chunk_frames", ctypes.c_int), ("device", ctypes.c_int),
     ]
 
 
@@ -176,9 +189,18 @@ def lib():
         "esrrf_sweep_squirrel": [vp, c_int, c_int, c_int, c_float, dp, dp, dp],
         "esrrf_qnr_map": [fp, fp, ctypes.c_long, c_int, c_float, c_float, fp, fp,
                           ctypes.POINTER(ctypes.c_long)],
+        "esrrf_rolling_create": [ctypes.POINTER(RollingParams), ctypes.POINTER(vp)],
+        "esrrf_rolling_destroy": [vp],
+        "esrrf_rolling_set_stream": [vp, vp],
+        "esrrf_rolling_begin": [vp, vp, c_int, c_int, ip],
+        "esrrf_rolling_begin_device": [vp, vp, c_int, c_int, ip],
+        "esrrf_rolling_next": [vp, fp, ip],
+        "esrrf_rolling_next_device": [vp, vp, ip],
+        "esrrf_rolling_last_error": [vp],
     }
     strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error",
-               "esrrf_rse_last_error", "esrrf_squirrel_last_error")
+               "esrrf_rse_last_error", "esrrf_squirrel_last_error",
+               "esrrf_rolling_last_error")
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args

@@ -20,6 +20,7 @@
 //               launch (liveSRRF.cl:363-393).
 //  k_generic_rgc  per-pixel restatement used when frames carry drift shifts (geometry then
 //               changes frame to frame) or M is outside the tiled kernel's range.
+#include "device_cumulants.h"
 #include "device_interp.h"
 #include "esrrf_internal.h"
 
@@ -175,17 +176,6 @@ __device__ __forceinline__ int axis_class(int X0, int xs_log2)
     return 32 - __clz(X0) - xs_log2;   // floor(log2 X0) - xs_log2 + 1
 }
 
-__device__ __forceinline__ float apply_sens(float C, float sens, int sens_int)
-{
-    if (!(C >= 0.0f)) return 0.0f;   // negative or NaN -> 0 (liveSRRF.cl:358-359)
-    switch (sens_int) {
-    case 1: return C;
-    case 2: return C * C;
-    case 3: return C * C * C;
-    case 4: { const float c2 = C * C; return c2 * c2; }
-    default: return powf(C, sens);
-    }
-}
 
 // Intensity of a border pixel (the reference's bilinear-extrapolation regime).
 __device__ __forceinline__ float border_intensity(const float *lR, int X0, int Y0,
@@ -615,22 +605,7 @@ __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, flo
                                          float &prv, float C0, float v, bool first,
                                          const BicRgcArgs &a)
 {
-    // temporal accumulation (liveSRRF.cl:363-393); Prev holds t/(n-1) (tac_prev_scaled)
-    const float C = apply_sens(C0, a.sens, a.sens_int);
-    const float t = INTW ? __fmul_rn(v, C) : C;
-    const float e2 = INTW ? __fmul_rn(__fmul_rn(t, v), C) : __fmul_rn(C, C);
-    if (first) {
-        avg = __fmul_rn(t, a.rn);
-        e2s = __fmul_rn(e2, a.rn);
-        tac = 0.0f;
-        in = __fmul_rn(v, a.rn);
-    } else {
-        avg = fmaf(t, a.rn, avg);
-        e2s = fmaf(e2, a.rn, e2s);
-        tac = fmaf(__fmul_rn(t, prv), a.rn1, tac);
-        in = fmaf(v, a.rn, in);
-    }
-    prv = __fmul_rn(t, a.rn1);
+    cumulate_bic<INTW>(avg, e2s, tac, in, prv, C0, v, first, a.sens, a.sens_int, a.rn, a.rn1);
 }
 
 template <int PPW, int MODE, bool INTW>
