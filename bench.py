@@ -51,10 +51,11 @@ def parse():
     ap.add_argument("--no-rse", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default RGC launch shape instead of esrrf_autotune's")
-    ap.add_argument("--mode", choices=("bic", "fht", "3d"), default="bic",
+    ap.add_argument("--mode", choices=("bic", "fht", "3d", "rolling"), default="bic",
                     help="bic: the production workload (default); fht: the same stack through "
                          "the Fourier-upsampling mode; 3d: 512 frames of a 9-plane 128x128 "
-                         "multi-focus montage, M=4")
+                         "multi-focus montage, M=4; rolling: the fused rolling engine "
+                         "against the per-window engine loop on the production stack")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain 1 GiB device copies (HBM counter calibration)")
     return ap.parse_args()
@@ -141,6 +142,11 @@ def bench_sweep(pkg, torch, stack):
     shapes = {f"M{m} fwhm{fw}": [sw.shape(i, j)[k] for k in ("nw", "ppw", "taps_in_lds",
                                                            "frames_per_pass")]
               for i, m in enumerate(mags) for j, fw in enumerate(fwhms)}
+    # the sweep's error mapping of the AVG images (Brent sigma, RSE, RSP per configuration)
+    t0 = time.perf_counter()
+    sq = sw.error_maps("AVG", crop_size=3)
+    sq_s = time.perf_counter() - t0
+    best = np.unravel_index(int(np.argmin(sq["rse"])), sq["rse"].shape)
     sw.close()
     # looped plain: a fresh engine per configuration, as the reference re-initialises
     d_out = torch.empty(4 * W * H * 25, dtype=torch.float32, device=dev)
@@ -167,9 +173,71 @@ def bench_sweep(pkg, torch, stack):
             "fused_configs_per_s": round(nconf / fused_s, 2),
             "looped_plain_configs_per_s": round(nconf / looped_s, 2),
             "speedup": round(looped_s / fused_s, 2),
+            "error_mapping_avg_s": round(sq_s, 3),
+            "error_mapping_configs_per_s": round(nconf / sq_s, 1),
+            "error_mapping_best_rse": {"rse": round(float(sq["rse"][best]), 5),
+                                       "rsp": round(float(sq["rsp"][best]), 5),
+                                       "M": mags[best[0]], "nf": nfs[best[1]],
+                                       "S": senss[best[2]], "fwhm": fwhms[best[3]]},
             "autotuned_shapes_nw_ppw_tlds_fpp": shapes,
             "workload": "500x256x256 u16; M {2,4,5} x fwhm {1..3 step .5} x S {1..4} x "
                         "first {100,250,500} frames; intensity weighting + MP on"}
+
+
+def bench_rolling(pkg, torch, stack):
+    """Rolling analysis of the 1000-frame stack (500 frames per SR image, a new SR image
+    every 50 frames: 11 SR images): the fused rolling engine against the plugin's loop of the
+    engine per SR image (gui/LiveSRRF_optimised_.java:276-339), frames resident in HBM."""
+    import numpy as np
+    C = pkg._capi
+    nF, gap = 500, 50
+    n = stack.shape[0]
+    plan = C.plan_rolling(n, nF, gap, True, n)
+    nsr = plan["nSRRFframe"]
+    dev = torch.device("cuda:0")
+    d_frames = torch.from_numpy(stack.view(np.int16).reshape(-1)).to(dev)
+    whM = W * M * H * M
+    d_out = torch.empty(4 * whM, dtype=torch.float32, device=dev)
+    r = pkg.Rolling(W, H, M, FWHM, SENS, nF, gap, True, True, chunk_frames=256, device=0)
+
+    def fused():
+        k = r.begin_device(d_frames.data_ptr(), C.ESRRF_U16, n)
+        for _ in range(k):
+            r.next_device(d_out.data_ptr())
+        torch.cuda.synchronize()
+
+    fused()   # warm-up (buffers, first launches)
+    t0 = time.perf_counter()
+    fused()
+    fused_s = time.perf_counter() - t0
+    r.close()
+    eng = pkg.Engine(params(C, W, H, M, FWHM, SENS, nF))
+    L, h = eng.L, eng.h
+    stream = torch.cuda.current_stream()
+    eng.check(L.esrrf_set_stream(h, ctypes.c_void_p(stream.cuda_stream)))
+    optr = ctypes.c_void_p(d_out.data_ptr())
+
+    def looped():
+        for k in range(nsr):
+            fptr = ctypes.c_void_p(d_frames.data_ptr() + 2 * W * H * k * gap)
+            eng.check(L.esrrf_reset_frame_position(h))
+            eng.check(L.esrrf_calculate_device(h, fptr, C.ESRRF_U16, nF, None, None))
+            eng.check(L.esrrf_read_result_device(h, optr))
+        torch.cuda.synchronize()
+
+    looped()
+    t0 = time.perf_counter()
+    looped()
+    looped_s = time.perf_counter() - t0
+    eng.close()
+    del d_frames, d_out
+    return {"sr_images": nsr, "frames_per_sr_image": nF, "frame_gap": gap,
+            "fused_s": round(fused_s, 4), "looped_engine_s": round(looped_s, 4),
+            "fused_sr_images_per_s": round(nsr / fused_s, 1),
+            "fused_output_mpx_per_s": round(nsr * nF * whM / fused_s / 1e6, 1),
+            "speedup": round(looped_s / fused_s, 2),
+            "workload": f"{n}x256x256 u16 in HBM; M=5, fwhm 1.5, S=2, {nF} frames per SR "
+                        f"image every {gap} frames; VAR/TAC2/MP as the headline"}
 
 
 def bench_rse(pkg, stack):
@@ -299,6 +367,9 @@ def main():
 
     nf = a.frames
     stack = synth.production_256x256x1000(n_frames=nf)
+    if a.mode == "rolling":
+        print(json.dumps({"rolling": bench_rolling(pkg, torch, stack)}))
+        return
     assert stack.shape == (nf, H, W) and stack.dtype == np.uint16
 
     dev = torch.device("cuda:0")
@@ -406,6 +477,7 @@ def main():
     if not a.no_cpu_baseline and not fht:
         res["cpu_baseline"] = cpu_baseline(stack, min(a.cpu_sample_frames, nf))
     if not a.no_sweep and not fht:
+        res["rolling"] = bench_rolling(pkg, torch, stack)
         res["sweep"] = bench_sweep(pkg, torch, synth.sweep_256x256x500())
     if not a.no_rse and not fht:
         res["rse_search"] = bench_rse(pkg, synth.rse_128x128x500())

@@ -202,32 +202,40 @@ int step_chunk(esrrf_rolling *r)
     a.sens_int = 1;
     a.int_weighting = r->intw;
     LCHK(r, launch_bic_rgc(a, r->shape.ppw, st));
-    // windows overlapping [c0, c1): r gap < c1 and r gap + nF > c0
+    // windows overlapping [c0, c1): r gap < c1 and r gap + nF > c0 -- consecutive windows
+    // from the first unfinished one, accumulated in launches of up to ROLL_GROUP
+    int wlo = first_open;
+    while (wlo < r->nSR && wlo * gap + nF <= c0) wlo++;
+    int whi = wlo;   // one past the last overlapping window
+    while (whi < r->nSR && whi * gap < c1) whi++;
+    if (whi - wlo > ROLL_MAX_WIN) return lfail(r, ESRRF_ERR_STATE, "too many windows");
+    if (whi - 1 - r->next_out >= r->nslots)
+        return lfail(r, ESRRF_ERR_STATE, "window %d has no free slot", whi - 1);
     RollAccArgs c{};
     c.C = r->d_C;
     c.V = r->d_V;
+    c.state = r->d_state;
     c.w = r->w;
     c.h = r->h;
     c.M = r->M;
     c.nframes = n;
     c.nF = nF;
+    c.c0 = c0;
+    c.gap = gap;
+    c.nslots = r->nslots;
     c.sens = r->sens;
     c.sens_int = r->sens_int;
     c.int_weighting = r->intw;
     c.rn = 1.0f / (float)nF;
     c.rn1 = 1.0f / (float)(nF - 1);
-    std::vector<int> done;
-    for (int win = first_open; win < r->nSR && win * gap < c1; win++) {
-        if (win * gap + nF <= c0) continue;
-        if (win - r->next_out >= r->nslots)
-            return lfail(r, ESRRF_ERR_STATE, "window %d has no free slot", win);
-        if (c.nwin == ROLL_MAX_WIN) return lfail(r, ESRRF_ERR_STATE, "too many windows");
-        c.k0[c.nwin] = c0 - win * gap;
-        c.state[c.nwin] = slot_of(r, win);
-        c.nwin++;
-        if (win * gap + nF <= c1) done.push_back(win);
+    for (int g = wlo; g < whi; g += ROLL_GROUP) {
+        c.r0 = g;
+        c.nwin = std::min(ROLL_GROUP, whi - g);
+        LCHK(r, launch_roll_acc(c, st));
     }
-    LCHK(r, launch_roll_acc(c, st));
+    std::vector<int> done;
+    for (int win = wlo; win < whi; win++)
+        if (win * gap + nF <= c1) done.push_back(win);
     for (int win : done) {
         float *out = slot_of(r, win);
         LCHK(r, launch_var(out, whM, st));

@@ -4,17 +4,20 @@
 
 namespace esrrf {
 
-constexpr int ROLL_GROUP = 8;     // windows per thread
-constexpr int ROLL_MAX_WIN = 64;  // windows per launch
+constexpr int ROLL_GROUP = 16;    // windows per launch: one wave each
+constexpr int ROLL_MAX_WIN = 64;  // windows one chunk may touch (launched in groups)
 
+// The launch covers the consecutive windows r0 .. r0 + nwin - 1; window r starts at frame
+// r gap of the stack, the chunk at frame c0, and window r keeps its cumulants in slot
+// r mod nslots of `state` ([nslots][5][hM][wM]: AVG, E2, TAC2, INT, Prev).
 struct RollAccArgs {
     const float *C, *V;           // emitted CGLH / intensity of the chunk [nframes][M*M][h][w]
+    float *state;
     int w, h, M;
     int nframes;                  // frames of the chunk
     int nF;                       // frames per SR window
-    int nwin;                     // windows in this launch
-    int k0[ROLL_MAX_WIN];         // index inside each window of the chunk's first frame
-    float *state[ROLL_MAX_WIN];   // each window's [5][hM][wM]: AVG, E2, TAC2, INT, Prev
+    int r0, nwin;                 // first window, windows (<= ROLL_GROUP)
+    int c0, gap, nslots;
     float sens;
     int sens_int;
     int int_weighting;
