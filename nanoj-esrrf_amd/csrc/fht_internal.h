@@ -11,9 +11,10 @@ struct FhtPlan {
     int scaleW, scaleH, xROIp, yROIp;   // mirror padding (FourierInterpolation.java:118-172)
     int xROI, yROI;                     // crop of the MN x MN result (:50-54)
     int chunk;                          // frames per batch of the scratch spectra
+    int scatter;                        // 1: last pass stores pixel by pixel (k_fht_inv_cols)
     float2 *F;                          // [chunk][N][N] spectra
     float2 *Y;                          // [chunk][M][N][N] row-transformed twiddled spectra
-    const float2 *tw;                   // N/2 twiddles e^{-2 pi i k / N}
+    const float2 *tw;                   // N twiddles e^{-2 pi i k / N}
     const float2 *twMN;                 // M*N twiddles e^{-2 pi i j / (M N)}
 };
 

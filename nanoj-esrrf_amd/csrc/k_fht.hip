@@ -18,31 +18,63 @@
 #include "fht_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace esrrf {
 
 // ---------------------------------------------------------------------------------------
-// Radix-2 Stockham FFT (forward, e^{-i}) of G signals of length N held in LDS: A holds the
-// input, B is scratch; returns the buffer holding the result.  tw[k] = e^{-2 pi i k / N}.
+// Stockham FFT (forward, e^{-i}) of G signals of length N = 2^log2N held in LDS: A holds the
+// input, B is scratch; returns the buffer holding the result.  Radix-4 stages (after one
+// radix-2 stage when log2N is odd): with Q = N/4 and the sub-transform length L, input j
+// (k = j mod L) combines A[j + m Q] w^{m k}, w = e^{-2 pi i / (4L)}, into the 4-point DFT
+// stored at 4 (j - k) + k + q L (the self-sorting form).  tw (in LDS) holds
+// tw[t] = e^{-2 pi i t / N} for t < N.
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 template <int G>
 __device__ __forceinline__ float2 *stockham(float2 *A, float2 *B, int N, int log2N,
-                                            const float2 *__restrict__ tw)
+                                            const float2 *tw)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int half = N >> 1;
-    for (int L = 1, s = 0; s < log2N; L <<= 1, s++) {
-        const int twstep = half / L;   // e^{-2 pi i k/(2L)} = tw[k * N/(2L)]
+    int L = 1;
+    if (log2N & 1) {   // radix-2 stage, L = 1 (all twiddles 1)
+        const int half = N >> 1;
         for (int e = tid; e < G * half; e += nthr) {
-            const int g = e / half, j = e - g * half;
+            const int g = e >> (log2N - 1), j = e - g * half;
+            const float2 u = A[g * N + j], x = A[g * N + j + half];
+            B[g * N + 2 * j] = make_float2(u.x + x.x, u.y + x.y);
+            B[g * N + 2 * j + 1] = make_float2(u.x - x.x, u.y - x.y);
+        }
+        __syncthreads();
+        float2 *t = A;
+        A = B;
+        B = t;
+        L = 2;
+    }
+    const int Q = N >> 2;
+    for (; L < N; L <<= 2) {
+        const int tstep = N / (4 * L);
+        for (int e = tid; e < G * Q; e += nthr) {
+            const int g = e >> (log2N - 2), j = e - g * Q;
             const int k = j & (L - 1);
-            const float2 wv = tw[k * twstep];
-            const float2 u = A[g * N + j];
-            const float2 x = A[g * N + j + half];
-            const float2 v = make_float2(x.x * wv.x - x.y * wv.y, x.x * wv.y + x.y * wv.x);
-            const int o = g * N + 2 * j - k;
-            B[o] = make_float2(u.x + v.x, u.y + v.y);
-            B[o + L] = make_float2(u.x - v.x, u.y - v.y);
+            const float2 *a = A + g * N + j;
+            const float2 a0 = a[0];
+            const float2 a1 = cmul(a[Q], tw[k * tstep]);
+            const float2 a2 = cmul(a[2 * Q], tw[2 * k * tstep]);
+            const float2 a3 = cmul(a[3 * Q], tw[3 * k * tstep]);
+            const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y);
+            const float2 t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+            const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
+            const float2 t3 = make_float2(a1.x - a3.x, a1.y - a3.y);
+            float2 *o = B + g * N + 4 * (j - k) + k;
+            o[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+            o[L] = make_float2(t1.x + t3.y, t1.y - t3.x);        // t1 - i t3
+            o[2 * L] = make_float2(t0.x - t2.x, t0.y - t2.y);
+            o[3 * L] = make_float2(t1.x - t3.y, t1.y + t3.x);    // t1 + i t3
         }
         __syncthreads();
         float2 *t = A;
@@ -50,6 +82,12 @@ __device__ __forceinline__ float2 *stockham(float2 *A, float2 *B, int N, int log
         B = t;
     }
     return A;
+}
+
+// the N twiddles of the block's transforms into LDS (call before the first barrier)
+__device__ __forceinline__ void stage_twiddles(float2 *TW, const float2 *__restrict__ tw, int N)
+{
+    for (int t = threadIdx.x; t < N; t += blockDim.x) TW[t] = tw[t];
 }
 
 // twiddle e^{-2 pi i k(q) v / (MN)} of frequency index q (signed frequency k(q), Nyquist on
@@ -71,7 +109,8 @@ __global__ void __launch_bounds__(256) k_fht_fwd_rows(const float *__restrict__ 
 {
     extern __shared__ float2 sm[];
     const int N = p.N;
-    float2 *A = sm, *B = sm + G * N;
+    float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
+    stage_twiddles(TW, p.tw, N);
     const int y0 = blockIdx.x * G, f = blockIdx.y;
     const float *S = src + src_stride * f;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
@@ -84,7 +123,7 @@ __global__ void __launch_bounds__(256) k_fht_fwd_rows(const float *__restrict__ 
         A[e] = make_float2(S[(long)qy * p.w + px], 0.0f);
     }
     __syncthreads();
-    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    const float2 *R = stockham<G>(A, B, N, p.log2N, TW);
     float2 *Fo = F + (long)N * N * f + (long)y0 * N;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) Fo[e] = R[e];
 }
@@ -95,7 +134,8 @@ __global__ void __launch_bounds__(256) k_fht_fwd_cols(float2 *__restrict__ F, Fh
 {
     extern __shared__ float2 sm[];
     const int N = p.N;
-    float2 *A = sm, *B = sm + G * N;
+    float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
+    stage_twiddles(TW, p.tw, N);
     const int a0 = blockIdx.x * G;
     float2 *Ff = F + (long)N * N * blockIdx.y;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
@@ -103,7 +143,7 @@ __global__ void __launch_bounds__(256) k_fht_fwd_cols(float2 *__restrict__ F, Fh
         A[g * N + k] = Ff[(long)k * N + a0 + g];
     }
     __syncthreads();
-    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    const float2 *R = stockham<G>(A, B, N, p.log2N, TW);
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
         const int g = e % G, k = e / G;
         Ff[(long)k * N + a0 + g] = R[g * N + k];
@@ -118,7 +158,8 @@ __global__ void __launch_bounds__(256) k_fht_inv_rows(const float2 *__restrict__
 {
     extern __shared__ float2 sm[];
     const int N = p.N, M = p.M, MN = p.M * p.N;
-    float2 *A = sm, *B = sm + G * N;
+    float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
+    stage_twiddles(TW, p.tw, N);
     const int y0 = blockIdx.x * G;
     const int f = blockIdx.y / M, b = blockIdx.y - f * M;
     const float2 *Fr = F + (long)N * N * f + (long)y0 * N;
@@ -130,7 +171,7 @@ __global__ void __launch_bounds__(256) k_fht_inv_rows(const float2 *__restrict__
         A[e] = make_float2(hv * t.x, hv * t.y);
     }
     __syncthreads();
-    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    const float2 *R = stockham<G>(A, B, N, p.log2N, TW);
     float2 *Ro = Rb + ((long)f * M + b) * N * N + (long)y0 * N;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) Ro[e] = R[e];
 }
@@ -146,7 +187,8 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__
     extern __shared__ float2 sm[];
     const int N = p.N, M = p.M, MN = p.M * p.N;
     const int wM = p.M * p.w, hM = p.M * p.h;
-    float2 *A = sm, *B = sm + G * N;
+    float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
+    stage_twiddles(TW, p.tw, N);
     const int a0 = blockIdx.x * G;
     const int MM = M * M;
     const int f = blockIdx.y / MM, bd = blockIdx.y - f * MM;
@@ -159,7 +201,7 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__
         A[g * N + k] = make_float2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
     }
     __syncthreads();
-    const float2 *R = stockham<G>(A, B, N, p.log2N, p.tw);
+    const float2 *R = stockham<G>(A, B, N, p.log2N, TW);
     float *O = out + out_stride * f;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
         const int g = e % G, c = e / G;
@@ -170,17 +212,91 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__
     }
 }
 
+// Step 4 with coalesced stores: the block takes column group a0..a0+G-1 of one (frame, d)
+// and runs the column transforms of every b in turn (each exactly as k_fht_inv_cols does),
+// collecting (Re - Im)/(MN)^2 in an LDS tile of output rows y = M c + d, x = M a + b; the
+// tile then goes out as runs of M G consecutive pixels per row instead of pixels M apart.
+// grid (N/G, frames * M), d from blockIdx.y
+template <int G>
+__global__ void __launch_bounds__(256) k_fht_inv_cols_rows(const float2 *__restrict__ Rb,
+                                                           float *__restrict__ out,
+                                                           long out_stride, FhtPlan p, float inv)
+{
+    extern __shared__ float2 sm[];
+    const int N = p.N, M = p.M, MN = p.M * p.N;
+    const int wM = p.M * p.w, hM = p.M * p.h;
+    float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
+    float *T = reinterpret_cast<float *>(TW + N);   // [N][M G]
+    stage_twiddles(TW, p.tw, N);
+    const int MG = M * G;
+    const int a0 = blockIdx.x * G;
+    const int f = blockIdx.y / M, d = blockIdx.y - f * M;
+    for (int b = 0; b < M; b++) {
+        const float2 *Rc = Rb + ((long)f * M + b) * N * N;
+        for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+            const int g = e % G, k = e / G;
+            const float2 v = Rc[(long)k * N + a0 + g];
+            const float2 t = twid(p.twMN, k, d, N, MN);
+            A[g * N + k] = make_float2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
+        }
+        __syncthreads();
+        const float2 *R = stockham<G>(A, B, N, p.log2N, TW);
+        for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+            const int g = e % G, c = e / G;
+            const float2 z = R[g * N + c];
+            T[c * MG + M * g + b] = (z.x - z.y) * inv;
+        }
+        __syncthreads();   // A and B are reused by the next b
+    }
+    float *O = out + out_stride * f;
+    const int x0 = M * a0 - p.xROI;
+    for (int e = threadIdx.x; e < N * MG; e += blockDim.x) {
+        const int c = e / MG, xx = e - c * MG;
+        const int X = x0 + xx, Y = M * c + d - p.yROI;
+        if (X < 0 || X >= wM || Y < 0 || Y >= hM) continue;
+        O[(long)Y * wM + X] = T[e];
+    }
+}
+
 static int grid_of(long n)
 {
     long g = (n + 255) / 256;
     return (int)(g > 4096 ? 4096 : (g < 1 ? 1 : g));
 }
 
+// The row-tiled last pass: G columns per block with A, B, the twiddles and the output tile
+// within 40 KiB (measured at N = 256, M = 5: G = 4 runs the upsampling of 341 frames in
+// 11.8 ms, G = 8 in 15.7 ms, G = 1 in 14.9 ms).
+template <int G>
+static void inv_cols_rows_g(const FhtPlan &p, const float2 *Y, float *out, long stride,
+                            float inv, int k, hipStream_t s)
+{
+    const size_t lds = sizeof(float2) * ((size_t)2 * G * p.N + p.N) +
+                       sizeof(float) * (size_t)p.N * p.M * G;
+    hipLaunchKernelGGL((k_fht_inv_cols_rows<G>), dim3(p.N / G, k * p.M), dim3(256), lds, s, Y,
+                       out, stride, p, inv);
+}
+
+static void inv_cols_rows(const FhtPlan &p, const float2 *Y, float *out, long stride, float inv,
+                          int k, hipStream_t s)
+{
+    const size_t per = 2 * sizeof(float2) * (size_t)p.N + sizeof(float) * (size_t)p.N * p.M;
+    int G = 16;
+    while (G > 1 && (G > p.N || (size_t)G * per + sizeof(float2) * p.N > 40 * 1024)) G >>= 1;
+    switch (G) {
+    case 16: inv_cols_rows_g<16>(p, Y, out, stride, inv, k, s); break;
+    case 8: inv_cols_rows_g<8>(p, Y, out, stride, inv, k, s); break;
+    case 4: inv_cols_rows_g<4>(p, Y, out, stride, inv, k, s); break;
+    case 2: inv_cols_rows_g<2>(p, Y, out, stride, inv, k, s); break;
+    default: inv_cols_rows_g<1>(p, Y, out, stride, inv, k, s); break;
+    }
+}
+
 template <int G>
 static void upsample_g(const FhtPlan &p, const float *frames, float *out, int nf, hipStream_t s)
 {
     const int N = p.N, MM = p.M * p.M;
-    const size_t lds = 2 * sizeof(float2) * (size_t)G * N;
+    const size_t lds = sizeof(float2) * ((size_t)2 * G * N + N);
     const long npx = (long)p.w * p.h, nM = (long)MM * npx;
     const float inv = (float)(1.0 / ((double)p.M * p.N * (double)p.M * p.N));
     for (int f0 = 0; f0 < nf; f0 += p.chunk) {
@@ -190,16 +306,20 @@ static void upsample_g(const FhtPlan &p, const float *frames, float *out, int nf
         hipLaunchKernelGGL((k_fht_fwd_cols<G>), dim3(N / G, k), dim3(256), lds, s, p.F, p);
         hipLaunchKernelGGL((k_fht_inv_rows<G>), dim3(N / G, k * p.M), dim3(256), lds, s, p.F,
                            p.Y, p);
-        hipLaunchKernelGGL((k_fht_inv_cols<G>), dim3(N / G, k * MM), dim3(256), lds, s, p.Y,
-                           out + nM * f0, nM, p, inv);
+        if (p.scatter)
+            hipLaunchKernelGGL((k_fht_inv_cols<G>), dim3(N / G, k * MM), dim3(256), lds, s,
+                               p.Y, out + nM * f0, nM, p, inv);
+        else
+            inv_cols_rows(p, p.Y, out + nM * f0, nM, inv, k, s);
     }
 }
 
 int fht_group(int N)
 {
-    // G signals per block (G <= N so the grid N / G is whole) with 2*G*N complex values of
-    // LDS <= 64 KiB
-    return std::max(1, std::min(std::min(16, N), 4096 / N));
+    // G signals per block (G <= N so the grid N / G is whole): about 16 KiB of signals per
+    // block, so that several blocks share a CU (measured at N = 256: G = 4 and 8 run the
+    // passes in 15.6 ms per 341 frames, G = 16 in 16.7 ms)
+    return std::max(1, std::min(std::min(16, N), 1024 / N));
 }
 
 // Upsample nf consecutive raw frames (float, w x h each) into nf magnified frames.

@@ -135,21 +135,24 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
             p.yROIp = (sH * c.h - N) / 2;
             p.xROI = (c.M * N - c.M * c.w) / 2;
             p.yROI = (c.M * N - c.M * c.h) / 2;
-            // twiddles e^{-2 pi i k/N} (k < N/2) then e^{-2 pi i j/(MN)} (j < MN), in double
+            // twiddles e^{-2 pi i k/N} (k < N) then e^{-2 pi i j/(MN)} (j < MN), in double
             const int MN = c.M * N;
-            std::vector<float2> tw(N / 2 + MN);
-            for (int k = 0; k < N / 2; k++) {
+            std::vector<float2> tw(N + MN);
+            for (int k = 0; k < N; k++) {
                 const double ang = -2.0 * M_PI * k / N;
                 tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
             }
             for (int j = 0; j < MN; j++) {
                 const double ang = -2.0 * M_PI * j / MN;
-                tw[N / 2 + j] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+                tw[N + j] = make_float2((float)std::cos(ang), (float)std::sin(ang));
             }
             // frames per batch: spectra of (M + 1) N x N complex per frame within 1 GiB
             const size_t per = (size_t)(c.M + 1) * N * N * sizeof(float2);
             p.chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)c.nF, (1ull << 30) / per));
             p.chunk = std::min(p.chunk, 65535 / (c.M * c.M));   // grid.y = chunk * M^2
+            // last pass: row-tiled stores (default) or the pixel-by-pixel kernel
+            const char *sc = getenv("ESRRF_FHT_SCATTER");
+            p.scatter = (sc && atoi(sc) != 0) ? 1 : 0;
             float2 *dtw = nullptr;
             if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N * p.chunk) ||
                 !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * p.chunk) ||
@@ -159,7 +162,7 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
                                       hipMemcpyHostToDevice);
             if (st != hipSuccess) return err(ms, ESRRF_ERR_HIP, "FHT twiddles", st);
             p.tw = dtw;
-            p.twMN = dtw + N / 2;
+            p.twMN = dtw + N;
         }
         build_fht_taps(ms, c.radius, c.tSS, c.tSO);
         if (!dmalloc(&ms.d_taps, sizeof(Tap) * ms.h_taps.size()) ||

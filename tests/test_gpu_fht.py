@@ -105,6 +105,29 @@ def test_fht_batched_rgc_matches_single_kernel(pkg, monkeypatch, M, w, h, loads)
     assert np.array_equal(res["0"], res["1"])
 
 
+@pytest.mark.parametrize("M,w,h", [(5, 32, 20), (3, 21, 17), (2, 16, 16), (4, 40, 33)])
+def test_fht_row_tiled_last_pass_matches_scatter(pkg, monkeypatch, M, w, h):
+    """The last inverse pass with row-tiled stores (the default) writes exactly what the
+    pixel-by-pixel kernel writes: the same column transforms, only the store order differs
+    (crops of padded non-power-of-two frames included)."""
+    st = stack(6, w, h, seed=80 + M)
+    res, grads = {}, {}
+    for sc in ("1", "0"):
+        monkeypatch.setenv("ESRRF_FHT_SCATTER", sc)
+        e = pkg.LiveSRRF_CL()
+        e.checkDevices()
+        e.initialise(w, h, M, 1.5, 2, 6, 6, 20000, e.allCLdevices[0], True, True, True, None,
+                     0, 1)
+        e.resetFramePosition()
+        e.loadDriftXYGPUbuffer(np.zeros(6), np.zeros(6))
+        e.calculateSRRF(st)
+        grads[sc] = np.asarray(e.readGradientBuffers()).copy()
+        res[sc] = np.asarray(e.readSRRFbuffer()).copy()
+        e.release()
+    assert np.array_equal(grads["0"], grads["1"])
+    assert np.array_equal(res["0"], res["1"])
+
+
 def test_fht_no_intensity_weighting(pkg, O):
     st = stack(5, 20, 20, seed=17)
     got, _, _ = run_engine(pkg, st, 3, 1.5, 1, intW=False)
