@@ -282,6 +282,31 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
 int esrrf_qnr_map(const float *frc, const float *rsp, long n, int auto_min_max, float res_min,
                   float res_max, float *qnr, float *best, long *best_index);
 
+/* ---- Fourier ring correlation (the sweep's FRC resolution, gui/ParametersSweep_.java:403-447)
+ * FRC.calculateFireNumber(ip1, ip2, FIXED_1_OVER_7) belongs to NanoJ-Core, which the reference
+ * does not contain; this is the standard FRC: centred square crop, Tukey(0.25) taper, zero
+ * padding to the power of two N >= the square's side (N <= 4096), ring r = nearest integer to
+ * |k|, FRC(r) = sum Re(F1 conj F2) / sqrt(sum |F1|^2 sum |F2|^2) in double, resolution
+ * N / r* in pixels at the first crossing r* of 1/7 after ring 0 (linear between rings), NaN
+ * when the curve never crosses.  NaN pixels read as 0. */
+typedef struct esrrf_frc esrrf_frc;
+int esrrf_frc_create(int width, int height, int device, esrrf_frc **out);
+int esrrf_frc_destroy(esrrf_frc *q);
+int esrrf_frc_set_stream(esrrf_frc *q, void *hip_stream);
+/* Rings of the curve (N/2) and the transform size N. */
+int esrrf_frc_rings(esrrf_frc *q, int *n_rings, int *transform_size);
+/* Two [height][width] device images; *resolution in pixels; curve (host, N/2 values) may be
+ * NULL.  Blocking. */
+int esrrf_frc_run_device(esrrf_frc *q, const float *d_a, const float *d_b, double *resolution,
+                         float *curve);
+int esrrf_frc_run(esrrf_frc *q, const float *a, const float *b, double *resolution,
+                  float *curve);
+const char *esrrf_frc_last_error(esrrf_frc *q);
+/* FRC resolution (SR pixels) of every configuration of two sweeps with identical parameters
+ * run on the odd and the even frames (:403-447 with :808-816's frame mapping), for channel 0
+ * AVG, 1 VAR or 2 TAC2: [n_magnifications][n_n_frames][n_sensitivity][n_fwhm]. */
+int esrrf_sweep_frc(esrrf_sweep *odd, esrrf_sweep *even, int channel, double *resolution);
+
 /* ---- Rolling analysis of a long stack (gui/LiveSRRF_optimised_.java:276-384) ------------
  * SR image r (0-based) covers frames [r*gap, r*gap + nF) of an n_slices stack, with nF, gap
  * and the number of SR images from the plugin's plan (:621-627, esrrf_plan_rolling with

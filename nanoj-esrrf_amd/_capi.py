@@ -41,6 +41,8 @@ EXPORTS = [
     "esrrf_rolling_create", "esrrf_rolling_destroy", "esrrf_rolling_set_stream",
     "esrrf_rolling_begin", "esrrf_rolling_begin_device", "esrrf_rolling_next",
     "esrrf_rolling_next_device", "esrrf_rolling_last_error",
+    "esrrf_frc_create", "esrrf_frc_destroy", "esrrf_frc_set_stream", "esrrf_frc_rings",
+    "esrrf_frc_run_device", "esrrf_frc_run", "esrrf_frc_last_error", "esrrf_sweep_frc",
 ]
 
 
@@ -197,10 +199,18 @@ def lib():
         "esrrf_rolling_next": [vp, fp, ip],
         "esrrf_rolling_next_device": [vp, vp, ip],
         "esrrf_rolling_last_error": [vp],
+        "esrrf_frc_create": [c_int, c_int, c_int, ctypes.POINTER(vp)],
+        "esrrf_frc_destroy": [vp],
+        "esrrf_frc_set_stream": [vp, vp],
+        "esrrf_frc_rings": [vp, ip, ip],
+        "esrrf_frc_run_device": [vp, vp, vp, dp, fp],
+        "esrrf_frc_run": [vp, fp, fp, dp, fp],
+        "esrrf_frc_last_error": [vp],
+        "esrrf_sweep_frc": [vp, vp, c_int, dp],
     }
     strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error",
                "esrrf_rse_last_error", "esrrf_squirrel_last_error",
-               "esrrf_rolling_last_error")
+               "esrrf_rolling_last_error", "esrrf_frc_last_error")
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args

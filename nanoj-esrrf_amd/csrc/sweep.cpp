@@ -464,3 +464,35 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
     }
     return ESRRF_OK;
 }
+
+int esrrf_sweep_frc(esrrf_sweep *odd, esrrf_sweep *even, int channel, double *resolution)
+{
+    if (!odd || !even || odd == even || channel < 0 || channel > 2 || !resolution)
+        return ESRRF_ERR_INVALID;
+    std::scoped_lock lk(odd->mu, even->mu);
+    if (!odd->have_result || !even->have_result)
+        return sfail(odd, ESRRF_ERR_STATE, "both sweeps must have run");
+    if (odd->Ms != even->Ms || odd->fwhm != even->fwhm || odd->sens != even->sens ||
+        odd->nfs != even->nfs || odd->w != even->w || odd->h != even->h || odd->dev != even->dev)
+        return sfail(odd, ESRRF_ERR_INVALID, "the two sweeps must have the same parameters");
+    SCHK(odd, hipSetDevice(odd->dev));
+    SCHK(odd, hipStreamSynchronize(even->stream));
+    const long ncfg = (long)odd->nfs.size() * odd->sens.size() * odd->fwhm.size();
+    for (size_t im = 0; im < odd->Ms.size(); im++) {
+        const int M = odd->Ms[im];
+        esrrf_frc *q = nullptr;
+        int rc = esrrf_frc_create(odd->w * M, odd->h * M, odd->dev, &q);
+        if (rc) return sfail(odd, rc, "FRC plan for M=%d: error %d", M, rc);
+        esrrf_frc_set_stream(q, odd->stream);
+        const long whM = (long)odd->w * odd->h * M * M;
+        for (long c = 0; c < ncfg && rc == ESRRF_OK; c++) {
+            const long off = c * 4 * whM + channel * whM;
+            rc = esrrf_frc_run_device(q, odd->d_out[im] + off, even->d_out[im] + off,
+                                      &resolution[(long)im * ncfg + c], nullptr);
+            if (rc) odd->err = std::string("FRC: ") + esrrf_frc_last_error(q);
+        }
+        esrrf_frc_destroy(q);
+        if (rc) return rc;
+    }
+    return ESRRF_OK;
+}

@@ -1,5 +1,5 @@
-"""SQUIRREL error map of an SR image (ErrorMapLiveSRRF.java) and the QnR map
-(gui/GetQnRmap_.java) over the C ABI.
+"""SQUIRREL error map of an SR image (ErrorMapLiveSRRF.java), the FRC resolution and the
+QnR map (gui/GetQnRmap_.java) over the C ABI.
 
 ``ErrorMapLiveSRRF`` keeps the reference class's constructor, ``optimise(ref, sr)`` and
 ``calculateErrorMap()``, and its public results (``globalRMSE``, ``globalPPMCC``, ``fpSRC``,
@@ -96,3 +96,45 @@ def qnr_map(frc_map, rsp_map, auto_min_max=True, res_min=0.05, res_max=0.2):
                                   C.f32ptr(q), ctypes.byref(best), ctypes.byref(idx)))
     w = frc.shape[-1] if frc.ndim >= 1 else 1
     return q, best.value, (idx.value % w, idx.value // w)
+
+
+class FRC:
+    """Fourier ring correlation of two images of one size (the sweep's
+    ``FRC.calculateFireNumber(ip1, ip2, FIXED_1_OVER_7)``; NanoJ-Core's class is not part
+    of the reference, see esrrf_frc_create for the definition used)."""
+
+    def __init__(self, width, height, device=-1):
+        self.L = C.lib()
+        h = ctypes.c_void_p()
+        C.check(self.L.esrrf_frc_create(int(width), int(height), int(device), ctypes.byref(h)))
+        self.h = h
+        self.shape = (int(height), int(width))
+        nr, n = ctypes.c_int(), ctypes.c_int()
+        self.L.esrrf_frc_rings(self.h, ctypes.byref(nr), ctypes.byref(n))
+        self.n_rings, self.transform_size = nr.value, n.value
+
+    def calculateFireNumber(self, ip1, ip2):
+        """Resolution in pixels; the curve of the last call is in ``self.curve``."""
+        a = np.ascontiguousarray(ip1, dtype=np.float32)
+        b = np.ascontiguousarray(ip2, dtype=np.float32)
+        if a.shape != self.shape or b.shape != self.shape:
+            raise ValueError(f"images must be {self.shape}")
+        res = ctypes.c_double()
+        self.curve = np.empty(self.n_rings, np.float32)
+        rc = self.L.esrrf_frc_run(self.h, C.f32ptr(a), C.f32ptr(b), ctypes.byref(res),
+                                  C.f32ptr(self.curve))
+        if rc != C.ESRRF_OK:
+            m = self.L.esrrf_frc_last_error(self.h)
+            raise C.EsrrfError(rc, m.decode() if m else "")
+        return res.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.esrrf_frc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

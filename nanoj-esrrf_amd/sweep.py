@@ -110,6 +110,17 @@ class ParametersSweep:
             rse.ctypes.data_as(dp), rsp.ctypes.data_as(dp), sig.ctypes.data_as(dp)))
         return dict(rse=rse, rsp=rsp, sigma=sig)
 
+    def frc_maps(self, even, channel="AVG"):
+        """FRC resolution (SR pixels) of every configuration between this sweep, run on the
+        odd frames, and ``even``, run on the even frames with the same parameters
+        (gui/ParametersSweep_.java:403-447): [n_magnifications][n_n_frames][n_S][n_fwhm]."""
+        ch = {"AVG": 0, "VAR": 1, "TAC2": 2}[channel]
+        out = np.empty((self.mags.size, self.nfs.size, self.sens.size, self.fwhm.size),
+                       np.float64)
+        self._check(self.L.esrrf_sweep_frc(self.h, even.h, ch,
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def close(self):
         if self.h:
             self.L.esrrf_sweep_destroy(self.h)
@@ -120,3 +131,11 @@ class ParametersSweep:
             self.close()
         except Exception:
             pass
+
+
+def odd_even(stack, n_frames):
+    """The odd and even frame sets of the sweep's FRC mode (gui/ParametersSweep_.java:808-816:
+    frame f of a configuration is stack slice 2(f-1)+1, resp. 2f, 1-based)."""
+    a = np.asarray(stack)
+    n = int(max(n_frames))
+    return a[0:2 * n:2], a[1:2 * n:2]

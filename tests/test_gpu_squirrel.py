@@ -139,3 +139,50 @@ def test_qnr_map_matches_oracle(pkg):
         eq, ebest, eidx = R.qnr_map(frc, rsp, auto_min_max=auto, res_min=0.05, res_max=0.2)
         assert np.allclose(q.ravel(), eq, rtol=1e-6, equal_nan=True)
         assert best == pytest.approx(ebest, rel=1e-6) and y * 5 + x == eidx
+
+
+@pytest.mark.parametrize("w,h", [(100, 90), (64, 64), (130, 257)])
+def test_frc_matches_oracle(pkg, w, h):
+    from oracle import frc as F
+    rng = np.random.default_rng(w + h)
+    N = 128
+    k = np.fft.fftfreq(N) * N
+    base = np.fft.ifft2(np.fft.fft2(rng.normal(size=(N, N))) *
+                        ((k[:, None] ** 2 + k[None, :] ** 2) <= 20 ** 2)).real
+    sig = np.tile(base, (3, 3))[:h, :w] * 5.0
+    a = (sig + rng.normal(size=(h, w))).astype(np.float32)
+    b = (sig + rng.normal(size=(h, w))).astype(np.float32)
+    a[3, 4] = np.nan
+    q = pkg.FRC(w, h)
+    res = q.calculateFireNumber(a, b)
+    curve, eres = F.frc(a, b)
+    assert np.allclose(q.curve, curve, atol=2e-5), float(np.abs(q.curve - curve).max())
+    assert math.isclose(res, eres, rel_tol=1e-4), (res, eres)
+    assert math.isnan(q.calculateFireNumber(a, a)) and np.allclose(q.curve[1:], 1.0, atol=1e-5)
+    q.close()
+
+
+def test_sweep_frc_maps_match_per_configuration(pkg):
+    from nanoj_esrrf_amd import synth
+    from oracle import frc as F
+    w, h = 24, 22
+    st = synth.blinking_stack(24, w, h, 50, 3, seed=98)
+    odd, even = pkg.odd_even(st, [8, 12])
+    assert np.array_equal(odd[1], st[2]) and np.array_equal(even[0], st[1])
+    sw = [pkg.ParametersSweep(w, h, [2, 3], [1.0, 2.0], [1, 2], [8, 12]) for _ in range(2)]
+    sw[0].run(odd)
+    sw[1].run(even)
+    for ch, name in ((0, "AVG"), (2, "TAC2")):
+        maps = sw[0].frc_maps(sw[1], name)
+        for im in range(2):
+            for inf in range(2):
+                for isn in range(2):
+                    for ifw in range(2):
+                        a = sw[0].read(im, inf, isn, ifw)[ch]
+                        b = sw[1].read(im, inf, isn, ifw)[ch]
+                        _, eres = F.frc(a, b)
+                        got = maps[im, inf, isn, ifw]
+                        assert (math.isnan(got) and math.isnan(eres)) or \
+                            math.isclose(got, eres, rel_tol=1e-3), (name, im, inf, got, eres)
+    for s in sw:
+        s.close()
