@@ -20,6 +20,8 @@
 //               launch (liveSRRF.cl:363-393).
 //  k_generic_rgc  per-pixel restatement used when frames carry drift shifts (geometry then
 //               changes frame to frame) or M is outside the tiled kernel's range.
+#include <cstdlib>
+
 #include "device_cumulants.h"
 #include "device_interp.h"
 #include "esrrf_internal.h"
@@ -94,11 +96,11 @@ __device__ __forceinline__ float gint_at(const float *t, int pitch, int ox, int 
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             float p = 0.0f;
-            p = p + r[0] * wx0;
-            p = p + r[1] * wx1;
-            p = p + r[2] * wx2;
-            p = p + r[3] * wx3;
-            q = q + p * wy[j];
+            p = fmaf(r[0], wx0, p);
+            p = fmaf(r[1], wx1, p);
+            p = fmaf(r[2], wx2, p);
+            p = fmaf(r[3], wx3, p);
+            q = fmaf(p, wy[j], q);
             r += pitch;
         }
         return q;
@@ -129,6 +131,133 @@ __global__ void __launch_bounds__(256) k_bic_prep(const float *__restrict__ px,
     U[(long)f * wI * hI + (long)ky * wI + kx] = u;
 }
 
+// ---------------------------------------------------------------------------------------
+// k_bic_prep2: the same values as k_bic_prep over 32x32 sample points per block, with the
+// interpolation's row sums shared: the 4-tap sum along x of a gradient row depends only on
+// the row and the sample's x (u0 and parity), so each block forms it once per (row, x) in
+// LDS and every sample point combines 4 of them along y, in the order gint_at uses.
+// Border points take gint_at's bilinear-extrapolation branch on the same tile.
+// ---------------------------------------------------------------------------------------
+constexpr int P2_T = 32;   // sample points per side of a block
+constexpr int P2_G = 20;   // gradient tile: image columns kx0/2 - 2 .. kx0/2 + 17
+constexpr int P2_R = 21;   // raw tile: one more column and row on the low side
+
+__device__ __forceinline__ void cubic_half_weights(int odd, float &w0, float &w1, float &w2,
+                                                   float &w3)
+{
+    const float c0 = -0.0625f, c1 = 0.5625f;
+    w0 = odd ? c0 : 0.0f;
+    w1 = odd ? c1 : 1.0f;
+    w2 = odd ? c1 : 0.0f;
+    w3 = odd ? c0 : 0.0f;
+}
+
+__global__ void __launch_bounds__(256) k_bic_prep2(const float *__restrict__ px,
+                                                   float2 *__restrict__ U, int w, int h)
+{
+    __shared__ float lraw[P2_R * P2_R];
+    __shared__ float lgx[P2_G * P2_G], lgy[P2_G * P2_G];
+    __shared__ float rsx[P2_G * P2_T], rsy[P2_G * P2_T];   // row sums [tile row][sample x]
+    const int f = blockIdx.z;
+    const int kx0 = blockIdx.x * P2_T, ky0 = blockIdx.y * P2_T;
+    const int gx0 = (kx0 >> 1) - 2, gy0 = (ky0 >> 1) - 2;
+    const int tid = threadIdx.x;
+    const float *P = px + (long)w * h * f;
+    for (int e = tid; e < P2_R * P2_R; e += 256) {
+        const int r = e / P2_R, c = e - r * P2_R;
+        const int ix = min(max(gx0 - 1 + c, 0), w - 1);
+        const int iy = min(max(gy0 - 1 + r, 0), h - 1);
+        lraw[e] = P[(long)iy * w + ix];
+    }
+    __syncthreads();
+    // 2-point gradient (liveSRRF.cl:199-204) at image (gx0 + c, gy0 + r)
+    for (int e = tid; e < P2_G * P2_G; e += 256) {
+        const int r = e / P2_G, c = e - r * P2_G;
+        const int x1 = gx0 + c, y1 = gy0 + r;
+        float gx = 0.0f, gy = 0.0f;
+        if (x1 >= 0 && x1 < w && y1 >= 0 && y1 < h) {
+            const float p = lraw[(r + 1) * P2_R + (c + 1)];
+            gx = p - lraw[(r + 1) * P2_R + c];
+            gy = p - lraw[r * P2_R + (c + 1)];
+        }
+        lgx[e] = gx;
+        lgy[e] = gy;
+    }
+    __syncthreads();
+    // row sums along x: GxInt samples x = kx, GyInt samples x = kx - 1
+    for (int e = tid; e < P2_G * P2_T; e += 256) {
+        const int r = e / P2_T, q = e - r * P2_T;
+        const int kx = kx0 + q;
+        float w0, w1, w2, w3;
+        {
+            const int X = kx, u0 = X >> 1;
+            cubic_half_weights(X & 1, w0, w1, w2, w3);
+            const float *t = lgx + r * P2_G + (u0 - 1 - gx0);
+            float p = 0.0f;
+            p = fmaf(t[0], w0, p);
+            p = fmaf(t[1], w1, p);
+            p = fmaf(t[2], w2, p);
+            p = fmaf(t[3], w3, p);
+            rsx[e] = p;
+        }
+        {
+            const int X = kx - 1, u0 = X >> 1;
+            cubic_half_weights(X & 1, w0, w1, w2, w3);
+            const float *t = lgy + r * P2_G + (u0 - 1 - gx0);
+            float p = 0.0f;
+            p = fmaf(t[0], w0, p);
+            p = fmaf(t[1], w1, p);
+            p = fmaf(t[2], w2, p);
+            p = fmaf(t[3], w3, p);
+            rsy[e] = p;
+        }
+    }
+    __syncthreads();
+    const int wI = 2 * w, hI = 2 * h;
+    float2 *Uf = U + (long)wI * hI * f;
+#pragma unroll
+    for (int o = 0; o < (P2_T * P2_T) / 256; o++) {
+        const int e = tid + o * 256;
+        const int q = e & (P2_T - 1), yy = e / P2_T;
+        const int kx = kx0 + q, ky = ky0 + yy;
+        if (kx >= wI || ky >= hI) continue;
+        float2 u = make_float2(0.0f, 0.0f);
+        if (kx >= 1 && ky >= 1) {   // gates 0 < vx, 0 < vy of liveSRRF.cl:282,285
+            float tx, ty, w0, w1, w2, w3;
+            {   // GxInt at (kx, ky - 1)
+                const int X = kx, Y = ky - 1, u0 = X >> 1, v0 = Y >> 1;
+                if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+                    cubic_half_weights(Y & 1, w0, w1, w2, w3);
+                    const float *c = rsx + (v0 - 1 - gy0) * P2_T + q;
+                    tx = 0.0f;
+                    tx = fmaf(c[0], w0, tx);
+                    tx = fmaf(c[P2_T], w1, tx);
+                    tx = fmaf(c[2 * P2_T], w2, tx);
+                    tx = fmaf(c[3 * P2_T], w3, tx);
+                } else {
+                    tx = interp_tile(lgx, P2_G, gx0, gy0, w, h, (float)X / 2.0f, (float)Y / 2.0f);
+                }
+            }
+            {   // GyInt at (kx - 1, ky)
+                const int X = kx - 1, Y = ky, u0 = X >> 1, v0 = Y >> 1;
+                if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+                    cubic_half_weights(Y & 1, w0, w1, w2, w3);
+                    const float *c = rsy + (v0 - 1 - gy0) * P2_T + q;
+                    ty = 0.0f;
+                    ty = fmaf(c[0], w0, ty);
+                    ty = fmaf(c[P2_T], w1, ty);
+                    ty = fmaf(c[2 * P2_T], w2, ty);
+                    ty = fmaf(c[3 * P2_T], w3, ty);
+                } else {
+                    ty = interp_tile(lgy, P2_G, gx0, gy0, w, h, (float)X / 2.0f, (float)Y / 2.0f);
+                }
+            }
+            u = unit_or_zero(tx, ty);
+        }
+        Uf[(long)ky * wI + kx] = u;
+    }
+}
+
 // Debug read-back of GxInt / GyInt of one frame (LiveSRRF_CL.readGradientBuffers).
 __global__ void __launch_bounds__(256) k_bic_gint(const float *__restrict__ px,
                                                   float *__restrict__ gxi,
@@ -149,8 +278,14 @@ __global__ void __launch_bounds__(256) k_bic_gint(const float *__restrict__ px,
 hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s)
 {
     if (nF <= 0) return hipSuccess;
-    dim3 grid((2 * w + PREP_T - 1) / PREP_T, (2 * h + PREP_T - 1) / PREP_T, nF);
-    hipLaunchKernelGGL(k_bic_prep, grid, dim3(PREP_T, PREP_T), 0, s, px, U, w, h);
+    const char *v = getenv("ESRRF_BIC_PREP");
+    if (v && atoi(v) == 1) {   // one 4x4 stencil per sample point (k_bic_prep)
+        dim3 grid((2 * w + PREP_T - 1) / PREP_T, (2 * h + PREP_T - 1) / PREP_T, nF);
+        hipLaunchKernelGGL(k_bic_prep, grid, dim3(PREP_T, PREP_T), 0, s, px, U, w, h);
+        return hipGetLastError();
+    }
+    dim3 grid((2 * w + P2_T - 1) / P2_T, (2 * h + P2_T - 1) / P2_T, nF);
+    hipLaunchKernelGGL(k_bic_prep2, grid, dim3(256), 0, s, px, U, w, h);
     return hipGetLastError();
 }
 
