@@ -507,15 +507,22 @@ __device__ __forceinline__ float tap3_acc(const float4 u, const float4 q, float 
     return fmaf(q.w, __builtin_amdgcn_fmed3f(-1.0f, -sn, -1.0f - c), acc);
 }
 
-__global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tables t)
+// SPLIT waves per phase: with 2, the two waves of a phase take the two halves of its tap
+// list and meet in LDS at the frame's barrier (twice the waves per CU for the same LDS).
+template <int SPLIT>
+__global__ void __launch_bounds__(256 * SPLIT) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tables t)
 {
+    constexpr int NT = 256 * SPLIT;            // threads per block
+    constexpr int FE = K3_FE / SPLIT;          // tile elements each thread loads per frame
     extern __shared__ __align__(16) unsigned char smem[];
     float4 *lU = reinterpret_cast<float4 *>(smem);
     float4 *lTv = lU + t.NZ * t.PZ;
     int *lTo = reinterpret_cast<int *>(lTv + t.gtaps);
+    float *lAcc = reinterpret_cast<float *>(lTo + t.gtaps);   // [4][64] second halves
 
     const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wave = wv & 3, half = wv >> 2;   // phase slot, tap half
     const int lx = lane & 7, ly = lane >> 3;
     const int w = a.w, h = a.h, nP = a.nP, M = a.M, MM = M * M;
     const int tiles_x = (w + 7) >> 3;
@@ -534,7 +541,7 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
     // the tap tables of the block's phases (plus the spares after them)
     const int tg0 = PB[ph0].first_tap;
     const int tg1 = PB[phN - 1].first_tap + PB[phN - 1].ntaps;
-    for (int e = tid; e < tg1 - tg0 + TAP_SPARE; e += 256) {
+    for (int e = tid; e < tg1 - tg0 + TAP_SPARE; e += NT) {
         lTv[e] = t.tapv[tg0 + e];
         lTo[e] = t.tapo[tg0 + e];
     }
@@ -545,10 +552,10 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
     const int wI = 2 * w, hI = 2 * h, whI = wI * hI;
     const int per_plane = t.TH * t.TW;
     const int nel = nz * per_plane;
-    int gofs[K3_FE], lidx[K3_FE];
+    int gofs[FE], lidx[FE];
 #pragma unroll
-    for (int q = 0; q < K3_FE; q++) {
-        const int e = tid + q * 256;
+    for (int q = 0; q < FE; q++) {
+        const int e = tid + q * NT;
         gofs[q] = -1;
         lidx[q] = -1;
         if (e < nel) {
@@ -613,7 +620,7 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
     {
         const float4 *Uf = a.U + (long)a.frame0 * fstride;
 #pragma unroll
-        for (int q = 0; q < K3_FE; q++)
+        for (int q = 0; q < FE; q++)
             if (lidx[q] >= 0)
                 lU[lidx[q]] = gofs[q] >= 0 ? Uf[gofs[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
@@ -627,24 +634,27 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
         // lateral intensities of this frame first, then the next frame's tile: the tile
         // loads stay in flight through the tap loop
         float v_1 = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-        if (inimg) {
+        if (inimg && half == 0) {
             const float *Lf = a.L + (long)fi * nP * whM + (long)yM * a.wM + xM;
             v_1 = Lf[whM * zm1];
             v0 = Lf[whM * z0];
             v1 = Lf[whM * (z0 + 1)];
             v2 = Lf[whM * zp2];
         }
-        float4 st[K3_FE];
+        float4 st[FE];
         if (more) {
             const float4 *Un = a.U + (long)(a.frame0 + fi + 1) * fstride;
 #pragma unroll
-            for (int q = 0; q < K3_FE; q++)
+            for (int q = 0; q < FE; q++)
                 st[q] = gofs[q] >= 0 ? Un[gofs[q]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
+        float acc = 0.0f;
         if (active) {
-            float acc = 0.0f;
+            // this wave's share of the tap list (both halves multiples of 4)
+            const int h0 = SPLIT == 1 ? 0 : (half ? ((nt >> 2) + 1) / 2 * 4 : 0);
+            const int h1 = SPLIT == 1 ? nt : (half ? nt : ((nt >> 2) + 1) / 2 * 4);
 #pragma unroll 1
-            for (int k = 0; k < nt; k += 4) {
+            for (int k = h0; k < h1; k += 4) {
                 const int4 o = *reinterpret_cast<const int4 *>(To + k);
                 const float4 q0 = Tv[k], q1 = Tv[k + 1], q2 = Tv[k + 2], q3 = Tv[k + 3];
                 const float4 u0 = lU[base + o.x], u1 = lU[base + o.y];
@@ -654,6 +664,11 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
                 acc = tap3_acc(u2, q2, acc);
                 acc = tap3_acc(u3, q3, acc);
             }
+            if (SPLIT == 2 && half) lAcc[wave * 64 + lane] = acc;
+        }
+        if (SPLIT == 2) __syncthreads();   // partial sums written; every wave done with the tile
+        if (active && half == 0) {
+            if (SPLIT == 2) acc += lAcc[wave * 64 + lane];
             if (inimg) {
                 float C = (wsum + acc) / wsg;
                 C = apply_sens3(C, a.sens, a.sens_int);
@@ -687,14 +702,16 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
             }
         }
         if (more) {
-            __syncthreads();   // every wave is done with this frame's tile
+            if (SPLIT == 1) __syncthreads();   // every wave is done with this frame's tile
 #pragma unroll
-            for (int q = 0; q < K3_FE; q++)
+            for (int q = 0; q < FE; q++)
                 if (lidx[q] >= 0) lU[lidx[q]] = st[q];
             __syncthreads();
+        } else if (SPLIT == 2) {
+            __syncthreads();   // the partial sums are read before the next frame reuses them
         }
     }
-    if (inimg) {
+    if (inimg && half == 0) {
         a.out[vox] = s0;
         a.out[vox + whdM] = s1;
         a.out[vox + 2 * whdM] = s2;
@@ -705,7 +722,8 @@ __global__ void __launch_bounds__(256) k3d_rgc3(const Rgc3dArgs a, const Rgc3Tab
 
 size_t rgc3_lds_bytes(const Rgc3Tables &t)
 {
-    return sizeof(float4) * ((size_t)t.NZ * t.PZ + t.gtaps) + sizeof(int) * (size_t)t.gtaps;
+    return sizeof(float4) * ((size_t)t.NZ * t.PZ + t.gtaps) + sizeof(int) * (size_t)t.gtaps +
+           sizeof(float) * 4 * 64;
 }
 
 // Gated samples of one frame without drift, summed over all voxels.
@@ -803,7 +821,10 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t
     if (drift_zero && t && kern == 3 && off31 && rgc3_lds_bytes(*t) <= lds_limit) {
         const int tiles = ((a.w + 7) >> 3) * ((a.h + 7) >> 3);
         const dim3 g3(tiles, (a.M * a.M + 3) / 4, a.nP * a.M);
-        hipLaunchKernelGGL(k3d_rgc3, g3, dim3(256), rgc3_lds_bytes(*t), s, a, *t);
+        if (getenv_int("ESRRF_3D_SPLIT", 2) == 1)
+            hipLaunchKernelGGL(k3d_rgc3<1>, g3, dim3(256), rgc3_lds_bytes(*t), s, a, *t);
+        else
+            hipLaunchKernelGGL(k3d_rgc3<2>, g3, dim3(512), rgc3_lds_bytes(*t), s, a, *t);
         if (kname) *kname = "k3d_rgc3";
     } else if (drift_zero && off32 && kern >= 2) {
         hipLaunchKernelGGL(k3d_rgc2<8>, grid, dim3(64, 4), 0, s, a);
