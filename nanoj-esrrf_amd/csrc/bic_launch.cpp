@@ -61,14 +61,14 @@ static bool tlds_fits(const BicTables &t, bool adj)
 
 // LDS of the frame-pair kernel for this geometry within the 64 KiB a block may take by
 // default
-static bool pair_fits(const BicTables &t)
+static bool pair_fits(const BicTables &t, bool tsm = false)
 {
     BicRgcArgs a{};
     a.PP = t.geo.PP;
     a.PH = t.geo.PH;
     a.TW = t.geo.TW;
     a.max_class_taps = t.geo.max_class_taps;
-    return bic_pair_lds_bytes(a) <= 64 * 1024;
+    return bic_pair_lds_bytes(a, tsm) <= 64 * 1024;
 }
 
 BicShape bic_default_shape(const BicTables &t, int M, bool adj)
@@ -102,7 +102,8 @@ std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj)
             s.ppw = ppw;
             s.taps_in_lds = tl;
             out.push_back(s);
-            if (tl == 1 && !adj && ppw <= 4 && pair_fits(t)) {
+            // frame pairs, taps in LDS (tl 1) or through scalar loads (tl 0)
+            if (!adj && ppw <= 4 && pair_fits(t, tl == 0)) {
                 s.fp = 2;
                 out.push_back(s);
             }

@@ -124,7 +124,7 @@ hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h
 // *kname (when given) receives the name of the kernel launched.
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s,
                           const char **kname = nullptr);
-size_t bic_pair_lds_bytes(const BicRgcArgs &a);   // LDS of the frame-pair kernel
+size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm);   // LDS of the frame-pair kernel
 hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
 hipError_t launch_var(float *out, long nvox, hipStream_t s);
 hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,

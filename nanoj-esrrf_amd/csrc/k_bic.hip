@@ -743,7 +743,9 @@ __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, flo
     cumulate_bic<INTW>(avg, e2s, tac, in, prv, C0, v, first, a.sens, a.sens_int, a.rn, a.rn1);
 }
 
-template <int PPW, int MODE, bool INTW>
+// TSM: the tap tables are read with scalar loads from global memory (wave-uniform
+// addresses, held in SGPRs, the next chunk prefetched) instead of from an LDS copy.
+template <int PPW, int MODE, bool INTW, bool TSM>
 __global__ void __launch_bounds__(1024)
 __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
                                                    const Tap *__restrict__ taps,
@@ -755,7 +757,8 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
     const int usz = 4 * a.PP * a.PH;
     float2 *lR = reinterpret_cast<float2 *>(lU + usz);   // raw tiles of f and f+1, interleaved
     Tap *lT = reinterpret_cast<Tap *>(lR + RAW_SZ);
-    int *lPlan = reinterpret_cast<int *>(lT + a.max_class_taps + TAP_SPARE);
+    int *lPlan = TSM ? reinterpret_cast<int *>(lT)
+                     : reinterpret_cast<int *>(lT + a.max_class_taps + TAP_SPARE);
     constexpr int PH_UNROLL = PPW <= 4 ? PPW : 1;
 
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -829,7 +832,7 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
         lPlan[e] = (pl * planesz + r * a.PP + c) | (gy << 15) | (gx << 23);
     }
     const int tcls0 = PHS[0].first_tap;
-    {
+    if (!TSM) {
         const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
         for (int e = tid; e < tcls1 - tcls0 + TAP_SPARE; e += nthr) lT[e] = taps[tcls0 + e];
     }
@@ -875,12 +878,36 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
             if (ph < MM) {
                 const Phase P = PHf[ph];
                 float acc0 = 0.0f, acc1 = 0.0f;
-                const Tap *Tl = lT + (P.first_tap - tcls0);
+                const Tap *Tl = TSM ? taps + P.first_tap : lT + (P.first_tap - tcls0);
                 // (one chunk of 4 taps per iteration: two in flight exceed the register
-                // budget of 5 waves per SIMD)
+                // budget of 5 waves per SIMD; with TSM the next chunk's taps are loaded
+                // into SGPRs during this one -- every phase's list is a multiple of 4 long
+                // and the table ends in TAP_SPARE zero taps, so t + 7 stays inside it)
+                Tap n0, n1, n2, n3;
+                if (TSM) {
+                    n0 = Tl[0];
+                    n1 = Tl[1];
+                    n2 = Tl[2];
+                    n3 = Tl[3];
+                }
 #pragma unroll 1
                 for (int t = 0; t < P.ntaps; t += 4) {
-                    const Tap c0 = Tl[t], c1 = Tl[t + 1], c2 = Tl[t + 2], c3 = Tl[t + 3];
+                    Tap c0, c1, c2, c3;
+                    if (TSM) {
+                        c0 = n0;
+                        c1 = n1;
+                        c2 = n2;
+                        c3 = n3;
+                        n0 = Tl[t + 4];
+                        n1 = Tl[t + 5];
+                        n2 = Tl[t + 6];
+                        n3 = Tl[t + 7];
+                    } else {
+                        c0 = Tl[t];
+                        c1 = Tl[t + 1];
+                        c2 = Tl[t + 2];
+                        c3 = Tl[t + 3];
+                    }
                     const float4 u0 = lU[base + c0.off];
                     const float4 u1 = lU[base + c1.off];
                     const float4 u2 = lU[base + c2.off];
@@ -968,18 +995,25 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
     }
 }
 
-size_t bic_pair_lds_bytes(const BicRgcArgs &a)
+size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm)
 {
     return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float2) * RAW_SZ +
-           sizeof(Tap) * ((size_t)a.max_class_taps + TAP_SPARE) +
+           (tsm ? 0 : sizeof(Tap) * ((size_t)a.max_class_taps + TAP_SPARE)) +
            sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
 }
 
+// taps_in_lds 0: the tap tables are read with scalar loads (no LDS copy)
 template <int PPW, int MODE, bool INTW>
 static hipError_t launch_pair_m(const BicRgcArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW>), dim3(a.tiles_x * a.tiles_y),
-                       dim3(64 * a.nw), bic_pair_lds_bytes(a), s, a, a.taps, a.phases, a.tapk);
+    if (!a.taps_in_lds)
+        hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW, true>), dim3(a.tiles_x * a.tiles_y),
+                           dim3(64 * a.nw), bic_pair_lds_bytes(a, true), s, a, a.taps, a.phases,
+                           a.tapk);
+    else
+        hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW, false>), dim3(a.tiles_x * a.tiles_y),
+                           dim3(64 * a.nw), bic_pair_lds_bytes(a, false), s, a, a.taps,
+                           a.phases, a.tapk);
     return hipGetLastError();
 }
 
@@ -1036,8 +1070,8 @@ static hipError_t launch_ppw(const BicRgcArgs &a, hipStream_t s)
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s, const char **kname)
 {
     if (a.nframes <= 0) return hipSuccess;
-    const bool pair = a.frames_per_pass == 2 && a.mode != 2 && a.taps_in_lds && ppw <= 4 &&
-                      a.tac_prev_scaled && bic_pair_lds_bytes(a) <= 64 * 1024;
+    const bool pair = a.frames_per_pass == 2 && a.mode != 2 && ppw <= 4 && a.tac_prev_scaled &&
+                      bic_pair_lds_bytes(a, !a.taps_in_lds) <= 64 * 1024;
     if (kname) *kname = pair ? "k_bic_rgc2" : "k_bic_rgc";
     if (pair) {
         switch (ppw) {

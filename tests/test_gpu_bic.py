@@ -97,6 +97,37 @@ def test_bic_frame_pair_kernel_matches_single_frame(pkg, monkeypatch, M, w, h, l
     assert np.array_equal(res["1"], res["2"])
 
 
+@pytest.mark.parametrize("M,w,h,loads", [
+    (5, 40, 32, [(0, 5), (5, 9)]), (4, 27, 19, [(0, 7)]), (3, 64, 48, [(0, 4)]),
+])
+def test_bic_frame_pair_taps_from_scalar_loads(pkg, monkeypatch, M, w, h, loads):
+    """The frame-pair kernel reading its tap tables with scalar loads from global memory
+    reproduces the variant that stages them in LDS bit for bit (same operations, the tap
+    values only come from SGPRs instead of VGPRs)."""
+    n = loads[-1][1]
+    st = stack(n, w, h, seed=70 + M)
+    res = {}
+    monkeypatch.setenv("ESRRF_BIC_FP", "2")
+    for v in ("0", "1"):
+        monkeypatch.setenv("ESRRF_BIC_TLDS", v)
+        res[v], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
+                                  nFor=n, loads=loads)
+    assert np.array_equal(res["0"], res["1"])
+
+
+@pytest.mark.parametrize("nw", ["7", "9", "13"])
+def test_bic_frame_pair_balanced_wave_counts(pkg, monkeypatch, nw):
+    """Blocks of 7, 9 or 13 waves (the balanced shapes for 25 phases) give the same bits as
+    the default 8-wave block: each (pixel, phase) is evaluated the same way by whichever
+    wave holds it."""
+    st = stack(6, 40, 32, seed=91)
+    res = {}
+    for v in ("8", nw):
+        monkeypatch.setenv("ESRRF_BIC_NW", v)
+        res[v], _, _ = run_engine(pkg, st, 5, 1.5, 2, nOnGPU=6, loads=[(0, 6)])
+    assert np.array_equal(res["8"], res[nw])
+
+
 def test_bic_no_intensity_weighting(pkg, O):
     st = stack(6, 24, 20, seed=5)
     got, _, _ = run_engine(pkg, st, 4, 1.5, 2, intW=False)
