@@ -132,7 +132,8 @@ int esrrf_plan_load(const int *plan6, int r, int l, int *first_index, int *n_fra
 
 /* Autotuner of the tiled 2-D kernel (no counterpart in the reference, whose launch shape is
  * the GUI's blockLength, LiveSRRF_CL.java:699-735): times every supported launch shape
- * (waves per block x tap tables in LDS or not) over the frames of the current load, `reps`
+ * (waves per block x tap tables in LDS or through scalar loads x one or two frames per
+ * pass) over the frames of the current load, `reps`
  * launches each, into scratch outputs, and keeps the fastest for this engine.  Reports it. */
 int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_lds, float *ms);
 /* Launch shape the engine's tiled 2-D kernel uses now (no counterpart in the reference):

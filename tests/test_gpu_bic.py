@@ -128,6 +128,29 @@ def test_bic_frame_pair_balanced_wave_counts(pkg, monkeypatch, nw):
     assert np.array_equal(res["8"], res[nw])
 
 
+@pytest.mark.parametrize("M,w,h", [(3, 4, 4), (2, 5, 4), (5, 4, 9)])
+def test_bic_minimum_frame_sizes(pkg, O, M, w, h):
+    """Frames at the 4x4 minimum: every output pixel is a border pixel (bilinear
+    extrapolation, gated windows cut by both edges) and one tile overhangs the image."""
+    st = stack(5, w, h, seed=17 + w * h, emitters=6, fil=1)
+    got, cnt, _ = run_engine(pkg, st, M, 1.5, 2)
+    exp, ocnt, _ = run_oracle(O, st, M, 1.5, 2)
+    assert cnt == ocnt
+    assert_parity(got.reshape(4, h * M, w * M), exp.reshape(4, h * M, w * M),
+                  f"M{M} {w}x{h}")
+
+
+@pytest.mark.parametrize("w,h,M", [(3, 8, 2), (8, 3, 2), (8, 8, 0), (0, 8, 2)])
+def test_bic_rejects_invalid_arguments(pkg, w, h, M):
+    """Frames under 4x4 pixels and a magnification below 1 are refused at initialise
+    with the engine's error, before any device memory is allocated."""
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    with pytest.raises(pkg._capi.EsrrfError):
+        e.initialise(w, h, M, 1.5, 2, 4, 4, 20000, e.allCLdevices[0], True, True, False, None,
+                     0, 1)
+
+
 def test_bic_no_intensity_weighting(pkg, O):
     st = stack(6, 24, 20, seed=5)
     got, _, _ = run_engine(pkg, st, 4, 1.5, 2, intW=False)
