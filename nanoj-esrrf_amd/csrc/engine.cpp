@@ -69,6 +69,7 @@ struct esrrf_engine {
     BicShape shape;              // its launch shape (default or autotuned)
     bool tiled = false;          // tiled BIC kernel usable
     int stage = 0;
+    int pair_direct = 0;
     std::vector<float> h_drift;  // host copy of the drift buffer (x M)
     ModeState ms;                // FHT / 3-D state
     // bookkeeping
@@ -197,6 +198,7 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
         a.sens_int = e->sens_int;
         a.int_weighting = e->p.int_weighting ? 1 : 0;
         a.stage = e->stage;
+        a.pair_direct = e->pair_direct;
         ProfScope ps(e, 0);
         HIPCHK(e, launch_bic_rgc(a, e->shape.ppw, e->stream, &e->rgc_kernel));
         return ESRRF_OK;
@@ -497,6 +499,8 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
             if (const char *ov = getenv("ESRRF_BIC_FP")) e->shape.fp = atoi(ov) == 2 ? 2 : 1;
             e->stage = 0;
             if (const char *ov = getenv("ESRRF_BIC_STAGE")) e->stage = atoi(ov) != 0;
+            e->pair_direct = 0;
+            if (const char *ov = getenv("ESRRF_BIC_PAIR_DIRECT")) e->pair_direct = atoi(ov) != 0;
         }
     } else {
         ModeCreate mc{};

@@ -57,7 +57,9 @@ struct BicRgcArgs {
     int tiles_x, tiles_y;
     int nw;                // waves per block
     int zero;              // always 0 (defeats loop-invariant hoisting of phase tables)
-    int stage;             // 1: stage the next frame's tiles in registers
+    int stage;             // 1: stage the next frame's tiles in registers (k_bic_rgc)
+    int pair_direct;       // 1: the frame-pair kernel fills the next pair's tiles after the
+                           //    current pair instead of staging them in registers
     int taps_in_lds;       // 1: stage each block's tap tables in LDS
     int max_class_taps;    // largest tap-table size of one x/y class (LDS sizing)
     int frames_per_pass;   // 2: frame-pair kernel (modes 0, 1 with taps in LDS), else 1

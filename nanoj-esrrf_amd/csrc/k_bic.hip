@@ -744,10 +744,12 @@ __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, flo
 }
 
 // TSM: the tap tables are read with scalar loads from global memory (wave-uniform
-// addresses, held in SGPRs, the next chunk prefetched) instead of from an LDS copy.
-template <int PPW, int MODE, bool INTW, bool TSM>
+// addresses, held in SGPRs) instead of from an LDS copy.  STG > 0: the next frame pair's
+// tiles are loaded into registers (STG elements per thread) before the current pair is
+// evaluated and only stored to LDS between the barriers.
+template <int PPW, int MODE, bool INTW, bool TSM, int STG>
 __global__ void __launch_bounds__(1024)
-__attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
+__attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
                                                    const Tap *__restrict__ taps,
                                                    const Phase *__restrict__ phases,
                                                    const int *__restrict__ tapk)
@@ -862,12 +864,55 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
         }
     };
 
+    // register staging (STG > 0; the launch guarantees nU <= STG * nthr and
+    // RAW_T^2 <= nthr)
+    float4 su[STG > 0 ? STG : 1];
+    float2 sr = make_float2(0.0f, 0.0f);
+    auto prefetch = [&](int f, bool two) {
+        const float2 *Ua = a.U + whI * f;
+#pragma unroll
+        for (int k = 0; k < STG; k++) {
+            const int e = tid + k * nthr;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (e < nU) {
+                const int pk = lPlan[e];
+                const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
+                const int kx = kxo + gx, ky = kyo + gy;
+                if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI) {
+                    const long g = (long)ky * wI + kx;
+                    const float2 ua = Ua[g];
+                    const float2 ub = two ? Ua[g + whI] : make_float2(0.0f, 0.0f);
+                    v = make_float4(ua.x, ua.y, ub.x, ub.y);
+                }
+            }
+            su[k] = v;
+        }
+        if (tid < RAW_T * RAW_T) {
+            const float *Pa = a.px + wh * f;
+            const int r = tid / RAW_T, c = tid - r * RAW_T;
+            const long g = (long)min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+            sr = make_float2(Pa[g], two ? Pa[g + wh] : 0.0f);
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int k = 0; k < STG; k++) {
+            const int e = tid + k * nthr;
+            if (e < nU) lU[lPlan[e] & 0x7fff] = su[k];
+        }
+        if (tid < RAW_T * RAW_T) {
+            const int r = tid / RAW_T, c = tid - r * RAW_T;
+            lR[r * RAW_P + c] = sr;
+        }
+    };
+
     fill(a.frame0, a.nframes > 1);
     __syncthreads();
 
     for (int fi = 0; fi < a.nframes; fi += 2) {
         const bool two = fi + 1 < a.nframes;
         const bool first = (fi == 0) && a.first_init;
+        if (STG && fi + 2 < a.nframes) prefetch(a.frame0 + fi + 2, fi + 3 < a.nframes);
         // (a.zero is 0 at run time; it keeps the per-phase table reads inside the frame loop)
         const int zf = a.zero * fi;
         const Phase *PHf = PHS + zf;
@@ -974,7 +1019,10 @@ __attribute__((amdgpu_waves_per_eu(5))) k_bic_rgc2(const BicRgcArgs a,
         }
         if (fi + 2 < a.nframes) {
             __syncthreads();   // every wave is done with this pair's tiles
-            fill(a.frame0 + fi + 2, fi + 3 < a.nframes);
+            if (STG)
+                commit();
+            else
+                fill(a.frame0 + fi + 2, fi + 3 < a.nframes);
             __syncthreads();
         }
     }
@@ -1003,18 +1051,28 @@ size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm)
 }
 
 // taps_in_lds 0: the tap tables are read with scalar loads (no LDS copy)
+template <int PPW, int MODE, bool INTW, int STG>
+static hipError_t launch_pair_s(const BicRgcArgs &a, hipStream_t s)
+{
+    if (!a.taps_in_lds)
+        hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW, true, STG>),
+                           dim3(a.tiles_x * a.tiles_y), dim3(64 * a.nw),
+                           bic_pair_lds_bytes(a, true), s, a, a.taps, a.phases, a.tapk);
+    else
+        hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW, false, STG>),
+                           dim3(a.tiles_x * a.tiles_y), dim3(64 * a.nw),
+                           bic_pair_lds_bytes(a, false), s, a, a.taps, a.phases, a.tapk);
+    return hipGetLastError();
+}
+
 template <int PPW, int MODE, bool INTW>
 static hipError_t launch_pair_m(const BicRgcArgs &a, hipStream_t s)
 {
-    if (!a.taps_in_lds)
-        hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW, true>), dim3(a.tiles_x * a.tiles_y),
-                           dim3(64 * a.nw), bic_pair_lds_bytes(a, true), s, a, a.taps, a.phases,
-                           a.tapk);
-    else
-        hipLaunchKernelGGL((k_bic_rgc2<PPW, MODE, INTW, false>), dim3(a.tiles_x * a.tiles_y),
-                           dim3(64 * a.nw), bic_pair_lds_bytes(a, false), s, a, a.taps,
-                           a.phases, a.tapk);
-    return hipGetLastError();
+    // register staging (the default) needs every tile element within 2 per thread
+    const int nthr = 64 * a.nw, nU = 4 * a.PH * (a.TW >> 1);
+    if (!a.pair_direct && nthr >= RAW_T * RAW_T && nU <= 2 * nthr)
+        return launch_pair_s<PPW, MODE, INTW, 2>(a, s);
+    return launch_pair_s<PPW, MODE, INTW, 0>(a, s);
 }
 
 template <int PPW>

@@ -115,6 +115,22 @@ def test_bic_frame_pair_taps_from_scalar_loads(pkg, monkeypatch, M, w, h, loads)
     assert np.array_equal(res["0"], res["1"])
 
 
+@pytest.mark.parametrize("M,fwhm,w,h,nw", [(5, 1.5, 40, 32, "8"), (4, 2.0, 27, 19, "4"),
+                                            (2, 1.0, 24, 24, "4"), (5, 3.0, 16, 16, "12")])
+def test_bic_frame_pair_register_staging(pkg, monkeypatch, M, fwhm, w, h, nw):
+    """Staging the next frame pair's tiles in registers during the current pair (stored
+    to LDS between the barriers) gives the same bits as filling them after it, odd frame
+    tails and continuation across loads included."""
+    st = stack(7, w, h, seed=140 + M)
+    monkeypatch.setenv("ESRRF_BIC_FP", "2")
+    monkeypatch.setenv("ESRRF_BIC_NW", nw)
+    res = {}
+    for sg in ("0", "1"):
+        monkeypatch.setenv("ESRRF_BIC_PAIR_DIRECT", sg)
+        res[sg], _, _ = run_engine(pkg, st, M, fwhm, 2, nOnGPU=7, loads=[(0, 5), (5, 7)])
+    assert np.array_equal(res["0"], res["1"])
+
+
 @pytest.mark.parametrize("nw", ["7", "9", "13"])
 def test_bic_frame_pair_balanced_wave_counts(pkg, monkeypatch, nw):
     """Blocks of 7, 9 or 13 waves (the balanced shapes for 25 phases) give the same bits as
