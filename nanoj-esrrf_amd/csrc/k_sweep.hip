@@ -12,9 +12,14 @@
 
 namespace esrrf {
 
+template <bool POW>
 __device__ __forceinline__ float sweep_sens(float C, float sens, int sens_int)
 {
     if (!(C >= 0.0f)) return 0.0f;
+    if (!POW) {   // integer sensitivities 1..4 only
+        const float c2 = C * C;
+        return sens_int == 1 ? C : sens_int == 2 ? c2 : sens_int == 3 ? c2 * C : c2 * c2;
+    }
     switch (sens_int) {
     case 1: return C;
     case 2: return C * C;
@@ -24,7 +29,11 @@ __device__ __forceinline__ float sweep_sens(float C, float sens, int sens_int)
     }
 }
 
-// grid.x: pixels in emit order (phase, Y, X); grid.y: sensitivity index
+// grid.x: pixels in emit order (phase, Y, X); grid.y: groups of SG sensitivities.  A thread
+// reads each frame's CGLH and intensity of its pixel once and updates the cumulants of the
+// NQ frame counts of each of its SG sensitivities (the frames are read once per group, not
+// once per sensitivity).
+template <int NQ, int SG, bool POW>
 __global__ void __launch_bounds__(256) k_sweep_acc(const SweepAccArgs a)
 {
     const long wh = (long)a.w * a.h;
@@ -32,9 +41,15 @@ __global__ void __launch_bounds__(256) k_sweep_acc(const SweepAccArgs a)
     const long n = wh * MM;
     const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n) return;
-    const int si = blockIdx.y;
-    const float sens = a.sens[si];
-    const int sint = a.sens_int[si];
+    const int si0 = blockIdx.y * SG;
+    float sens[SG];
+    int sint[SG];
+#pragma unroll
+    for (int g = 0; g < SG; g++) {
+        const int si = min(si0 + g, a.nS - 1);
+        sens[g] = a.sens[si];
+        sint[g] = a.sens_int[si];
+    }
     const int ph = (int)(o / wh);
     const long r = o - (long)ph * wh;
     const int Y = (int)(r / a.w), X = (int)(r - (long)Y * a.w);
@@ -42,51 +57,89 @@ __global__ void __launch_bounds__(256) k_sweep_acc(const SweepAccArgs a)
     const int wM = a.w * a.M;
     const long whM = wh * MM;
     const long pix = (long)(Y * a.M + sy) * wM + (X * a.M + sx);
-    float s0[SWEEP_MAX_NF], s1[SWEEP_MAX_NF], s2[SWEEP_MAX_NF], s3[SWEEP_MAX_NF],
-        s4[SWEEP_MAX_NF];
+    // per (sensitivity, frame count): AVG, E2 and TAC2 sums; the INT sum does not depend on
+    // the sensitivity, and Prev = t/(n-1) is recomputed from the previous frame's t (the
+    // same rounded product the reference keeps)
+    float s0[SG][NQ], s1[SG][NQ], s2[SG][NQ], s3[NQ], tp[SG];
 #pragma unroll
-    for (int q = 0; q < SWEEP_MAX_NF; q++) s0[q] = s1[q] = s2[q] = s3[q] = s4[q] = 0.0f;
+    for (int q = 0; q < NQ; q++) {
+        s3[q] = 0.0f;
+#pragma unroll
+        for (int g = 0; g < SG; g++) s0[g][q] = s1[g][q] = s2[g][q] = 0.0f;
+    }
+#pragma unroll
+    for (int g = 0; g < SG; g++) tp[g] = 0.0f;
     const bool intw = a.int_weighting != 0;
+#pragma unroll 4
     for (int f = 0; f < a.nframes; f++) {
         const float C = a.C[(long)f * n + o];
         const float v = a.V[(long)f * n + o];
-        const float Cs = sweep_sens(C, sens, sint);
-        const float t = intw ? v * Cs : Cs;
-        const float e2 = intw ? (t * v) * Cs : Cs * Cs;
 #pragma unroll
-        for (int q = 0; q < SWEEP_MAX_NF; q++) {
-            if (q < a.nq && f < a.nf[q]) {
-                const float rn = a.rn[q], rn1 = a.rn1[q];
-                if (f == 0) {
-                    s0[q] = t * rn;
-                    s1[q] = e2 * rn;
-                    s2[q] = 0.0f;
-                    s3[q] = v * rn;
-                } else {
-                    s0[q] = s0[q] + t * rn;
-                    s1[q] = s1[q] + e2 * rn;
-                    s2[q] = s2[q] + (t * s4[q]) * rn1;
-                    s3[q] = s3[q] + v * rn;
-                }
-                s4[q] = t * rn1;   // BIC: Prev = t/(n-1) (liveSRRF.cl:374,377)
-                if (f == a.nf[q] - 1) {
-                    float *out = a.out + ((long)(q * a.nS + si) * a.n_fwhm + a.i_fwhm) * 4 * whM;
-                    out[pix] = s0[q];
-                    out[pix + whM] = s1[q];
-                    out[pix + 2 * whM] = s2[q];
-                    out[pix + 3 * whM] = s3[q];
+        for (int q = 0; q < NQ; q++)
+            if (q < a.nq && f < a.nf[q]) s3[q] = f == 0 ? v * a.rn[q] : s3[q] + v * a.rn[q];
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            const float Cs = sweep_sens<POW>(C, sens[g], sint[g]);
+            const float t = intw ? v * Cs : Cs;
+            const float e2 = intw ? (t * v) * Cs : Cs * Cs;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                if (q < a.nq && f < a.nf[q]) {
+                    const float rn = a.rn[q], rn1 = a.rn1[q];
+                    if (f == 0) {
+                        s0[g][q] = t * rn;
+                        s1[g][q] = e2 * rn;
+                        s2[g][q] = 0.0f;
+                    } else {
+                        s0[g][q] = s0[g][q] + t * rn;
+                        s1[g][q] = s1[g][q] + e2 * rn;
+                        // BIC: Prev = t/(n-1) (liveSRRF.cl:374,377)
+                        s2[g][q] = s2[g][q] + (t * (tp[g] * rn1)) * rn1;
+                    }
                 }
             }
+            tp[g] = t;
+        }
+    }
+    // the sums of frame count q stop changing after its last frame nf[q] - 1
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        if (q >= a.nq) continue;
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            if (si0 + g >= a.nS) continue;
+            float *out = a.out + ((long)(q * a.nS + si0 + g) * a.n_fwhm + a.i_fwhm) * 4 * whM;
+            out[pix] = s0[g][q];
+            out[pix + whM] = s1[g][q];
+            out[pix + 2 * whM] = s2[g][q];
+            out[pix + 3 * whM] = s3[q];
         }
     }
 }
 
-hipError_t launch_sweep_acc(const SweepAccArgs &a, hipStream_t s)
+template <int NQ, int SG>
+static hipError_t launch_acc(const SweepAccArgs &a, hipStream_t s)
 {
     const long n = (long)a.w * a.h * a.M * a.M;
-    const dim3 grid((unsigned)((n + 255) / 256), a.nS);
-    hipLaunchKernelGGL(k_sweep_acc, grid, dim3(256), 0, s, a);
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)((a.nS + SG - 1) / SG));
+    bool pow = false;
+    for (int i = 0; i < a.nS; i++) pow = pow || a.sens_int[i] < 1 || a.sens_int[i] > 4;
+    if (pow)
+        hipLaunchKernelGGL((k_sweep_acc<NQ, SG, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_sweep_acc<NQ, SG, false>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+// about 60 cumulant registers per thread: 4 sensitivities for up to 3 frame counts, 2 for 4,
+// 1 for up to SWEEP_MAX_NF
+hipError_t launch_sweep_acc(const SweepAccArgs &a, hipStream_t s)
+{
+    if (a.nq <= 1) return launch_acc<1, 4>(a, s);
+    if (a.nq <= 2) return launch_acc<2, 4>(a, s);
+    if (a.nq <= 3) return launch_acc<3, 4>(a, s);
+    if (a.nq <= 4) return launch_acc<4, 2>(a, s);
+    return launch_acc<SWEEP_MAX_NF, 1>(a, s);
 }
 
 }  // namespace esrrf

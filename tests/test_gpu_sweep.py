@@ -60,6 +60,21 @@ def test_sweep_matches_per_configuration_engine(pkg, swept):
         assert_parity(got, exp, f"M{MAGS[im]} nf{NFS[iq]} S{SENS[i_s]} fwhm{FWHM[i_f]}")
 
 
+@pytest.mark.parametrize("nfs", [[3, 5, 8, 12], [2, 4, 6, 9, 12]])
+def test_sweep_many_frame_counts(pkg, nfs):
+    """Four frame counts (two sensitivities per thread) and five (one per thread, every
+    frame-count slot) accumulate what the per-configuration engine computes."""
+    st = stack(12, 20, 16, seed=66)
+    sens = [1.0, 3.0, 1.5]
+    sw = pkg.ParametersSweep(20, 16, [3], [1.5], sens, nfs, use_graph=False)
+    sw.run(st)
+    for iq, nf in enumerate(nfs):
+        for i_s, S in enumerate(sens):
+            exp = plain(pkg, st, 3, 1.5, S, nf)
+            assert_parity(sw.read(0, iq, i_s, 0), exp, f"nf{nf} S{S}")
+    sw.close()
+
+
 def test_sweep_matches_oracle(pkg, O, swept):
     st, res = swept
     for im, iq, i_s, i_f in [(0, 1, 1, 1), (1, 0, 2, 0)]:
