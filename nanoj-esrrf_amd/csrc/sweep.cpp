@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -427,6 +428,11 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
         return sfail(s, ESRRF_ERR_INVALID, "crop_size %d outside [0, %d)", crop_size,
                      (std::min(s->w, s->h) + 1) / 2);
     SCHK(s, hipSetDevice(s->dev));
+    // the configurations run on SQ_WORKERS host threads, each with its own SQUIRREL plan and
+    // stream: a Brent search is a sequence of short launches and host reads, so searches of
+    // different configurations overlap on the device; each writes only its own results
+    SCHK(s, hipStreamSynchronize(s->stream));   // the sweep's outputs are complete
+    constexpr int SQ_WORKERS = 8;
     const long ncfg = (long)s->nfs.size() * s->sens.size() * s->fwhm.size();
     for (size_t im = 0; im < s->Ms.size(); im++) {
         const int M = s->Ms[im];
@@ -439,27 +445,48 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
         qp.fix_sigma = fix_sigma;
         qp.fixed_sigma = fixed_sigma;
         qp.device = s->dev;
-        esrrf_squirrel *q = nullptr;
-        int rc = esrrf_squirrel_create(&qp, &q);
-        if (rc) return sfail(s, rc, "SQUIRREL plan for M=%d: error %d", M, rc);
-        esrrf_squirrel_set_stream(q, s->stream);   // ordered after the sweep's launches
-        const long whM = (long)s->w * s->h * M * M;
-        for (long c = 0; c < ncfg && rc == ESRRF_OK; c++) {
-            // configuration c of [nf][S][fwhm]: SR = its channel, ref = its INT (:473)
-            const float *base = s->d_out[im] + c * 4 * whM;
-            esrrf_squirrel_result r{};
-            rc = esrrf_squirrel_run_device(q, base + 3 * whM, base + channel * whM, &r, nullptr,
-                                           nullptr);
-            if (rc) {
-                s->err = std::string("SQUIRREL: ") + esrrf_squirrel_last_error(q);
-                break;
-            }
-            const long o = (long)im * ncfg + c;
-            if (rse) rse[o] = r.rse;
-            if (rsp) rsp[o] = r.rsp;
-            if (sigma) sigma[o] = r.sigma;
+        const int nw = (int)std::min<long>(SQ_WORKERS, ncfg);
+        std::vector<esrrf_squirrel *> qs(nw, nullptr);
+        int rc = ESRRF_OK;
+        for (int t = 0; t < nw && rc == ESRRF_OK; t++) {
+            rc = esrrf_squirrel_create(&qp, &qs[t]);
+            if (rc) s->err = "SQUIRREL plan for M=" + std::to_string(M) + " failed";
         }
-        esrrf_squirrel_destroy(q);
+        const long whM = (long)s->w * s->h * M * M;
+        std::vector<int> rcs(nw, ESRRF_OK);
+        std::vector<std::string> errs(nw);
+        auto work = [&](int t) {
+            for (long c = t; c < ncfg; c += nw) {
+                // configuration c of [nf][S][fwhm]: SR = its channel, ref = its INT (:473)
+                const float *base = s->d_out[im] + c * 4 * whM;
+                esrrf_squirrel_result r{};
+                const int e = esrrf_squirrel_run_device(qs[t], base + 3 * whM,
+                                                        base + channel * whM, &r, nullptr,
+                                                        nullptr);
+                if (e) {
+                    rcs[t] = e;
+                    errs[t] = std::string("SQUIRREL: ") + esrrf_squirrel_last_error(qs[t]);
+                    return;
+                }
+                const long o = (long)im * ncfg + c;
+                if (rse) rse[o] = r.rse;
+                if (rsp) rsp[o] = r.rsp;
+                if (sigma) sigma[o] = r.sigma;
+            }
+        };
+        if (rc == ESRRF_OK) {
+            std::vector<std::thread> th;
+            for (int t = 1; t < nw; t++) th.emplace_back(work, t);
+            work(0);
+            for (auto &x : th) x.join();
+            for (int t = 0; t < nw && rc == ESRRF_OK; t++)
+                if (rcs[t]) {
+                    rc = rcs[t];
+                    s->err = errs[t];
+                }
+        }
+        for (esrrf_squirrel *q : qs)
+            if (q) esrrf_squirrel_destroy(q);
         if (rc) return rc;
     }
     return ESRRF_OK;
