@@ -188,7 +188,8 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
 }
 
 // Gated sample count of one whole frame (border gates included), geometry only: the
-// algorithmic work unit of the RGC (14 FLOP per sample by the survey's convention).
+// algorithmic work unit of the RGC (14 FLOP per sample: G.u 3, G x u 3, |G|^2 3, rsqrt 1,
+// |c| r 1, 1 - x 1, weighted accumulate 2).
 long long count_active_samples_bic(int w, int h, int M, float radius, float tSO)
 {
     const int R2lo = -(int)(2.0f * radius), R2hi = (int)(2.0f * radius + 1.0f);

@@ -1,7 +1,7 @@
 /*
  * esrrf_oracle.c -- CPU restatement of the eSRRF reconstruction path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP engine in
+ * TEST INFRASTRUCTURE ONLY.  This file is the CPU reference that the HIP engine in
  * nanoj-esrrf_amd/ is compared against.  Only tests/, __graft_entry__.smoke()
  * and the cpu_baseline leg of bench.py may load it.  The product never links it.
  *

@@ -3,7 +3,7 @@
 The parameter sweep scores configurations by FRC.calculateFireNumber(odd, even,
 FIXED_1_OVER_7) (gui/ParametersSweep_.java:437-446), a class of NanoJ-Core that is not in the
 reference tree, so its windowing, binning and smoothing are not available (parity
-unpinned).  This build uses the standard definition (Banterle et al., J. Struct. Biol. 2013;
+unpinned).  This restatement uses the standard definition (Banterle et al., J. Struct. Biol. 2013;
 Nieuwenhuizen et al., Nat. Methods 2013):
 
 * crop both images to the centred S x S square, S = min(width, height);

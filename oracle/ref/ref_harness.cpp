@@ -3,7 +3,7 @@
 // the AMD OpenCL runtime, with the buffer plan, argument binding and enqueue order of
 // LiveSRRF_CL (LiveSRRF_CL.java:266-459 BIC/3-D, :462-579 FHT, :636-794).
 //
-// TEST INFRASTRUCTURE ONLY: the checker of the oracle and of the HIP engine.  Nothing in
+// TEST INFRASTRUCTURE ONLY: what the oracle and the HIP engine are compared against.  Nothing in
 // the product links it.  Each .co is specialised for one configuration (the reference
 // bakes every constant into the program text, LiveSRRF_CL.java:307-342).
 #define CL_TARGET_OPENCL_VERSION 120

@@ -3,7 +3,7 @@ kernels (resources/liveSRRF.cl, liveSRRF_noInterpolation.cl, live3DSRRF.cl, buil
 sources into oracle/_ref/ by oracle/ref/build_ref.py) for small seeded inputs, run through
 the AMD OpenCL runtime with LiveSRRF_CL's enqueue order (oracle/refrun.py).
 
-Needs a GPU and a built oracle/_ref/ (run it on the GPU box):
+Needs an MI355X and a built oracle/_ref/:
 
     python tests/golden/make_ref_vectors.py
 
