@@ -475,9 +475,16 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
             }
         };
         if (rc == ESRRF_OK) {
+            // a worker the system cannot start is run on this thread instead (no exception
+            // leaves the C ABI)
             std::vector<std::thread> th;
-            for (int t = 1; t < nw; t++) th.emplace_back(work, t);
+            int started = 1;
+            try {
+                for (; started < nw; started++) th.emplace_back(work, started);
+            } catch (...) {
+            }
             work(0);
+            for (int t = started; t < nw; t++) work(t);
             for (auto &x : th) x.join();
             for (int t = 0; t < nw && rc == ESRRF_OK; t++)
                 if (rcs[t]) {
