@@ -240,6 +240,34 @@ def bench_rolling(pkg, torch, stack):
                         f"image every {gap} frames; VAR/TAC2/MP as the headline"}
 
 
+def bench_drift(pkg, torch, stack):
+    """Vibration-correction drift (XYShiftCalculator.calculateShiftArray) of the production
+    stack's 1000 frames against its first, frames resident in HBM as float32."""
+    import numpy as np
+    C = pkg._capi
+    n, h, w = stack.shape
+    d = torch.from_numpy(stack.astype(np.float32)).to(torch.device("cuda:0"))
+    L = C.lib()
+    hd = ctypes.c_void_p()
+    C.check(L.esrrf_drift_create(w, h, 0, ctypes.byref(hd)))
+    sx = np.empty(n, np.float32)
+    sy = np.empty(n, np.float32)
+
+    def once():
+        C.check(L.esrrf_drift_run_device(hd, ctypes.c_void_p(d.data_ptr()), n, 0, C.f32ptr(sx),
+                                         C.f32ptr(sy)))
+
+    once()
+    t0 = time.perf_counter()
+    once()
+    s = time.perf_counter() - t0
+    L.esrrf_drift_destroy(hd)
+    del d
+    return {"frames": n, "s": round(s, 4), "frames_per_s": round(n / s, 1),
+            "max_abs_shift_px": round(float(max(np.abs(sx).max(), np.abs(sy).max())), 3),
+            "workload": f"{n}x{h}x{w} float32 in HBM against frame 0; 256x256 transforms"}
+
+
 def bench_rse(pkg, stack):
     """30 adjoint descent steps vs a 25x25 grid (sigma by Brent per point), same box."""
     import numpy as np
@@ -478,6 +506,7 @@ def main():
         res["cpu_baseline"] = cpu_baseline(stack, min(a.cpu_sample_frames, nf))
     if not a.no_sweep and not fht:
         res["rolling"] = bench_rolling(pkg, torch, stack)
+        res["drift"] = bench_drift(pkg, torch, stack)
         res["sweep"] = bench_sweep(pkg, torch, synth.sweep_256x256x500())
     if not a.no_rse and not fht:
         res["rse_search"] = bench_rse(pkg, synth.rse_128x128x500())

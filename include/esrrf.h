@@ -308,6 +308,30 @@ const char *esrrf_frc_last_error(esrrf_frc *q);
  * AVG, 1 VAR or 2 TAC2: [n_magnifications][n_n_frames][n_sensitivity][n_fwhm]. */
 int esrrf_sweep_frc(esrrf_sweep *odd, esrrf_sweep *even, int channel, double *resolution);
 
+/* ---- Drift of a window's frames (XYShiftCalculator.calculateShiftArray,
+ * XYShiftCalculator.java:25-47, whose shiftX / shiftY feed loadDriftXYGPUbuffer,
+ * gui/LiveSRRF_optimised_.java:284-305) ------------------------------------------------------
+ * The reference reads the shift from NanoJ-Core's cross-correlation map and peak fit, which
+ * the reference does not contain; this is the standard method: both frames minus their means,
+ * zero-padded to the power of two N >= max(width, height) (N <= 4096), circular
+ * cross-correlation CC(t) = sum_x r(x) f(x + t) by FFT, the first maximum (row-major), lags
+ * of N/2 and above wrapped to negative, and per axis the parabola through the peak and its
+ * two neighbours.  shift = the displacement of a frame's content relative to the reference
+ * (f(x) = r(x - d) gives d), in raw pixels, the sign loadDriftXYGPUbuffer takes.  NaN pixels
+ * read as 0. */
+typedef struct esrrf_drift esrrf_drift;
+int esrrf_drift_create(int width, int height, int device, esrrf_drift **out);
+int esrrf_drift_destroy(esrrf_drift *q);
+int esrrf_drift_set_stream(esrrf_drift *q, void *hip_stream);
+/* Shifts of n_frames [height][width] device frames against frame ref_index (the reference
+ * uses the window's first frame, :27); shift_x, shift_y (host, n_frames each; either may be
+ * NULL).  Blocking. */
+int esrrf_drift_run_device(esrrf_drift *q, const float *d_frames, int n_frames, int ref_index,
+                           float *shift_x, float *shift_y);
+int esrrf_drift_run(esrrf_drift *q, const float *frames, int n_frames, int ref_index,
+                    float *shift_x, float *shift_y);
+const char *esrrf_drift_last_error(esrrf_drift *q);
+
 /* ---- Rolling analysis of a long stack (gui/LiveSRRF_optimised_.java:276-384) ------------
  * SR image r (0-based) covers frames [r*gap, r*gap + nF) of an n_slices stack, with nF, gap
  * and the number of SR images from the plugin's plan (:621-627, esrrf_plan_rolling with

@@ -10,9 +10,11 @@ The package directory is named ``nanoj-esrrf_amd``; load it with :func:`load` fr
 * ``rse``              -- SQUIRREL RSE of the AVG image and its gradient (ErrorMapLiveSRRF)
 * ``squirrel``         -- SQUIRREL error map / RSE / RSP of any (ref, SR) pair, QnR map
 * ``rolling``          -- rolling analysis of long stacks (gui/LiveSRRF_optimised_.java)
+* ``drift``            -- drift of SR windows for the vibration correction (XYShiftCalculator)
 * ``synth``            -- seeded synthetic blinking stacks for the workloads
 """
 from . import _capi  # noqa: F401
+from .drift import XYShiftCalculator  # noqa: F401
 from .liveSRRF_CL import Device, Engine, LiveSRRF_CL  # noqa: F401
 from .rolling import LiveSRRF_optimised, Rolling  # noqa: F401
 from .rse import RSE  # noqa: F401

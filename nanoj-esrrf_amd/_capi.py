@@ -43,6 +43,8 @@ EXPORTS = [
     "esrrf_rolling_next_device", "esrrf_rolling_last_error",
     "esrrf_frc_create", "esrrf_frc_destroy", "esrrf_frc_set_stream", "esrrf_frc_rings",
     "esrrf_frc_run_device", "esrrf_frc_run", "esrrf_frc_last_error", "esrrf_sweep_frc",
+    "esrrf_drift_create", "esrrf_drift_destroy", "esrrf_drift_set_stream",
+    "esrrf_drift_run_device", "esrrf_drift_run", "esrrf_drift_last_error",
 ]
 
 
@@ -206,11 +208,17 @@ def lib():
         "esrrf_frc_run_device": [vp, vp, vp, dp, fp],
         "esrrf_frc_run": [vp, fp, fp, dp, fp],
         "esrrf_frc_last_error": [vp],
+        "esrrf_drift_create": [c_int, c_int, c_int, ctypes.POINTER(vp)],
+        "esrrf_drift_destroy": [vp],
+        "esrrf_drift_set_stream": [vp, vp],
+        "esrrf_drift_run_device": [vp, vp, c_int, c_int, fp, fp],
+        "esrrf_drift_run": [vp, fp, c_int, c_int, fp, fp],
+        "esrrf_drift_last_error": [vp],
         "esrrf_sweep_frc": [vp, vp, c_int, dp],
     }
     strings = ("esrrf_last_error", "esrrf_version", "esrrf_sweep_last_error",
                "esrrf_rse_last_error", "esrrf_squirrel_last_error",
-               "esrrf_rolling_last_error", "esrrf_frc_last_error")
+               "esrrf_rolling_last_error", "esrrf_frc_last_error", "esrrf_drift_last_error")
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args

@@ -20,4 +20,7 @@ struct FrcPlan {
 
 hipError_t launch_frc(const FrcPlan &p, const float *a, const float *b, hipStream_t s);
 
+// Forward 2-D FFT, in place, of `batch` N x N complex images at p.buf (p.N, p.log2N, p.tw).
+hipError_t launch_fft2d(const FrcPlan &p, int batch, hipStream_t s);
+
 }  // namespace esrrf

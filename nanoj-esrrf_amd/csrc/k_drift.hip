@@ -1,0 +1,153 @@
+// k_drift.hip -- drift of frames against a reference frame (XYShiftCalculator.java:25-47):
+// mean-free frames zero-padded to a power-of-two square, circular cross-correlation by FFT,
+// integer peak and a three-point parabola per axis.
+#include <algorithm>
+
+#include "drift_internal.h"
+#include "frc_internal.h"
+
+namespace esrrf {
+
+// per frame of the batch: sum of its pixels (NaN as 0), in double; one block per frame
+__global__ void __launch_bounds__(256) k_drift_sums(const float *__restrict__ frames, int w,
+                                                     int h, double *__restrict__ sums)
+{
+    __shared__ double red[256];
+    const long wh = (long)w * h;
+    const float *f = frames + wh * blockIdx.x;
+    double acc = 0.0;
+    for (long e = threadIdx.x; e < wh; e += blockDim.x) {
+        const float v = f[e];
+        acc += v == v ? (double)v : 0.0;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+// frame b of the batch minus its mean into the top-left of an N x N complex image
+__global__ void k_drift_prep(const float *__restrict__ frames, const double *__restrict__ sums,
+                             int w, int h, int N, int batch, float2 *__restrict__ buf)
+{
+    const long NN = (long)N * N, wh = (long)w * h;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < batch * NN;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(e / NN);
+        const long r = e - b * NN;
+        const int y = (int)(r / N), x = (int)(r - (long)y * N);
+        float v = 0.0f;
+        if (x < w && y < h) {
+            const float s = frames[wh * b + (long)y * w + x];
+            v = (float)((double)(s == s ? s : 0.0f) - sums[b] / (double)wh);
+        }
+        buf[e] = make_float2(v, 0.0f);
+    }
+}
+
+// R(k) conj(F_b(k)): the conjugate of conj(R) F_b, whose inverse transform is the
+// correlation; a forward transform of this product has the correlation (times N^2) as its
+// real part
+__global__ void k_drift_xspec(float2 *__restrict__ buf, const float2 *__restrict__ R, int N,
+                              int batch)
+{
+    const long NN = (long)N * N;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < batch * NN;
+         e += (long)gridDim.x * blockDim.x) {
+        const float2 r = R[e % NN], f = buf[e];
+        buf[e] = make_float2(r.x * f.x + r.y * f.y, r.y * f.x - r.x * f.y);
+    }
+}
+
+// per frame: the first maximum of the correlation (row-major), its lags wrapped to
+// [-N/2, N/2) and the parabola through the peak and its neighbours per axis
+__global__ void __launch_bounds__(256) k_drift_peak(const float2 *__restrict__ buf, int N,
+                                                     float *__restrict__ sx,
+                                                     float *__restrict__ sy)
+{
+    __shared__ float bv[256];
+    __shared__ int bi[256];
+    const long NN = (long)N * N;
+    const float2 *c = buf + NN * blockIdx.x;
+    float best = -3.4028235e38f;
+    int bidx = 0x7fffffff;
+    for (int e = threadIdx.x; e < (int)NN; e += blockDim.x) {
+        const float v = c[e].x;
+        if (v > best) {   // ascending e per thread: keeps the thread's first maximum
+            best = v;
+            bidx = e;
+        }
+    }
+    bv[threadIdx.x] = best;
+    bi[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const float v = bv[threadIdx.x + s];
+            const int i = bi[threadIdx.x + s];
+            if (v > bv[threadIdx.x] || (v == bv[threadIdx.x] && i < bi[threadIdx.x])) {
+                bv[threadIdx.x] = v;
+                bi[threadIdx.x] = i;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int py = bi[0] / N, px = bi[0] - py * N;
+        const float c0 = bv[0];
+        const float xm = c[py * N + ((px - 1) & (N - 1))].x, xp = c[py * N + ((px + 1) & (N - 1))].x;
+        const float ym = c[((py - 1) & (N - 1)) * N + px].x, yp = c[((py + 1) & (N - 1)) * N + px].x;
+        const float dx_den = xm - 2.0f * c0 + xp, dy_den = ym - 2.0f * c0 + yp;
+        const float fx = dx_den != 0.0f ? 0.5f * (xm - xp) / dx_den : 0.0f;
+        const float fy = dy_den != 0.0f ? 0.5f * (ym - yp) / dy_den : 0.0f;
+        const int lx = px < N / 2 ? px : px - N, ly = py < N / 2 ? py : py - N;
+        sx[blockIdx.x] = (float)lx + fx;
+        sy[blockIdx.x] = (float)ly + fy;
+    }
+}
+
+static unsigned grid_for(long n)
+{
+    return (unsigned)std::min<long>((n + 255) / 256, 8192);
+}
+
+hipError_t launch_drift_reference(const DriftPlan &d, const float *ref, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_drift_sums, dim3(1), dim3(256), 0, s, ref, d.w, d.h, d.sums);
+    hipLaunchKernelGGL(k_drift_prep, dim3(grid_for((long)d.N * d.N)), dim3(256), 0, s, ref,
+                       d.sums, d.w, d.h, d.N, 1, d.R);
+    FrcPlan p{};
+    p.N = d.N;
+    p.log2N = d.log2N;
+    p.tw = d.tw;
+    p.buf = d.R;
+    return launch_fft2d(p, 1, s);
+}
+
+hipError_t launch_drift_batch(const DriftPlan &d, const float *frames, int batch, float *sx,
+                              float *sy, hipStream_t s)
+{
+    if (batch < 1 || batch > d.max_batch) return hipErrorInvalidValue;
+    const long n = (long)batch * d.N * d.N;
+    hipLaunchKernelGGL(k_drift_sums, dim3(batch), dim3(256), 0, s, frames, d.w, d.h, d.sums);
+    hipLaunchKernelGGL(k_drift_prep, dim3(grid_for(n)), dim3(256), 0, s, frames, d.sums, d.w,
+                       d.h, d.N, batch, d.buf);
+    FrcPlan p{};
+    p.N = d.N;
+    p.log2N = d.log2N;
+    p.tw = d.tw;
+    p.buf = d.buf;
+    hipError_t e = launch_fft2d(p, batch, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_drift_xspec, dim3(grid_for(n)), dim3(256), 0, s, d.buf, d.R, d.N,
+                       batch);
+    e = launch_fft2d(p, batch, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_drift_peak, dim3(batch), dim3(256), 0, s, d.buf, d.N, sx, sy);
+    return hipGetLastError();
+}
+
+}  // namespace esrrf

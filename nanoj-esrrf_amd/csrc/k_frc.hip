@@ -108,11 +108,29 @@ __global__ void k_frc_reduce(FrcPlan p)
 }
 
 template <int G>
-static void frc_fft(const FrcPlan &p, hipStream_t s)
+static void frc_fft(const FrcPlan &p, int batch, hipStream_t s)
 {
     const size_t lds = sizeof(float2) * ((size_t)2 * G * p.N + p.N);
-    hipLaunchKernelGGL((k_frc_rows<G>), dim3(p.N / G, 2), dim3(256), lds, s, p);
-    hipLaunchKernelGGL((k_frc_cols<G>), dim3(p.N / G, 2), dim3(256), lds, s, p);
+    hipLaunchKernelGGL((k_frc_rows<G>), dim3(p.N / G, batch), dim3(256), lds, s, p);
+    hipLaunchKernelGGL((k_frc_cols<G>), dim3(p.N / G, batch), dim3(256), lds, s, p);
+}
+
+static void fft_batch(const FrcPlan &p, int batch, hipStream_t s)
+{
+    switch (std::max(1, std::min(std::min(16, p.N), 1024 / p.N))) {
+    case 16: frc_fft<16>(p, batch, s); break;
+    case 8: frc_fft<8>(p, batch, s); break;
+    case 4: frc_fft<4>(p, batch, s); break;
+    case 2: frc_fft<2>(p, batch, s); break;
+    default: frc_fft<1>(p, batch, s); break;
+    }
+}
+
+hipError_t launch_fft2d(const FrcPlan &p, int batch, hipStream_t s)
+{
+    if (p.N < 4 || p.N > FRC_MAX_N || batch < 1 || batch > 65535) return hipErrorInvalidValue;
+    fft_batch(p, batch, s);
+    return hipGetLastError();
 }
 
 hipError_t launch_frc(const FrcPlan &p, const float *a, const float *b, hipStream_t s)
@@ -121,13 +139,7 @@ hipError_t launch_frc(const FrcPlan &p, const float *a, const float *b, hipStrea
     const long n2 = 2L * p.N * p.N;
     hipLaunchKernelGGL(k_frc_prep, dim3((unsigned)std::min<long>((n2 + 255) / 256, 8192)),
                        dim3(256), 0, s, a, b, p);
-    switch (std::max(1, std::min(std::min(16, p.N), 1024 / p.N))) {
-    case 16: frc_fft<16>(p, s); break;
-    case 8: frc_fft<8>(p, s); break;
-    case 4: frc_fft<4>(p, s); break;
-    case 2: frc_fft<2>(p, s); break;
-    default: frc_fft<1>(p, s); break;
-    }
+    fft_batch(p, 2, s);
     hipLaunchKernelGGL(k_frc_rings, dim3(p.nparts), dim3(256),
                        sizeof(double) * 3 * (size_t)(p.N / 2), s, p);
     hipLaunchKernelGGL(k_frc_reduce, dim3((3 * (p.N / 2) + 255) / 256), dim3(256), 0, s, p);

@@ -18,6 +18,7 @@ import zipfile
 import numpy as np
 
 from . import _capi as C
+from .drift import XYShiftCalculator
 from .liveSRRF_CL import LiveSRRF_CL
 
 CHANNELS = ("AVG", "VAR", "TAC2", "INT")
@@ -112,7 +113,7 @@ class LiveSRRF_optimised:
                  doRollingAnalysis=False, frameGap=0, intWeighting=True,
                  doMPmapCorrection=True, calculateAVG=True, calculateVAR=False,
                  calculateTAC2=False, getInterpolatedImage=False, doFHTinterpolation=False,
-                 chunk_frames=256):
+                 chunk_frames=256, correctVibration=False):
         self.magnification = int(magnification)
         self.fwhm, self.sensitivity = float(fwhm), float(sensitivity)
         self.nFrameForSRRF = int(nFrameForSRRF)
@@ -121,6 +122,7 @@ class LiveSRRF_optimised:
         self.keep = [calculateAVG, calculateVAR, calculateTAC2, getInterpolatedImage]
         self.doFHTinterpolation = bool(doFHTinterpolation)
         self.chunk_frames = int(chunk_frames)
+        self.correctVibration = bool(correctVibration)
         self.last_path = None   # "fused" or "engine"
 
     def plan(self, n_slices):
@@ -133,6 +135,10 @@ class LiveSRRF_optimised:
         n_slices, h, w = st.shape
         plan = self.plan(n_slices)
         nF, gap, nSR = plan["nFrameForSRRF"], plan["gap"], plan["nSRRFframe"]
+        if self.correctVibration and drift is None:
+            # per SR frame, the shifts of its window against its first frame (:284-287)
+            calc = XYShiftCalculator(st)
+            drift = lambda r: calc.calculateShiftArray((r - 1) * gap + 1, nF)  # noqa: E731
         if not self.doFHTinterpolation and drift is None:
             try:
                 r = Rolling(w, h, self.magnification, self.fwhm, self.sensitivity, nF, gap,
