@@ -282,6 +282,13 @@ int esrrf_sweep_squirrel(esrrf_sweep *s, int channel, int crop_size, int fix_sig
  * its index (first maximum) go to *best and *best_index (each may be NULL). */
 int esrrf_qnr_map(const float *frc, const float *rsp, long n, int auto_min_max, float res_min,
                   float res_max, float *qnr, float *best, long *best_index);
+/* One pass of ImageJ's "Remove NaNs..." on a [height][width] map, in place, as GetQnRmap_
+ * runs it on the RSP and FRC maps (gui/GetQnRmap_.java:69-77, radius 2): every NaN pixel
+ * becomes the median of the non-NaN pixels of the circular kernel dx^2 + dy^2 <= r^2 + 1
+ * (edge pixels extend the map; values read before the pass); a pixel with no such
+ * neighbour stays NaN.  ImageJ is not available: the kernel shape and the even-count median
+ * (the mean of the two middle values) are the build's, parity unpinned.  Host code. */
+int esrrf_remove_nans(float *map, int width, int height, double radius);
 
 /* ---- Fourier ring correlation (the sweep's FRC resolution, gui/ParametersSweep_.java:403-447)
  * FRC.calculateFireNumber(ip1, ip2, FIXED_1_OVER_7) belongs to NanoJ-Core, which the reference

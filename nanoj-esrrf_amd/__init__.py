@@ -18,5 +18,5 @@ from .drift import XYShiftCalculator  # noqa: F401
 from .liveSRRF_CL import Device, Engine, LiveSRRF_CL  # noqa: F401
 from .rolling import LiveSRRF_optimised, Rolling  # noqa: F401
 from .rse import RSE  # noqa: F401
-from .squirrel import FRC, ErrorMapLiveSRRF, qnr_map  # noqa: F401
+from .squirrel import FRC, ErrorMapLiveSRRF, qnr_map, remove_nans  # noqa: F401
 from .sweep import ParametersSweep, odd_even, sweep_axes  # noqa: F401

@@ -37,7 +37,7 @@ EXPORTS = [
     "esrrf_rse_best_sigma", "esrrf_rse_descend", "esrrf_rse_counters", "esrrf_rse_last_error",
     "esrrf_squirrel_create", "esrrf_squirrel_destroy", "esrrf_squirrel_set_stream",
     "esrrf_squirrel_dims", "esrrf_squirrel_run_device", "esrrf_squirrel_run",
-    "esrrf_squirrel_last_error", "esrrf_sweep_squirrel", "esrrf_qnr_map",
+    "esrrf_squirrel_last_error", "esrrf_sweep_squirrel", "esrrf_qnr_map", "esrrf_remove_nans",
     "esrrf_rolling_create", "esrrf_rolling_destroy", "esrrf_rolling_set_stream",
     "esrrf_rolling_begin", "esrrf_rolling_begin_device", "esrrf_rolling_next",
     "esrrf_rolling_next_device", "esrrf_rolling_last_error",
@@ -191,6 +191,7 @@ def lib():
         "esrrf_squirrel_run": [vp, fp, fp, ctypes.POINTER(SquirrelResult), fp, fp],
         "esrrf_squirrel_last_error": [vp],
         "esrrf_sweep_squirrel": [vp, c_int, c_int, c_int, c_float, dp, dp, dp],
+        "esrrf_remove_nans": [fp, c_int, c_int, ctypes.c_double],
         "esrrf_qnr_map": [fp, fp, ctypes.c_long, c_int, c_float, c_float, fp, fp,
                           ctypes.POINTER(ctypes.c_long)],
         "esrrf_rolling_create": [ctypes.POINTER(RollingParams), ctypes.POINTER(vp)],

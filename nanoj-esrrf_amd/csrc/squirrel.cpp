@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/esrrf.h"
 #include "rse_internal.h"
@@ -231,6 +232,39 @@ int esrrf_squirrel_run(esrrf_squirrel *q, const float *ref, const float *sr,
 }
 
 const char *esrrf_squirrel_last_error(esrrf_squirrel *q) { return q ? q->err.c_str() : ""; }
+
+int esrrf_remove_nans(float *map, int width, int height, double radius)
+{
+    if (!map || width < 1 || height < 1 || !(radius >= 0.0) || radius > 64.0)
+        return ESRRF_ERR_INVALID;
+    // kernel lines: for dy in [-kr, kr], dx in [-dxmax(dy), dxmax(dy)] with
+    // dx^2 + dy^2 <= r^2 + 1
+    const double r2 = radius * radius + 1.0;
+    const int kr = (int)std::sqrt(r2 + 1e-10);
+    std::vector<int> dxmax(2 * kr + 1);
+    for (int dy = -kr; dy <= kr; dy++) dxmax[dy + kr] = (int)std::sqrt(r2 - dy * dy + 1e-10);
+    const std::vector<float> src(map, map + (size_t)width * height);
+    std::vector<float> vals;
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            if (!std::isnan(src[(size_t)y * width + x])) continue;
+            vals.clear();
+            for (int dy = -kr; dy <= kr; dy++) {
+                const int yy = std::min(std::max(y + dy, 0), height - 1);
+                for (int dx = -dxmax[dy + kr]; dx <= dxmax[dy + kr]; dx++) {
+                    const int xx = std::min(std::max(x + dx, 0), width - 1);
+                    const float v = src[(size_t)yy * width + xx];
+                    if (!std::isnan(v)) vals.push_back(v);
+                }
+            }
+            if (vals.empty()) continue;
+            std::sort(vals.begin(), vals.end());
+            const size_t n = vals.size();
+            map[(size_t)y * width + x] =
+                n & 1 ? vals[n / 2] : (float)(0.5 * ((double)vals[n / 2 - 1] + vals[n / 2]));
+        }
+    return ESRRF_OK;
+}
 
 int esrrf_qnr_map(const float *frc, const float *rsp, long n, int auto_min_max, float res_min,
                   float res_max, float *qnr, float *best, long *best_index)

@@ -80,14 +80,31 @@ class ErrorMapLiveSRRF:
             pass
 
 
-def qnr_map(frc_map, rsp_map, auto_min_max=True, res_min=0.05, res_max=0.2):
+def remove_nans(img, radius=2.0):
+    """One pass of ImageJ's "Remove NaNs..." (see esrrf_remove_nans) on a copy of a 2-D map."""
+    a = np.array(img, dtype=np.float32, order="C", copy=True)
+    if a.ndim != 2:
+        raise ValueError("map must be 2-D")
+    C.check(C.lib().esrrf_remove_nans(C.f32ptr(a), a.shape[1], a.shape[0], float(radius)))
+    return a
+
+
+def qnr_map(frc_map, rsp_map, auto_min_max=True, res_min=0.05, res_max=0.2, removeNaNs=False):
     """GetQnRmap_ (gui/GetQnRmap_.java:179-258): logistic conversion of the FRC resolution
     map, QnR = 2 L RSP / (L + RSP), and its first maximum.  Returns (qnr, best, (x, y))
-    with x the fwhm index and y the sensitivity index of a [n_sensitivity][n_fwhm] map."""
+    with x the fwhm index and y the sensitivity index of a [n_sensitivity][n_fwhm] map.
+    removeNaNs runs "Remove NaNs..." (radius 2) as the plugin does (:69-77): three passes on
+    the RSP map, two on the FRC map, then a fourth on the RSP map."""
     frc = np.ascontiguousarray(frc_map, dtype=np.float32)
     rsp = np.ascontiguousarray(rsp_map, dtype=np.float32)
     if frc.shape != rsp.shape:
         raise ValueError("FRC and RSP maps must have the same shape")
+    if removeNaNs:
+        for _ in range(3):
+            rsp = remove_nans(rsp, 2.0)
+        for _ in range(2):
+            frc = remove_nans(frc, 2.0)
+        rsp = remove_nans(rsp, 2.0)
     q = np.empty_like(frc)
     best = ctypes.c_float()
     idx = ctypes.c_long()

@@ -213,3 +213,33 @@ def qnr_map(frc, rsp, auto_min_max=True, res_min=0.05, res_max=0.2):
         if v > best:
             best, idx = v, i
     return q, float(best), idx
+
+
+def remove_nans(img, radius=2.0):
+    """One pass of ImageJ's "Remove NaNs..." as GetQnRmap_ runs it (gui/GetQnRmap_.java:69-77):
+    each NaN pixel becomes the median of the non-NaN pixels of the circular kernel
+    dx^2 + dy^2 <= r^2 + 1, edge pixels extending the map, reading the map before the pass.
+    ImageJ (RankFilters) is not available: the kernel shape and the even-count median (mean
+    of the two middle values) are parity unpinned."""
+    a = np.asarray(img, np.float32)
+    h, w = a.shape
+    r2 = radius * radius + 1.0
+    kr = int(math.sqrt(r2 + 1e-10))
+    out = a.copy()
+    for y in range(h):
+        for x in range(w):
+            if not np.isnan(a[y, x]):
+                continue
+            vals = []
+            for dy in range(-kr, kr + 1):
+                dxm = int(math.sqrt(r2 - dy * dy + 1e-10))
+                yy = min(max(y + dy, 0), h - 1)
+                for dx in range(-dxm, dxm + 1):
+                    v = a[yy, min(max(x + dx, 0), w - 1)]
+                    if not np.isnan(v):
+                        vals.append(float(v))
+            if vals:
+                vals.sort()
+                n = len(vals)
+                out[y, x] = vals[n // 2] if n % 2 else np.float32(0.5 * (vals[n // 2 - 1] + vals[n // 2]))
+    return out

@@ -186,3 +186,27 @@ def test_sweep_frc_maps_match_per_configuration(pkg):
                             math.isclose(got, eres, rel_tol=1e-3), (name, im, inf, got, eres)
     for s in sw:
         s.close()
+
+
+def test_remove_nans_matches_oracle(pkg):
+    """The library's Remove NaNs pass equals the oracle's on maps with scattered NaNs and
+    NaN blocks (odd and even neighbour counts, edges), and qnr_map with removeNaNs runs the
+    plugin's passes (RSP three times, FRC twice, RSP once more)."""
+    rng = np.random.default_rng(12)
+    for shape in [(5, 4), (9, 20), (16, 16)]:
+        m = rng.random(shape).astype(np.float32)
+        m[rng.random(shape) < 0.3] = np.nan
+        m[1:4, 1:3] = np.nan
+        np.testing.assert_array_equal(pkg.remove_nans(m, 2.0), R.remove_nans(m, 2.0))
+    frc = (0.05 + 0.2 * rng.random((4, 5))).astype(np.float32)
+    rsp = rng.random((4, 5)).astype(np.float32)
+    frc[1, 2] = np.nan
+    rsp[2, 3] = rsp[0, 0] = np.nan
+    r = rsp
+    for _ in range(4):
+        r = R.remove_nans(r, 2.0)
+    f = R.remove_nans(R.remove_nans(frc, 2.0), 2.0)
+    q, best, idx = pkg.qnr_map(frc, rsp, removeNaNs=True)
+    q2, best2, idx2 = pkg.qnr_map(f, r)
+    np.testing.assert_array_equal(q, q2)
+    assert best == best2 and idx == idx2

@@ -140,3 +140,28 @@ def test_qnr_logistic_end_points_and_harmonic_mean():
     # L(midpoint) = 0.5 by the symmetry of the two defining points
     assert abs(q[2] - 2 * 0.5 * 0.9 / 1.4) < 1e-6
     assert np.isnan(q[3]) and idx == 0 and best == q[0]
+
+
+def test_remove_nans_closed_forms():
+    """Remove NaNs (radius 2: the 21-pixel disc): a NaN in a linear ramp takes the ramp's
+    value (its 20 neighbours pair off symmetrically and the two middle ones equal it), a
+    constant map stays constant, a NaN block wider than the kernel keeps its centre NaN
+    after one pass, and edge pixels extend the map."""
+    y, x = np.mgrid[0:9, 0:11]
+    ramp = (3.0 * x - 2.0 * y + 5.0).astype(np.float32)
+    r = ramp.copy()
+    r[4, 5] = np.nan
+    np.testing.assert_array_equal(R.remove_nans(r, 2.0), ramp)
+    c = np.full((7, 7), 2.5, np.float32)
+    c[0, 0] = c[3, 3] = c[6, 2] = np.nan
+    np.testing.assert_array_equal(R.remove_nans(c, 2.0), np.full((7, 7), 2.5, np.float32))
+    b = np.ones((15, 15), np.float32)
+    b[2:13, 2:13] = np.nan
+    one = R.remove_nans(b, 2.0)
+    assert np.isnan(one[7, 7]) and one[2, 2] == 1.0
+    assert not np.isnan(R.remove_nans(R.remove_nans(one))).any()   # 2 cells per pass
+    e = np.arange(16, dtype=np.float32).reshape(4, 4)
+    e[0, 0] = np.nan
+    # at the corner the disc reads replicated edge pixels: rows 0, 0, 0, 1, 2 give the
+    # non-NaN values 1 | 1 2 | 1 2 | 4 4 4 5 6 | 8 8 9, median 4
+    assert R.remove_nans(e, 2.0)[0, 0] == np.float32(4.0)
