@@ -96,6 +96,7 @@ int esrrf_drift_create(int width, int height, int device, esrrf_drift **out)
     if (N > FRC_MAX_N) return ESRRF_ERR_UNSUPPORTED;
     esrrf_drift *q = new esrrf_drift();
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         drift_free(q);
         delete q;
         return code;

@@ -109,6 +109,7 @@ int dalloc(esrrf_engine *e, T **ptr, size_t bytes)
     hipError_t st = hipMalloc((void **)ptr, bytes);
     if (st != hipSuccess) {
         *ptr = nullptr;
+        (void)hipGetLastError();   // not to surface in the next launch's error check
         return fail(e, ESRRF_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st));
     }
     return ESRRF_OK;
@@ -414,6 +415,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
     esrrf_engine *e = new esrrf_engine();
     e->p = *p;
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         free_all(e);
         std::string msg = e->err;
         delete e;

@@ -121,6 +121,7 @@ int esrrf_frc_create(int width, int height, int device, esrrf_frc **out)
     if (N > FRC_MAX_N) return ESRRF_ERR_UNSUPPORTED;
     esrrf_frc *q = new esrrf_frc();
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         frc_free(q);
         delete q;
         return code;

@@ -27,7 +27,9 @@ template <class T>
 bool alloc(T **p, size_t bytes)
 {
     *p = nullptr;
-    return hipMalloc((void **)p, bytes ? bytes : 4) == hipSuccess;
+    if (hipMalloc((void **)p, bytes ? bytes : 4) == hipSuccess) return true;
+    (void)hipGetLastError();   // a failed allocation must not surface in later checks
+    return false;
 }
 
 Prep3dArgs prep_args(const ModeCtx &c, const ModeState &ms)

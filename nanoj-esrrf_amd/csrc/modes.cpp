@@ -38,7 +38,9 @@ template <class T>
 bool dmalloc(T **p, size_t bytes)
 {
     *p = nullptr;
-    return hipMalloc((void **)p, bytes ? bytes : 4) == hipSuccess;
+    if (hipMalloc((void **)p, bytes ? bytes : 4) == hipSuccess) return true;
+    (void)hipGetLastError();   // a failed allocation must not surface in later checks
+    return false;
 }
 
 // Java: (int) Math.ceil(Math.log(n)/Math.log(2)) then (int) Math.pow(2, .)

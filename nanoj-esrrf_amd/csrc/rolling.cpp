@@ -284,6 +284,7 @@ int esrrf_rolling_create(const esrrf_rolling_params *p, esrrf_rolling **out)
         return ESRRF_ERR_INVALID;
     esrrf_rolling *r = new esrrf_rolling();
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         rolling_free(r);
         delete r;
         return code;

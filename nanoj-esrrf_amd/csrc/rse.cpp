@@ -158,6 +158,7 @@ int esrrf_rse_create(const esrrf_rse_params *p, esrrf_rse **out)
         return ESRRF_ERR_INVALID;
     esrrf_rse *r = new esrrf_rse();
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         rse_free(r);
         delete r;
         return code;

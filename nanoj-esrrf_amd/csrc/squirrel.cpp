@@ -121,6 +121,7 @@ int esrrf_squirrel_create(const esrrf_squirrel_params *pp, esrrf_squirrel **out)
         return ESRRF_ERR_INVALID;
     esrrf_squirrel *q = new esrrf_squirrel();
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         squirrel_free(q);
         delete q;
         return code;

@@ -264,6 +264,7 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
         return ESRRF_ERR_INVALID;
     esrrf_sweep *s = new esrrf_sweep();
     auto bad = [&](int code) {
+        (void)hipGetLastError();   // a failed call must not surface in later checks
         sweep_free(s);
         delete s;
         return code;

@@ -167,6 +167,62 @@ def test_bic_rejects_invalid_arguments(pkg, w, h, M):
                      0, 1)
 
 
+def test_bic_cancel_between_chunks(pkg):
+    """A cancel flag raised before calculateSRRF (the plugin's Esc key, LiveSRRF_CL.java:731)
+    stops the load after its first chunk of 16 frames: calculateSRRF reports the
+    cancellation, the counters stand at 16, and the state equals a load of exactly those 16
+    frames."""
+    st = stack(40, 24, 20, seed=33)
+    res = {}
+    for mode in ("cancel", "sixteen"):
+        e = pkg.LiveSRRF_CL()
+        e.checkDevices()
+        e.initialise(24, 20, 3, 1.5, 2, 40, 40, 20000, e.allCLdevices[0], True, True, False,
+                     None, 0, 1)
+        e.resetFramePosition()
+        e.loadDriftXYGPUbuffer(np.zeros(40), np.zeros(40))
+        if mode == "cancel":
+            assert e.calculateSRRF(st, cancel=lambda: True) is True
+        else:
+            assert e.calculateSRRF(st[:16]) is False
+        assert list(e.frameCounters()) == [16, 16]
+        res[mode] = np.asarray(e.readSRRFbuffer()).copy()
+        e.release()
+    assert np.array_equal(res["cancel"], res["sixteen"])
+
+
+def test_bic_single_frame_sr_image(pkg, O):
+    """nFrameForSRRF = 1: TAC2's 1/(n - 1) is a division by zero in the reference
+    (liveSRRF.cl:374,377); the first frame initialises TAC2 to 0, so the result is finite:
+    AVG, TAC2 (= -AVG^2) and INT equal the oracle's, and VAR = E2 - AVG^2 is rounding
+    noise below 1e-5 of AVG^2 in both (without the MP correction, which would divide that
+    noise by its own macro-pixel mean)."""
+    st = stack(1, 20, 16, seed=21)
+    got, cnt, _ = run_engine(pkg, st, 4, 1.5, 2, mp=False)
+    exp, ocnt, _ = run_oracle(O, st, 4, 1.5, 2, mp=False)
+    assert cnt == ocnt == [1, 1]
+    assert np.isfinite(got).all()
+    g, x = got.reshape(4, 64, 80), exp.reshape(4, 64, 80)
+    for c, name in ((0, "AVG"), (2, "TAC2"), (3, "INT")):
+        ok, ex, idx = normalised_close(g[c], x[c])
+        assert ok, f"nF=1: {name} differs (excess {ex:.3g} at {idx})"
+    bound = 1e-5 * float(np.max(x[0].astype(np.float64) ** 2))
+    assert np.abs(g[1]).max() <= bound and np.abs(x[1]).max() <= bound
+
+
+def test_bic_allocation_failure_is_reported(pkg):
+    """A frame budget the device cannot hold (1e7 frames of 256 x 256) is refused by
+    initialise with the engine's error, and the device stays usable."""
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    with pytest.raises(pkg._capi.EsrrfError):
+        e.initialise(256, 256, 5, 1.5, 2, 10_000_000, 10_000_000, 20000, e.allCLdevices[0],
+                     True, True, False, None, 0, 1)
+    st = stack(3, 16, 16, seed=2)
+    got, cnt, _ = run_engine(pkg, st, 2, 1.5, 2)
+    assert cnt == [3, 3] and np.isfinite(got).all()
+
+
 def test_bic_no_intensity_weighting(pkg, O):
     st = stack(6, 24, 20, seed=5)
     got, _, _ = run_engine(pkg, st, 4, 1.5, 2, intW=False)
