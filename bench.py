@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=NF, help="frames per SR frame (default 1000)")
-    ap.add_argument("--cpu-sample-frames", type=int, default=48)
+    ap.add_argument("--cpu-sample-frames", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-rse", action="store_true")
