@@ -180,11 +180,25 @@ __global__ void __launch_bounds__(RB) k_rse_sums(const float *__restrict__ Y,
     }
 }
 
+// red[10] = sum y^2 of the cropped reference (once per reference image), from the sums
+// pass run with X = Y (its third partial)
+__global__ void __launch_bounds__(RB) k_rse_yy(const double *__restrict__ part, int nparts,
+                                               double *__restrict__ red)
+{
+    __shared__ double sh[RB];
+    double t = 0;
+    for (int b = threadIdx.x; b < nparts; b += RB) t += part[4 * b + 2];
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) red[10] = t;
+}
+
 // red[0..3] = totals; red[4] = alpha, red[5] = beta (ErrorMapLiveSRRF.java:201-233 with the
-// blurred ones equal to 1, so N = sum 1^2 = n)
+// blurred ones equal to 1, so N = sum 1^2 = n).  closed: also red[6] = sum of squared
+// residuals = sum y^2 - alpha sum xy - beta sum y, exact at the least-squares alpha, beta
+// (the value-only evaluation needs no residual pass)
 __global__ void __launch_bounds__(RB) k_rse_alpha_beta(const double *__restrict__ part,
                                                        int nparts, long n,
-                                                       double *__restrict__ red)
+                                                       double *__restrict__ red, int closed)
 {
     __shared__ double sh[RB];
     double t[4] = {0, 0, 0, 0};
@@ -200,6 +214,7 @@ __global__ void __launch_bounds__(RB) k_rse_alpha_beta(const double *__restrict_
         for (int k = 0; k < 4; k++) red[k] = t[k];
         red[4] = a;
         red[5] = b;
+        if (closed) red[6] = red[10] - a * t[3] - b * t[0];
     }
 }
 
@@ -256,13 +271,22 @@ static unsigned grid_n(long n)
     return (unsigned)(g > 2048 ? 2048 : (g < 1 ? 1 : g));
 }
 
+hipError_t launch_rse_yy(const RsePlan &p, hipStream_t s)
+{
+    const unsigned gn = grid_n(p.n);
+    hipLaunchKernelGGL(k_rse_sums, dim3(gn), dim3(RB), 0, s, p.Y, p.Y, p.n, p.part);
+    hipLaunchKernelGGL(k_rse_yy, dim3(1), dim3(RB), 0, s, p.part, (int)gn, p.red);
+    return hipGetLastError();
+}
+
 hipError_t launch_rse_prep(const RsePlan &p, const float *out4, hipStream_t s)
 {
     if (p.mp) hipLaunchKernelGGL(k_rse_mpstats, dim3(p.M * p.M), dim3(RB), 0, s, out4, p.w, p.h,
                                  p.M, p.stats);
     hipLaunchKernelGGL(k_rse_prep, dim3(grid_n(p.n)), dim3(RB), 0, s, out4, p.stats, p.w, p.h,
                        p.M, p.crop, p.mp, p.Y, p.SR, p.Dt, p.DS);
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_rse_yy(p, s);
 }
 
 hipError_t launch_rse_eval(const RsePlan &p, float sigma, bool grad, hipStream_t s)
@@ -289,7 +313,9 @@ hipError_t launch_rse_eval(const RsePlan &p, float sigma, bool grad, hipStream_t
                            (const float *)nullptr, p.G2, p.Wc, p.Hc, K);
     }
     hipLaunchKernelGGL(k_rse_sums, dim3(gn), dim3(RB), 0, s, p.Y, p.X, p.n, p.part);
-    hipLaunchKernelGGL(k_rse_alpha_beta, dim3(1), dim3(RB), 0, s, p.part, (int)gn, p.n, p.red);
+    hipLaunchKernelGGL(k_rse_alpha_beta, dim3(1), dim3(RB), 0, s, p.part, (int)gn, p.n, p.red,
+                       grad ? 0 : 1);
+    if (!grad) return hipGetLastError();   // red[6] in closed form
     hipLaunchKernelGGL(k_rse_resid, dim3(gn), dim3(RB), 0, s, p.Y, p.X, p.red,
                        grad ? p.G1 : (const float *)nullptr, grad ? p.G2 : (const float *)nullptr,
                        grad ? p.G3 : (const float *)nullptr, p.n, p.part);

@@ -105,7 +105,8 @@ hipError_t launch_squirrel_crop(const float *ref, const float *sr, int W, int cr
 {
     hipLaunchKernelGGL(k_sq_crop, dim3(sq_grid(p.n)), dim3(SB), 0, s, ref, sr, W, crop, p.Wc,
                        p.Hc, p.Y, p.SR);
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_rse_yy(p, s);   // sum y^2 of the new reference
 }
 
 hipError_t launch_squirrel_maps(const RsePlan &p, float *emap, float *rsc, double *part,

@@ -202,7 +202,7 @@ int esrrf_rse_create(const esrrf_rse_params *p, esrrf_rse **out)
     for (float **b : bufs) ok = ok && alloc((void **)b, cb);
     ok = ok && alloc((void **)&q.taps, sizeof(float) * 2 * (2 * q.K + 1)) &&
          alloc((void **)&q.part, sizeof(double) * 4 * rse_max_parts(q.n)) &&
-         alloc((void **)&q.red, sizeof(double) * 10);
+         alloc((void **)&q.red, sizeof(double) * 12);
     if (!ok) return bad(ESRRF_ERR_NOMEM);
     if (hipMemset(q.stats, 0, sizeof(double) * 3 * r->M * r->M) != hipSuccess)
         return bad(ESRRF_ERR_HIP);

@@ -20,10 +20,12 @@ struct RsePlan {
     float *T1, *T2, *X, *G1, *G2, *G3;   // blur scratch and blurred images
     float *taps;           // [2][2K+1]
     double *part;          // [rse_max_parts(n)][4]
-    double *red;           // [10]: sums, alpha, beta, E, E-gradient dot products
+    double *red;           // [12]: sums, alpha, beta, E, E-gradient dot products, sum y^2
 };
 
 hipError_t launch_rse_prep(const RsePlan &p, const float *out4, hipStream_t s);
+// sum y^2 of the cropped reference into red[10] (after the reference changes)
+hipError_t launch_rse_yy(const RsePlan &p, hipStream_t s);
 // value (grad = false) or value and the dot products of the gradient (grad = true)
 hipError_t launch_rse_eval(const RsePlan &p, float sigma, bool grad, hipStream_t s);
 int rse_max_parts(long n);

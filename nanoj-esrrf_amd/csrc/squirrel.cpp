@@ -158,7 +158,7 @@ int esrrf_squirrel_create(const esrrf_squirrel_params *pp, esrrf_squirrel **out)
     for (float **b : bufs) ok = ok && alloc((void **)b, cb);
     ok = ok && alloc((void **)&p.taps, sizeof(float) * 2 * (2 * p.K + 1)) &&
          alloc((void **)&p.part, sizeof(double) * 4 * rse_max_parts(p.n)) &&
-         alloc((void **)&p.red, sizeof(double) * 10) &&
+         alloc((void **)&p.red, sizeof(double) * 12) &&
          alloc((void **)&q->d_part, sizeof(double) * 6 * squirrel_max_parts(p.n)) &&
          alloc((void **)&q->d_out, sizeof(double) * 4);
     if (!ok) return bad(ESRRF_ERR_NOMEM);
