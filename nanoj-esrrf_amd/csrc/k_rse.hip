@@ -115,6 +115,65 @@ __global__ void __launch_bounds__(RB) k_gauss_taps(float sigma, int K, float *__
     }
 }
 
+// The value taps of k_gauss_taps, computed by one block into LDS with the same arithmetic
+// and reduction (so the same bits); for the blur kernels of the value-only evaluation.
+__device__ void gauss_taps_block(float sigma, int K, float *t, double *sh)
+{
+    const int n = 2 * K + 1;
+    double ez = 0.0;
+    const double s = (double)sigma;
+    for (int i = threadIdx.x; i < n; i += RB) {
+        const double k = (double)(i - K);
+        ez += s > 0.0 ? exp(-k * k / (2.0 * s * s)) : (i == K ? 1.0 : 0.0);
+    }
+    const double Z = block_sum(ez, sh);
+    for (int i = threadIdx.x; i < n; i += RB) {
+        const double k = (double)(i - K);
+        const double e = s > 0.0 ? exp(-k * k / (2.0 * s * s)) : (i == K ? 1.0 : 0.0);
+        t[i] = (float)(e / Z);
+    }
+    __syncthreads();
+}
+
+// k_blur_cols / k_blur_rows (one image) with the taps of sigma formed in LDS
+__global__ void __launch_bounds__(RB) k_blur_cols_sig(const float *__restrict__ in,
+                                                      float *__restrict__ out, int W, int H,
+                                                      float sigma, int K)
+{
+    extern __shared__ float tl[];
+    __shared__ double sh[RB];
+    gauss_taps_block(sigma, K, tl, sh);
+    const long n = (long)W * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i - (long)y * W);
+        float acc = 0.0f;
+        for (int k = -K; k <= K; k++) {
+            const int yy = min(max(y + k, 0), H - 1);
+            acc = fmaf(tl[k + K], in[(long)yy * W + x], acc);
+        }
+        out[i] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(RB) k_blur_rows_sig(const float *__restrict__ a,
+                                                      float *__restrict__ out, int W, int H,
+                                                      float sigma, int K)
+{
+    extern __shared__ float tl[];
+    __shared__ double sh[RB];
+    gauss_taps_block(sigma, K, tl, sh);
+    const long n = (long)W * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(i / W), x = (int)(i - (long)y * W);
+        const float *ra = a + (long)y * W;
+        float acc = 0.0f;
+        for (int k = -K; k <= K; k++) acc = fmaf(tl[k + K], ra[min(max(x + k, 0), W - 1)], acc);
+        out[i] = acc;
+    }
+}
+
 // out = sum_k taps[k] in[clamp(y + k - K)][x]
 __global__ void k_blur_cols(const float *__restrict__ in, float *__restrict__ out, int W, int H,
                             const float *__restrict__ taps, int K)
@@ -294,6 +353,19 @@ hipError_t launch_rse_eval(const RsePlan &p, float sigma, bool grad, hipStream_t
     const int K = p.K;
     const float *g = p.taps, *dg = p.taps + 2 * K + 1;
     const unsigned gn = grid_n(p.n);
+    if (!grad) {
+        // value only: x = Gx Gy SR with the taps formed per block, the first-pass sums, and
+        // alpha, beta and the squared residuals in closed form
+        const size_t lt = sizeof(float) * (size_t)(2 * K + 1);
+        hipLaunchKernelGGL(k_blur_cols_sig, dim3(gn), dim3(RB), lt, s, p.SR, p.T1, p.Wc, p.Hc,
+                           sigma, K);
+        hipLaunchKernelGGL(k_blur_rows_sig, dim3(gn), dim3(RB), lt, s, p.T1, p.X, p.Wc, p.Hc,
+                           sigma, K);
+        hipLaunchKernelGGL(k_rse_sums, dim3(gn), dim3(RB), 0, s, p.Y, p.X, p.n, p.part);
+        hipLaunchKernelGGL(k_rse_alpha_beta, dim3(1), dim3(RB), 0, s, p.part, (int)gn, p.n,
+                           p.red, 1);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_gauss_taps, dim3(1), dim3(RB), 0, s, sigma, K, p.taps);
     // x = Gx Gy SR (T1 = Gy SR)
     hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), 0, s, p.SR, p.T1, p.Wc, p.Hc, g, K);
@@ -314,8 +386,7 @@ hipError_t launch_rse_eval(const RsePlan &p, float sigma, bool grad, hipStream_t
     }
     hipLaunchKernelGGL(k_rse_sums, dim3(gn), dim3(RB), 0, s, p.Y, p.X, p.n, p.part);
     hipLaunchKernelGGL(k_rse_alpha_beta, dim3(1), dim3(RB), 0, s, p.part, (int)gn, p.n, p.red,
-                       grad ? 0 : 1);
-    if (!grad) return hipGetLastError();   // red[6] in closed form
+                       0);
     hipLaunchKernelGGL(k_rse_resid, dim3(gn), dim3(RB), 0, s, p.Y, p.X, p.red,
                        grad ? p.G1 : (const float *)nullptr, grad ? p.G2 : (const float *)nullptr,
                        grad ? p.G3 : (const float *)nullptr, p.n, p.part);
