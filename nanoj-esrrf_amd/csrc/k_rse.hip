@@ -174,10 +174,14 @@ __global__ void __launch_bounds__(RB) k_blur_rows_sig(const float *__restrict__ 
     }
 }
 
-// out = sum_k taps[k] in[clamp(y + k - K)][x]
-__global__ void k_blur_cols(const float *__restrict__ in, float *__restrict__ out, int W, int H,
-                            const float *__restrict__ taps, int K)
+// out = sum_k taps[k] in[clamp(y + k - K)][x] (taps staged in LDS)
+__global__ void __launch_bounds__(RB) k_blur_cols(const float *__restrict__ in,
+                                                  float *__restrict__ out, int W, int H,
+                                                  const float *__restrict__ taps_g, int K)
 {
+    extern __shared__ float taps[];
+    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) taps[i] = taps_g[i];
+    __syncthreads();
     const long n = (long)W * H;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
@@ -191,11 +195,21 @@ __global__ void k_blur_cols(const float *__restrict__ in, float *__restrict__ ou
     }
 }
 
-// out = sum_k ta[k] a[y][clamp(x + k - K)] (+ sum_k tb[k] b[...] when b is given)
-__global__ void k_blur_rows(const float *__restrict__ a, const float *__restrict__ ta,
-                            const float *__restrict__ b, const float *__restrict__ tb,
-                            float *__restrict__ out, int W, int H, int K)
+// out = sum_k ta[k] a[y][clamp(x + k - K)] (+ sum_k tb[k] b[...] when b is given; taps
+// staged in LDS)
+__global__ void __launch_bounds__(RB) k_blur_rows(const float *__restrict__ a,
+                                                  const float *__restrict__ ta_g,
+                                                  const float *__restrict__ b,
+                                                  const float *__restrict__ tb_g,
+                                                  float *__restrict__ out, int W, int H, int K)
 {
+    extern __shared__ float tl[];
+    float *ta = tl, *tb = tl + 2 * K + 1;
+    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) {
+        ta[i] = ta_g[i];
+        if (b) tb[i] = tb_g[i];
+    }
+    __syncthreads();
     const long n = (long)W * H;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
@@ -367,21 +381,22 @@ hipError_t launch_rse_eval(const RsePlan &p, float sigma, bool grad, hipStream_t
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_gauss_taps, dim3(1), dim3(RB), 0, s, sigma, K, p.taps);
+    const size_t lt1 = sizeof(float) * (size_t)(2 * K + 1), lt2 = 2 * lt1;
     // x = Gx Gy SR (T1 = Gy SR)
-    hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), 0, s, p.SR, p.T1, p.Wc, p.Hc, g, K);
-    hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), 0, s, p.T1, g, (const float *)nullptr,
+    hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), lt1, s, p.SR, p.T1, p.Wc, p.Hc, g, K);
+    hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), lt2, s, p.T1, g, (const float *)nullptr,
                        (const float *)nullptr, p.X, p.Wc, p.Hc, K);
     if (grad) {
         // dx/dsigma = dGx Gy SR + Gx dGy SR
-        hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), 0, s, p.SR, p.T2, p.Wc, p.Hc, dg, K);
-        hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), 0, s, p.T1, dg, p.T2, g, p.G3, p.Wc,
+        hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), lt1, s, p.SR, p.T2, p.Wc, p.Hc, dg, K);
+        hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), lt2, s, p.T1, dg, p.T2, g, p.G3, p.Wc,
                            p.Hc, K);
         // blurred forward-mode derivatives of SR
-        hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), 0, s, p.Dt, p.T2, p.Wc, p.Hc, g, K);
-        hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), 0, s, p.T2, g, (const float *)nullptr,
+        hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), lt1, s, p.Dt, p.T2, p.Wc, p.Hc, g, K);
+        hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), lt2, s, p.T2, g, (const float *)nullptr,
                            (const float *)nullptr, p.G1, p.Wc, p.Hc, K);
-        hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), 0, s, p.DS, p.T2, p.Wc, p.Hc, g, K);
-        hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), 0, s, p.T2, g, (const float *)nullptr,
+        hipLaunchKernelGGL(k_blur_cols, dim3(gn), dim3(RB), lt1, s, p.DS, p.T2, p.Wc, p.Hc, g, K);
+        hipLaunchKernelGGL(k_blur_rows, dim3(gn), dim3(RB), lt2, s, p.T2, g, (const float *)nullptr,
                            (const float *)nullptr, p.G2, p.Wc, p.Hc, K);
     }
     hipLaunchKernelGGL(k_rse_sums, dim3(gn), dim3(RB), 0, s, p.Y, p.X, p.n, p.part);
