@@ -46,11 +46,15 @@ int dfail(esrrf_drift *q, int code, const char *fmt, ...)
     return code;
 }
 
+// a failed call also leaves the runtime's last error set: cleared, so that later launch
+// checks on this thread do not report it as theirs
 #define DCHK(q, x)                                                                         \
     do {                                                                                   \
         hipError_t _st = (x);                                                              \
-        if (_st != hipSuccess)                                                             \
+        if (_st != hipSuccess) {                                                           \
+            (void)hipGetLastError();                                                       \
             return dfail((q), ESRRF_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_st)); \
+        }                                                                                  \
     } while (0)
 
 void drift_free(esrrf_drift *q)
