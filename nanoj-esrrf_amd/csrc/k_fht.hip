@@ -33,6 +33,21 @@ __device__ __forceinline__ float2 twid(const float2 *__restrict__ twMN, int q, i
     return twMN[j];
 }
 
+// Tile of this block in a (gx, gy) grid.  Workgroups go to the eight XCDs in turn
+// (workgroup i to XCD i mod 8), each XCD with its own L2.  A column pass reads G adjacent
+// columns per block, a part of each 128-byte line, so in launch order the neighbouring column
+// groups that share a line run on different XCDs and each XCD fetches the line.  With xcd
+// set (and a grid that divides by eight) XCD j takes the contiguous tile range
+// [j total/8, (j + 1) total/8) in order instead, and neighbours share the line in one L2.
+__device__ __forceinline__ int2 fht_tile(int xcd)
+{
+    const int gx = gridDim.x, total = gx * gridDim.y;
+    int lin = blockIdx.y * gx + blockIdx.x;
+    if (xcd && (total & 7) == 0) lin = (lin & 7) * (total >> 3) + (lin >> 3);
+    const int ty = lin / gx;
+    return make_int2(lin - ty * gx, ty);
+}
+
 // Step 1: forward FFT along x of G rows of the mirror-padded frame
 // (FourierInterpolation.java:118-172; the identity for power-of-two squares).
 // grid (N/G, frames); F[f][y][kx]
@@ -69,8 +84,9 @@ __global__ void __launch_bounds__(256) k_fht_fwd_cols(float2 *__restrict__ F, Fh
     const int N = p.N;
     float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
     stage_twiddles(TW, p.tw, N);
-    const int a0 = blockIdx.x * G;
-    float2 *Ff = F + (long)N * N * blockIdx.y;
+    const int2 t = fht_tile(p.xcd);
+    const int a0 = t.x * G;
+    float2 *Ff = F + (long)N * N * t.y;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
         const int g = e % G, k = e / G;          // consecutive threads: consecutive columns
         A[g * N + k] = Ff[(long)k * N + a0 + g];
@@ -122,9 +138,10 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__
     const int wM = p.M * p.w, hM = p.M * p.h;
     float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
     stage_twiddles(TW, p.tw, N);
-    const int a0 = blockIdx.x * G;
+    const int2 tl = fht_tile(p.xcd);
+    const int a0 = tl.x * G;
     const int MM = M * M;
-    const int f = blockIdx.y / MM, bd = blockIdx.y - f * MM;
+    const int f = tl.y / MM, bd = tl.y - f * MM;
     const int b = bd / M, d = bd - b * M;
     const float2 *Rc = Rb + ((long)f * M + b) * N * N;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
@@ -162,8 +179,9 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols_rows(const float2 *__restr
     float *T = reinterpret_cast<float *>(TW + N);   // [N][M G]
     stage_twiddles(TW, p.tw, N);
     const int MG = M * G;
-    const int a0 = blockIdx.x * G;
-    const int f = blockIdx.y / M, d = blockIdx.y - f * M;
+    const int2 tl = fht_tile(p.xcd);
+    const int a0 = tl.x * G;
+    const int f = tl.y / M, d = tl.y - f * M;
     for (int b = 0; b < M; b++) {
         const float2 *Rc = Rb + ((long)f * M + b) * N * N;
         for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
@@ -294,19 +312,22 @@ __device__ __forceinline__ float2 unit_or_zero_f(float tx, float ty)
     return make_float2(sx * r, sy * r);
 }
 
-__global__ void k_fht_prep(const float *__restrict__ px, float2 *__restrict__ U, int W, int H)
+// grid (H * ceil(W / 256), frames): a block takes 256 consecutive pixels of one row, so the
+// pixel's coordinates come from one division per block rather than a 64-bit one per pixel
+__global__ void __launch_bounds__(256) k_fht_prep(const float *__restrict__ px,
+                                                  float2 *__restrict__ U, int W, int H)
 {
     const int f = blockIdx.y;
+    const int cw = (W + 255) >> 8;
+    const int m = blockIdx.x / cw;
+    const int n = (blockIdx.x - m * cw) * 256 + threadIdx.x;
+    if (n >= W) return;
     const long wh = (long)W * H;
     const float *P = px + wh * f;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < wh;
-         e += (long)gridDim.x * blockDim.x) {
-        const int m = (int)(e / W), n = (int)(e - (long)m * W);
-        float gx, gy, dummy;
-        robx(P, W, H, min(n + 1, W - 1), m, gx, dummy);
-        robx(P, W, H, n, min(m + 1, H - 1), dummy, gy);
-        U[wh * f + e] = unit_or_zero_f(gx, gy);
-    }
+    float gx, gy, dummy;
+    robx(P, W, H, min(n + 1, W - 1), m, gx, dummy);
+    robx(P, W, H, n, min(m + 1, H - 1), dummy, gy);
+    U[wh * f + (long)m * W + n] = unit_or_zero_f(gx, gy);
 }
 
 __global__ void k_fht_grad(const float *__restrict__ px, float *__restrict__ gx,
@@ -325,7 +346,8 @@ __global__ void k_fht_grad(const float *__restrict__ px, float *__restrict__ gx,
 
 hipError_t launch_fht_prep(const float *px, float2 *U, int W, int H, int nF, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fht_prep, dim3(grid_of((long)W * H), nF), dim3(256), 0, s, px, U, W, H);
+    hipLaunchKernelGGL(k_fht_prep, dim3((unsigned)H * ((W + 255) / 256), nF), dim3(256), 0, s,
+                       px, U, W, H);
     return hipGetLastError();
 }
 

@@ -155,6 +155,9 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
             // last pass: row-tiled stores (default) or the pixel-by-pixel kernel
             const char *sc = getenv("ESRRF_FHT_SCATTER");
             p.scatter = (sc && atoi(sc) != 0) ? 1 : 0;
+            // column passes: tiles grouped per XCD (default) or in launch order
+            const char *xo = getenv("ESRRF_FHT_XCD");
+            p.xcd = (xo && atoi(xo) == 0) ? 0 : 1;
             float2 *dtw = nullptr;
             if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N * p.chunk) ||
                 !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * p.chunk) ||
