@@ -73,4 +73,20 @@ __device__ __forceinline__ void stage_twiddles(float2 *TW, const float2 *__restr
     for (int t = threadIdx.x; t < N; t += blockDim.x) TW[t] = tw[t];
 }
 
+// Tile of this block in a (gx, gy) grid.  Workgroups go to the eight XCDs in turn
+// (workgroup i to XCD i mod 8), each XCD with its own L2.  A column pass of the Stockham FFTs
+// reads G adjacent columns per block, a part of each 128-byte line, so in launch order the
+// neighbouring column groups that share a line run on different XCDs and each XCD fetches
+// the line.  With xcd set (and a grid that divides by eight) XCD j takes the contiguous tile
+// range [j total/8, (j + 1) total/8) in order instead, and neighbours share the line in one
+// L2.  Either order visits every tile once.
+__device__ __forceinline__ int2 xcd_tile(int xcd)
+{
+    const int gx = gridDim.x, total = gx * gridDim.y;
+    int lin = blockIdx.y * gx + blockIdx.x;
+    if (xcd && (total & 7) == 0) lin = (lin & 7) * (total >> 3) + (lin >> 3);
+    const int ty = lin / gx;
+    return make_int2(lin - ty * gx, ty);
+}
+
 }  // namespace esrrf

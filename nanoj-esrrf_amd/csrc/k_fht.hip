@@ -33,21 +33,6 @@ __device__ __forceinline__ float2 twid(const float2 *__restrict__ twMN, int q, i
     return twMN[j];
 }
 
-// Tile of this block in a (gx, gy) grid.  Workgroups go to the eight XCDs in turn
-// (workgroup i to XCD i mod 8), each XCD with its own L2.  A column pass reads G adjacent
-// columns per block, a part of each 128-byte line, so in launch order the neighbouring column
-// groups that share a line run on different XCDs and each XCD fetches the line.  With xcd
-// set (and a grid that divides by eight) XCD j takes the contiguous tile range
-// [j total/8, (j + 1) total/8) in order instead, and neighbours share the line in one L2.
-__device__ __forceinline__ int2 fht_tile(int xcd)
-{
-    const int gx = gridDim.x, total = gx * gridDim.y;
-    int lin = blockIdx.y * gx + blockIdx.x;
-    if (xcd && (total & 7) == 0) lin = (lin & 7) * (total >> 3) + (lin >> 3);
-    const int ty = lin / gx;
-    return make_int2(lin - ty * gx, ty);
-}
-
 // Step 1: forward FFT along x of G rows of the mirror-padded frame
 // (FourierInterpolation.java:118-172; the identity for power-of-two squares).
 // grid (N/G, frames); F[f][y][kx]
@@ -84,7 +69,7 @@ __global__ void __launch_bounds__(256) k_fht_fwd_cols(float2 *__restrict__ F, Fh
     const int N = p.N;
     float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
     stage_twiddles(TW, p.tw, N);
-    const int2 t = fht_tile(p.xcd);
+    const int2 t = xcd_tile(p.xcd);
     const int a0 = t.x * G;
     float2 *Ff = F + (long)N * N * t.y;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
@@ -138,7 +123,7 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__
     const int wM = p.M * p.w, hM = p.M * p.h;
     float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
     stage_twiddles(TW, p.tw, N);
-    const int2 tl = fht_tile(p.xcd);
+    const int2 tl = xcd_tile(p.xcd);
     const int a0 = tl.x * G;
     const int MM = M * M;
     const int f = tl.y / MM, bd = tl.y - f * MM;
@@ -179,7 +164,7 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols_rows(const float2 *__restr
     float *T = reinterpret_cast<float *>(TW + N);   // [N][M G]
     stage_twiddles(TW, p.tw, N);
     const int MG = M * G;
-    const int2 tl = fht_tile(p.xcd);
+    const int2 tl = xcd_tile(p.xcd);
     const int a0 = tl.x * G;
     const int f = tl.y / M, d = tl.y - f * M;
     for (int b = 0; b < M; b++) {

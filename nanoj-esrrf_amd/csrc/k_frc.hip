@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_frc_rows(FrcPlan p)
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) R0[e] = R[e];
 }
 
-// G columns of one image per block, in place. grid (N/G, 2)
+// G columns of one image per block, in place, tiles in XCD-grouped order. grid (N/G, batch)
 template <int G>
 __global__ void __launch_bounds__(256) k_frc_cols(FrcPlan p)
 {
@@ -49,8 +49,9 @@ __global__ void __launch_bounds__(256) k_frc_cols(FrcPlan p)
     const int N = p.N;
     float2 *A = sm, *B = sm + G * N, *TW = sm + 2 * G * N;
     stage_twiddles(TW, p.tw, N);
-    const int a0 = blockIdx.x * G;
-    float2 *F = p.buf + (long)blockIdx.y * N * N;
+    const int2 t = xcd_tile(1);
+    const int a0 = t.x * G;
+    float2 *F = p.buf + (long)t.y * N * N;
     for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
         const int g = e % G, k = e / G;
         A[g * N + k] = F[(long)k * N + a0 + g];
