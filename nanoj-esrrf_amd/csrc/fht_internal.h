@@ -13,6 +13,7 @@ struct FhtPlan {
     int chunk;                          // frames per batch of the scratch spectra
     int scatter;                        // 1: last pass stores pixel by pixel (k_fht_inv_cols)
     int xcd;                            // 1: column passes take tiles in XCD-grouped order
+    int no_prefetch;                    // 1: last pass without the register prefetch
     float2 *F;                          // [chunk][N][N] spectra
     float2 *Y;                          // [chunk][M][N][N] row-transformed twiddled spectra
     const float2 *tw;                   // N twiddles e^{-2 pi i k / N}

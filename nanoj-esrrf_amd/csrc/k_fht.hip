@@ -151,8 +151,10 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols(const float2 *__restrict__
 // and runs the column transforms of every b in turn (each exactly as k_fht_inv_cols does),
 // collecting (Re - Im)/(MN)^2 in an LDS tile of output rows y = M c + d, x = M a + b; the
 // tile then goes out as runs of M G consecutive pixels per row instead of pixels M apart.
+// With PF (G N <= 1024 and 256 threads) each thread holds its elements of the next b's
+// columns in registers, loaded while this b's transform runs in LDS.
 // grid (N/G, frames * M), d from blockIdx.y
-template <int G>
+template <int G, bool PF>
 __global__ void __launch_bounds__(256) k_fht_inv_cols_rows(const float2 *__restrict__ Rb,
                                                            float *__restrict__ out,
                                                            long out_stride, FhtPlan p, float inv)
@@ -167,15 +169,37 @@ __global__ void __launch_bounds__(256) k_fht_inv_cols_rows(const float2 *__restr
     const int2 tl = xcd_tile(p.xcd);
     const int a0 = tl.x * G;
     const int f = tl.y / M, d = tl.y - f * M;
-    for (int b = 0; b < M; b++) {
+    float2 nv[4];
+    auto fetch = [&](int b) {
         const float2 *Rc = Rb + ((long)f * M + b) * N * N;
-        for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
-            const int g = e % G, k = e / G;
-            const float2 v = Rc[(long)k * N + a0 + g];
-            const float2 t = twid(p.twMN, k, d, N, MN);
-            A[g * N + k] = make_float2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int e = threadIdx.x + 256 * i, g = e % G, k = e / G;
+            if (e < G * N) nv[i] = Rc[(long)k * N + a0 + g];
+        }
+    };
+    if (PF) fetch(0);
+    for (int b = 0; b < M; b++) {
+        if (PF) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = threadIdx.x + 256 * i, g = e % G, k = e / G;
+                if (e >= G * N) break;
+                const float2 v = nv[i];
+                const float2 t = twid(p.twMN, k, d, N, MN);
+                A[g * N + k] = make_float2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
+            }
+        } else {
+            const float2 *Rc = Rb + ((long)f * M + b) * N * N;
+            for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
+                const int g = e % G, k = e / G;
+                const float2 v = Rc[(long)k * N + a0 + g];
+                const float2 t = twid(p.twMN, k, d, N, MN);
+                A[g * N + k] = make_float2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
+            }
         }
         __syncthreads();
+        if (PF && b + 1 < M) fetch(b + 1);
         const float2 *R = stockham<G>(A, B, N, p.log2N, TW);
         for (int e = threadIdx.x; e < G * N; e += blockDim.x) {
             const int g = e % G, c = e / G;
@@ -209,8 +233,13 @@ static void inv_cols_rows_g(const FhtPlan &p, const float2 *Y, float *out, long 
 {
     const size_t lds = sizeof(float2) * ((size_t)2 * G * p.N + p.N) +
                        sizeof(float) * (size_t)p.N * p.M * G;
-    hipLaunchKernelGGL((k_fht_inv_cols_rows<G>), dim3(p.N / G, k * p.M), dim3(256), lds, s, Y,
-                       out, stride, p, inv);
+    const dim3 grid(p.N / G, k * p.M);
+    if (G * p.N <= 1024 && !p.no_prefetch)
+        hipLaunchKernelGGL((k_fht_inv_cols_rows<G, true>), grid, dim3(256), lds, s, Y, out,
+                           stride, p, inv);
+    else
+        hipLaunchKernelGGL((k_fht_inv_cols_rows<G, false>), grid, dim3(256), lds, s, Y, out,
+                           stride, p, inv);
 }
 
 static void inv_cols_rows(const FhtPlan &p, const float2 *Y, float *out, long stride, float inv,

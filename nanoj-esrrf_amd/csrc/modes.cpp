@@ -158,6 +158,8 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
             // column passes: tiles grouped per XCD (default) or in launch order
             const char *xo = getenv("ESRRF_FHT_XCD");
             p.xcd = (xo && atoi(xo) == 0) ? 0 : 1;
+            const char *pf = getenv("ESRRF_FHT_PREFETCH");
+            p.no_prefetch = (pf && atoi(pf) == 0) ? 1 : 0;
             float2 *dtw = nullptr;
             if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N * p.chunk) ||
                 !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * p.chunk) ||
