@@ -60,7 +60,7 @@ int dfail(esrrf_drift *q, int code, const char *fmt, ...)
 void drift_free(esrrf_drift *q)
 {
     DriftPlan &p = q->plan;
-    void *ptrs[] = {(void *)p.tw, p.R, p.buf, p.sums, q->d_shift, q->d_stage};
+    void *ptrs[] = {(void *)p.tw, p.R, p.buf, p.sums, p.pkv, p.pki, q->d_shift, q->d_stage};
     for (void *x : ptrs)
         if (x) (void)hipFree(x);
     if (q->own_stream) (void)hipStreamDestroy(q->own_stream);
@@ -128,7 +128,10 @@ int esrrf_drift_create(int width, int height, int device, esrrf_drift **out)
     p.tw = d_tw;
     if (hipMalloc((void **)&p.R, sizeof(float2) * (size_t)N * N) != hipSuccess ||
         hipMalloc((void **)&p.buf, sizeof(float2) * (size_t)p.max_batch * N * N) != hipSuccess ||
-        hipMalloc((void **)&p.sums, sizeof(double) * (size_t)p.max_batch) != hipSuccess ||
+        hipMalloc((void **)&p.sums, sizeof(double) * (size_t)p.max_batch * (1 + DRIFT_SEG)) !=
+            hipSuccess ||
+        hipMalloc((void **)&p.pkv, sizeof(float) * (size_t)p.max_batch * DRIFT_SEG) != hipSuccess ||
+        hipMalloc((void **)&p.pki, sizeof(int) * (size_t)p.max_batch * DRIFT_SEG) != hipSuccess ||
         hipMalloc((void **)&q->d_shift, sizeof(float) * 2 * (size_t)p.max_batch) != hipSuccess)
         return bad(ESRRF_ERR_NOMEM);
     if (hipMemcpy(d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice) != hipSuccess)
