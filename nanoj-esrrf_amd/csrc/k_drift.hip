@@ -43,23 +43,24 @@ __global__ void k_drift_sums_fin(const double *__restrict__ part, int batch,
     sums[b] = acc;
 }
 
-// frame b of the batch minus its mean into the top-left of an N x N complex image
-__global__ void k_drift_prep(const float *__restrict__ frames, const double *__restrict__ sums,
-                             int w, int h, int N, int batch, float2 *__restrict__ buf)
+// frame b of the batch minus its mean into the top-left of an N x N complex image;
+// grid (N * ceil(N / 256), frames): a block takes 256 consecutive elements of one row
+__global__ void __launch_bounds__(256) k_drift_prep(const float *__restrict__ frames,
+                                                     const double *__restrict__ sums, int w,
+                                                     int h, int N, float2 *__restrict__ buf)
 {
-    const long NN = (long)N * N, wh = (long)w * h;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < batch * NN;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = (int)(e / NN);
-        const long r = e - b * NN;
-        const int y = (int)(r / N), x = (int)(r - (long)y * N);
-        float v = 0.0f;
-        if (x < w && y < h) {
-            const float s = frames[wh * b + (long)y * w + x];
-            v = (float)((double)(s == s ? s : 0.0f) - sums[b] / (double)wh);
-        }
-        buf[e] = make_float2(v, 0.0f);
+    const int b = blockIdx.y;
+    const int cw = (N + 255) >> 8;
+    const int y = blockIdx.x / cw;
+    const int x = (blockIdx.x - y * cw) * 256 + threadIdx.x;
+    if (x >= N) return;
+    const long wh = (long)w * h;
+    float v = 0.0f;
+    if (x < w && y < h) {
+        const float s = frames[wh * b + (long)y * w + x];
+        v = (float)((double)(s == s ? s : 0.0f) - sums[b] / (double)wh);
     }
+    buf[(long)N * N * b + (long)y * N + x] = make_float2(v, 0.0f);
 }
 
 // R(k) conj(F_b(k)): the conjugate of conj(R) F_b, whose inverse transform is the
@@ -160,8 +161,9 @@ hipError_t launch_drift_reference(const DriftPlan &d, const float *ref, hipStrea
                        d.sums + d.max_batch);
     hipLaunchKernelGGL(k_drift_sums_fin, dim3(1), dim3(64), 0, s, d.sums + d.max_batch, 1,
                        d.sums);
-    hipLaunchKernelGGL(k_drift_prep, dim3(grid_for((long)d.N * d.N)), dim3(256), 0, s, ref,
-                       d.sums, d.w, d.h, d.N, 1, d.R);
+    const unsigned rows = (unsigned)d.N * ((d.N + 255) / 256);
+    hipLaunchKernelGGL(k_drift_prep, dim3(rows, 1), dim3(256), 0, s, ref, d.sums, d.w, d.h, d.N,
+                       d.R);
     FrcPlan p{};
     p.N = d.N;
     p.log2N = d.log2N;
@@ -179,8 +181,9 @@ hipError_t launch_drift_batch(const DriftPlan &d, const float *frames, int batch
                        d.sums + d.max_batch);
     hipLaunchKernelGGL(k_drift_sums_fin, dim3((batch + 63) / 64), dim3(64), 0, s,
                        d.sums + d.max_batch, batch, d.sums);
-    hipLaunchKernelGGL(k_drift_prep, dim3(grid_for(n)), dim3(256), 0, s, frames, d.sums, d.w,
-                       d.h, d.N, batch, d.buf);
+    const unsigned rows = (unsigned)d.N * ((d.N + 255) / 256);
+    hipLaunchKernelGGL(k_drift_prep, dim3(rows, batch), dim3(256), 0, s, frames, d.sums, d.w,
+                       d.h, d.N, d.buf);
     FrcPlan p{};
     p.N = d.N;
     p.log2N = d.log2N;
