@@ -193,3 +193,27 @@ def test_fht_upsampling_of_band_limited_frames(pkg):
     fm = 5.0 + np.cos(2 * np.pi * (3 * xm + 2 * ym) / 32) + \
         0.5 * np.sin(2 * np.pi * (xm - 5 * ym) / 32)
     assert np.abs(up - fm).max() < 2e-4 * np.abs(fm).max()
+
+
+@pytest.mark.parametrize("var", ["ESRRF_FHT_XCD", "ESRRF_FHT_PREFETCH"])
+@pytest.mark.parametrize("M,w,h", [(5, 32, 20), (3, 64, 48), (2, 16, 16)])
+def test_fht_tile_order_and_prefetch_do_not_change_results(pkg, monkeypatch, var, M, w, h):
+    """The XCD-grouped tile order of the column passes and the register prefetch of the last
+    pass only reorder work: with either switched off the upsampled gradients and the SR
+    image are the same bit for bit."""
+    st = stack(12, w, h, seed=90 + M)
+    res, grads = {}, {}
+    for on in ("1", "0"):
+        monkeypatch.setenv(var, on)
+        e = pkg.LiveSRRF_CL()
+        e.checkDevices()
+        e.initialise(w, h, M, 1.5, 2, 12, 12, 20000, e.allCLdevices[0], True, True, True, None,
+                     0, 1)
+        e.resetFramePosition()
+        e.loadDriftXYGPUbuffer(np.zeros(12), np.zeros(12))
+        e.calculateSRRF(st)
+        grads[on] = np.asarray(e.readGradientBuffers()).copy()
+        res[on] = np.asarray(e.readSRRFbuffer()).copy()
+        e.release()
+    assert np.array_equal(grads["0"], grads["1"])
+    assert np.array_equal(res["0"], res["1"])
