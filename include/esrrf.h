@@ -114,8 +114,12 @@ int esrrf_set_stream(esrrf_engine *e, void *hip_stream);
 int esrrf_profile_enable(esrrf_engine *e, int enable);
 int esrrf_profile_read(esrrf_engine *e, int which, double *total_ms, long *launches);
 
-/* Constants of prepareBuffersBICinterpolation (:304-305,317-320). */
+/* Constants of prepareBuffersBICinterpolation (:304-305,317-320).  tSO is the float the
+ * kernel reads from the text `"" + 2 * sigma + 1` (:318), which Java evaluates as string
+ * concatenation: Float.toString(2*sigma) followed by "1" (1.27442651 at fwhm 1.5). */
 int esrrf_bic_constants(float fwhm, float *sigma, float *radius, float *tSS, float *tSO);
+/* The $TWOSIGpONE$ text of :318 itself, NUL-terminated, into buf[len]. */
+int esrrf_bic_tso_text(float fwhm, char *buf, int len);
 /* Constants of prepareBuffersFHTinterpolation (:491-500). */
 int esrrf_fht_constants(float fwhm, int magnification, float *sigma, float *radius,
                         float *tSS, float *tSO);

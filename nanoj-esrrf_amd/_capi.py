@@ -27,7 +27,7 @@ EXPORTS = [
     "esrrf_read_gradients", "esrrf_read_aligned", "esrrf_read_mp_maps",
     "esrrf_read_frame_counters", "esrrf_get_dims", "esrrf_finish", "esrrf_set_stream",
     "esrrf_profile_enable", "esrrf_profile_read", "esrrf_bic_constants",
-    "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
+    "esrrf_bic_tso_text", "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
     "esrrf_last_error", "esrrf_version", "esrrf_autotune", "esrrf_launch_shape",
     "esrrf_rgc_kernel",
     "esrrf_sweep_create", "esrrf_sweep_destroy", "esrrf_sweep_set_stream", "esrrf_sweep_run",
@@ -156,6 +156,7 @@ def lib():
         "esrrf_profile_read": [vp, c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_long)],
         "esrrf_bic_constants": [c_float, fp, fp, fp, fp],
+        "esrrf_bic_tso_text": [c_float, ctypes.c_char_p, c_int],
         "esrrf_fht_constants": [c_float, c_int, fp, fp, fp, fp],
         "esrrf_active_samples": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "esrrf_plan_rolling": [c_int, c_int, c_int, c_int, c_int, ip],
@@ -245,6 +246,13 @@ def bic_constants(fwhm: float):
     out = [ctypes.c_float() for _ in range(4)]
     check(lib().esrrf_bic_constants(fwhm, *[ctypes.byref(o) for o in out]))
     return dict(zip(("sigma", "radius", "tSS", "tSO"), (o.value for o in out)))
+
+
+def bic_tso_text(fwhm: float) -> str:
+    """The $TWOSIGpONE$ text LiveSRRF_CL.java:318 bakes into the BIC/3-D kernel."""
+    buf = ctypes.create_string_buffer(64)
+    check(lib().esrrf_bic_tso_text(fwhm, buf, 64))
+    return buf.value.decode()
 
 
 def fht_constants(fwhm: float, magnification: int):

@@ -22,6 +22,7 @@
 #include "bic_launch.h"
 #include "esrrf_internal.h"
 #include "geometry.h"
+#include "java_text.h"
 #include "modes.h"
 
 using namespace esrrf;
@@ -367,7 +368,9 @@ int esrrf_fastest_device(int *device)
 
 int esrrf_bic_constants(float fwhm, float *sigma, float *radius, float *tSS, float *tSO)
 {
-    // LiveSRRF_CL.java:304-305,317-318 (float arithmetic, left to right)
+    // LiveSRRF_CL.java:304-305,317-318 (float arithmetic, left to right).  `"" + 2 * sigma
+    // + 1` (:318) is string concatenation: the kernel reads tSO from the text
+    // Float.toString(2*sigma) + "1" as a double literal and casts it to float.
     volatile float s = fwhm / 2.354f;
     volatile float g2s = 2.0f * 2.0f;
     g2s = g2s * s;
@@ -375,12 +378,23 @@ int esrrf_bic_constants(float fwhm, float *sigma, float *radius, float *tSS, flo
     r = r + 1.0f;
     volatile float tss = 2.0f * s;
     tss = tss * s;
-    volatile float tso = 2.0f * s;
-    tso = tso + 1.0f;
+    volatile float two_s = 2.0f * s;
+    const float tso = java_two_sigma_plus_one(two_s);
     if (sigma) *sigma = s;
     if (radius) *radius = r;
     if (tSS) *tSS = tss;
     if (tSO) *tSO = tso;
+    return ESRRF_OK;
+}
+
+int esrrf_bic_tso_text(float fwhm, char *buf, int len)
+{
+    if (!buf || len <= 0) return ESRRF_ERR_INVALID;
+    volatile float s = fwhm / 2.354f;
+    volatile float two_s = 2.0f * s;
+    const std::string t = java_float_to_string(two_s) + "1";
+    if ((int)t.size() >= len) return ESRRF_ERR_INVALID;
+    snprintf(buf, (size_t)len, "%s", t.c_str());
     return ESRRF_OK;
 }
 

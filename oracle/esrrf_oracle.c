@@ -28,6 +28,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -35,6 +36,110 @@
 #endif
 
 #include "esrrf_oracle.h"
+
+/* ------------------------------------------------------------------------- */
+/* Java's Float.toString and the OpenCL literal read-back (LiveSRRF_CL.java:318) */
+/* ------------------------------------------------------------------------- */
+
+/* value = dig * 10^e */
+static float dec_as_float(unsigned long long dig, int e)
+{
+    char b[48];
+    snprintf(b, sizeof b, "%llue%d", dig, e);
+    return strtof(b, NULL);
+}
+
+static int dec_len(unsigned long long v)
+{
+    int n = 1;
+    while (v >= 10) { v /= 10; n++; }
+    return n;
+}
+
+/* Float.toString(x) as the Java SE specification states it (java.lang.Double.toString,
+ * JDK 19+ wording): among the decimals that round to x, those of minimal length n (and of
+ * length 2 as well when n == 1); the one closest to x, even last digit on a tie; written
+ * plainly for 1e-3 <= |x| < 1e7, otherwise as d.dddE<n>; always a digit after the point.
+ * Assumes the "C" numeric locale (the oracle runs in test processes). */
+void orc_java_float_to_string(float x, char *out, int len)
+{
+    if (isnan(x)) { snprintf(out, len, "NaN"); return; }
+    if (isinf(x)) { snprintf(out, len, x > 0 ? "Infinity" : "-Infinity"); return; }
+    if (x == 0.0f) { snprintf(out, len, signbit(x) ? "-0.0" : "0.0"); return; }
+    const float a = fabsf(x);
+    unsigned long long cd[64];
+    int ce[64], nc = 0;
+    for (int q = 1; q <= 9; q++) {
+        char b[64];
+        snprintf(b, sizeof b, "%.*e", q - 1, (double)a);
+        unsigned long long D = 0;
+        const char *c = b;
+        for (; *c && *c != 'e'; c++)
+            if (*c >= '0' && *c <= '9') D = D * 10 + (unsigned)(*c - '0');
+        int e = atoi(c + 1) - (q - 1);
+        unsigned long long p10 = 1;
+        for (int k = 1; k < q; k++) p10 *= 10;
+        unsigned long long trial[4] = {D, D + 1, D - 1, p10 * 10 - 1};
+        int te[4] = {e, e, e, e - 1};
+        for (int t = 0; t < 4; t++) {
+            if (t == 2 && D <= 1) continue;
+            if (t == 3 && D != p10) continue;
+            unsigned long long v = trial[t];
+            int ve = te[t];
+            while (v % 10 == 0) { v /= 10; ve++; }
+            if (dec_as_float(v, ve) != a) continue;
+            cd[nc] = v;
+            ce[nc] = ve;
+            nc++;
+        }
+    }
+    int n = 100;
+    for (int i = 0; i < nc; i++) if (dec_len(cd[i]) < n) n = dec_len(cd[i]);
+    int nmax = n == 1 ? 2 : n, bi = -1;
+    long double bd = 0;
+    for (int i = 0; i < nc; i++) {
+        int l = dec_len(cd[i]);
+        if (l < n || l > nmax) continue;
+        char b[48];
+        snprintf(b, sizeof b, "%llue%d", cd[i], ce[i]);
+        long double dist = fabsl(strtold(b, NULL) - (long double)a);
+        if (bi < 0 || dist < bd || (dist == bd && cd[i] % 2 == 0 && cd[bi] % 2 != 0)) {
+            bi = i;
+            bd = dist;
+        }
+    }
+    char ds[32];
+    snprintf(ds, sizeof ds, "%llu", cd[bi]);
+    int nd = (int)strlen(ds), ex = ce[bi] + nd - 1;
+    char body[64];
+    if (ex >= -3 && ex <= 6) {
+        if (ex >= 0) {
+            char ip[32], fp[32];
+            int k;
+            for (k = 0; k <= ex; k++) ip[k] = k < nd ? ds[k] : '0';
+            ip[k] = 0;
+            if (ex + 1 < nd) snprintf(fp, sizeof fp, "%s", ds + ex + 1);
+            else snprintf(fp, sizeof fp, "0");
+            snprintf(body, sizeof body, "%s.%s", ip, fp);
+        } else {
+            snprintf(body, sizeof body, "0.%.*s%s", -ex - 1, "000000", ds);
+        }
+    } else {
+        snprintf(body, sizeof body, "%c.%sE%d", ds[0], nd > 1 ? ds + 1 : "0", ex);
+    }
+    snprintf(out, len, "%s%s", x < 0 ? "-" : "", body);
+}
+
+/* tSO of the BIC and 3-D kernels: `"" + 2 * sigma + 1` (LiveSRRF_CL.java:318) is the
+ * text Float.toString(2*sigma) + "1", read by `(float) tSO` (liveSRRF.cl:291) as a double
+ * literal cast to float. */
+float orc_java_tso(float two_sigma)
+{
+    char t[80];
+    orc_java_float_to_string(two_sigma, t, (int)sizeof t - 2);
+    strcat(t, "1");
+    return (float)strtod(t, NULL);
+}
 
 /* ------------------------------------------------------------------------- */
 /* Constants (LiveSRRF_CL.java:304-305,317-320 and :491-500)                  */
@@ -49,9 +154,10 @@ void orc_bic_constants(float fwhm, int gradientMag, float *sigma, float *radius,
     float gm2s = (float)gradientMag * 2.0f;
     gm2s = gm2s * s;
     float r = ((float)((int)gm2s)) / (float)gradientMag + 1.0f;
-    /* "" + 2*sigma*sigma  and  "" + 2*sigma + 1                    :317-318 */
+    /* "" + 2*sigma*sigma is numeric (multiplication first)            :317 */
     float tss = (2.0f * s) * s;
-    float tso = 2.0f * s + 1.0f;
+    /* "" + 2*sigma + 1 concatenates: Float.toString(2*sigma) + "1"     :318 */
+    float tso = orc_java_tso(2.0f * s);
     if (sigma) *sigma = s;
     if (radius) *radius = r;
     if (tSS) *tSS = tss;

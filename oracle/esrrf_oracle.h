@@ -33,6 +33,8 @@ typedef struct {
     int intWeighting;
 } orc_3d_cfg;
 
+void orc_java_float_to_string(float x, char *out, int len);
+float orc_java_tso(float two_sigma);
 void orc_bic_constants(float fwhm, int gradientMag, float *sigma, float *radius, float *tSS,
                        float *tSO);
 void orc_fht_constants(float fwhm, int magnification, float *sigma, float *radius, float *tSS,

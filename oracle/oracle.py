@@ -65,6 +65,10 @@ def lib():
     fp = ctypes.POINTER(ctypes.c_float)
     L.orc_bic_constants.argtypes = [ctypes.c_float, ctypes.c_int, fp, fp, fp, fp]
     L.orc_fht_constants.argtypes = [ctypes.c_float, ctypes.c_int, fp, fp, fp, fp]
+    L.orc_java_float_to_string.argtypes = [ctypes.c_float, ctypes.c_char_p, ctypes.c_int]
+    L.orc_java_float_to_string.restype = None
+    L.orc_java_tso.argtypes = [ctypes.c_float]
+    L.orc_java_tso.restype = ctypes.c_float
     L.orc_cubic.argtypes = [ctypes.c_float]
     L.orc_cubic.restype = ctypes.c_float
     L.orc_interp.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float]
@@ -108,6 +112,13 @@ def bic_constants(fwhm: float, gradient_mag: int = 2):
     lib().orc_bic_constants(ctypes.c_float(fwhm), gradient_mag, *[ctypes.byref(o) for o in out])
     s, r, tss, tso = (o.value for o in out)
     return dict(sigma=s, radius=r, tSS=tss, tSO=tso)
+
+
+def java_float_to_string(x: float) -> str:
+    """Float.toString of a float32 (the oracle's restatement, esrrf_oracle.c)."""
+    buf = ctypes.create_string_buffer(64)
+    lib().orc_java_float_to_string(ctypes.c_float(x), buf, 64)
+    return buf.value.decode()
 
 
 def fht_constants(fwhm: float, magnification: int):

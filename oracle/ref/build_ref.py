@@ -15,6 +15,7 @@ import json
 import os
 import subprocess
 import sys
+from fractions import Fraction
 
 import numpy as np
 
@@ -39,10 +40,32 @@ CONFIGS = {
 
 
 def jf(x) -> str:
-    """Java Float.toString of a float32 (shortest round-trip digits, at least one
-    decimal)."""
-    s = str(np.float32(x))
-    return s if ("." in s or "e" in s or "inf" in s or "nan" in s) else s + ".0"
+    """Java Float.toString of a finite float32: the shortest round-trip digits (numpy's
+    Dragon4 in unique mode), written as ddd.ddd for 1e-3 <= |x| < 1e7 and as d.dddE<n>
+    otherwise, always with a digit after the point."""
+    x = np.float32(x)
+    if x == 0:
+        return "-0.0" if np.signbit(x) else "0.0"
+    sci = np.format_float_scientific(x, unique=True, trim="-", exp_digits=1)
+    if len(sci.split("e")[0].lstrip("-").replace(".", "")) == 1:
+        # one digit suffices: Java picks the closest of the one- and two-digit decimals
+        # that round to x (Float.MIN_VALUE prints 1.4E-45)
+        two = np.format_float_scientific(x, precision=1, unique=False, exp_digits=1)
+        if np.float32(two) == x and abs(Fraction(two) - Fraction(float(x))) < \
+                abs(Fraction(sci) - Fraction(float(x))):
+            sci = two
+    mant, ex = sci.split("e")
+    sign = "-" if mant.startswith("-") else ""
+    digits = mant.lstrip("-").replace(".", "")
+    ex = int(ex)
+    if -3 <= ex <= 6:
+        if ex >= 0:
+            ip = (digits[:ex + 1]).ljust(ex + 1, "0")
+            fp = digits[ex + 1:] or "0"
+        else:
+            ip, fp = "0", "0" * (-ex - 1) + digits
+        return f"{sign}{ip}.{fp}"
+    return f"{sign}{digits[0]}.{digits[1:] or '0'}E{ex}"
 
 
 def defines(mode, w, h, nP, M, fwhm, S, nF, iw):
@@ -54,7 +77,8 @@ def defines(mode, w, h, nP, M, fwhm, S, nF, iw):
         d.update({
             "MAGNIFICATION": str(M), "SENSITIVITY": jf(S), "GXGYMAGNIFICATION": "2",
             "TWOSIGSQUARE": jf(f32(f32(2) * sigma) * sigma),
-            "TWOSIGpONE": jf(f32(f32(2) * sigma) + f32(1)), "RADIUS": jf(radius),
+            # `"" + 2 * sigma + 1` (LiveSRRF_CL.java:318) concatenates strings
+            "TWOSIGpONE": jf(f32(2) * sigma) + "1", "RADIUS": jf(radius),
             "WIDTH": str(w), "HEIGHT": str(h), "WH": str(w * h), "WM": str(w * M),
             "HM": str(h * M), "WHM": str(w * h * M * M), "WINT": str(2 * w),
             "HINT": str(2 * h), "WHINT": str(4 * w * h), "VXY_OFFSET": "0.5",

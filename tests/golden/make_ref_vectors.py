@@ -5,7 +5,9 @@ the AMD OpenCL runtime with LiveSRRF_CL's enqueue order (oracle/refrun.py).
 
 Needs an MI355X and a built oracle/_ref/:
 
-    python tests/golden/make_ref_vectors.py
+    python tests/golden/make_ref_vectors.py [OUT_DIR]
+
+(OUT_DIR defaults to tests/golden/.)
 
 One ref_<config>.npz per configuration holds the input stack exactly as the kernels saw it
 (float32; FHT: the upsampled frames and the raw frames; 3-D: the reshaped planes), the
@@ -25,6 +27,8 @@ sys.path.insert(0, REPO)
 
 
 def main():
+    out_dir = sys.argv[1] if len(sys.argv) > 1 else HERE
+    os.makedirs(out_dir, exist_ok=True)
     import __graft_entry__
     __graft_entry__.load()
     from nanoj_esrrf_amd import synth
@@ -61,7 +65,7 @@ def main():
             z = np.zeros(nP)
             out, mp, _ = refrun.run(name, planes, st3d=refrun.st3d_of(z, z, z))
             rec.update(planes=planes, out=out, mp=mp)
-        path = os.path.join(HERE, f"ref_{name}.npz")
+        path = os.path.join(out_dir, f"ref_{name}.npz")
         np.savez_compressed(path, **rec)
         print(f"wrote {path} ({os.path.getsize(path)} bytes)", flush=True)
 

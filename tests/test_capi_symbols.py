@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 from conftest import REPO
 
 
@@ -31,6 +33,8 @@ def test_host_only_entry_points(pkg):
     # pure host computations of the ABI (no device touched)
     k = pkg._capi.bic_constants(1.5)
     assert abs(k["radius"] - 2.0) < 1e-7
-    assert abs(k["tSO"] - 2.2744265) < 1e-6
+    # Float.toString(2*sigma) + "1" (LiveSRRF_CL.java:318), read as a float
+    assert k["tSO"] == float(np.float32(1.27442651))
+    assert pkg._capi.bic_tso_text(1.5) == "1.27442651"
     f = pkg._capi.fht_constants(1.5, 5)
     assert f["radius"] == 10.0

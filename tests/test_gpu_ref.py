@@ -6,6 +6,8 @@ pin the CPU oracle, and the HIP engine is checked against them directly.
 Tolerance: per channel, normalised by the reference image's max |value|,
 |a - b| <= 1e-5 + 1e-4 |b|.  Integer-valued gradients: exact.
 """
+import zlib
+
 import numpy as np
 import pytest
 
@@ -44,7 +46,8 @@ D3 = [k for k, v in refrun.configs().items() if v[0] == "3d"]
 def test_bic_oracle_and_engine_match_reference_kernels(pkg, O, name):
     cfg = refrun.configs()[name]
     _, w, h, _, M, fwhm, S, nF, iw = cfg
-    st = stack(cfg, seed=hash(name) % 1000)
+    # a fixed seed per configuration (str hashes change from process to process)
+    st = stack(cfg, seed=zlib.crc32(name.encode()) % 1000)
     loads = [4, 4, 3] if name.endswith("loads") else [nF]
     ref, ref_mp, gint = refrun.run(name, st.astype(np.float32), loads=loads, want_gint=True)
     # oracle

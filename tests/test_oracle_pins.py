@@ -19,19 +19,108 @@ f32 = np.float32
 # ---------------------------------------------------------------------------------------
 # Constants (LiveSRRF_CL.java:304-305,317-320; :491-500) and Appendix-B values
 # ---------------------------------------------------------------------------------------
-@pytest.mark.parametrize("fwhm,sigma,radius,tSO,tSS", [
-    (1.0, 0.424809, 1.5, 1.84962, 0.36093), (1.5, 0.637213, 2.0, 2.27443, 0.81208),
-    (2.0, 0.849618, 2.5, 2.69924, 1.44370), (2.5, 1.062022, 3.0, 3.12404, 2.25578),
-    (3.0, 1.274426, 3.5, 3.54885, 3.24833)])
-def test_bic_constants(O, fwhm, sigma, radius, tSO, tSS):
+# tSO: `"" + 2 * sigma + 1` (LiveSRRF_CL.java:318) is Java string concatenation, so the
+# kernel text is Float.toString(2*sigma) followed by "1"; the texts below are typed from
+# Java's rules (JLS 15.18.1, Float.toString), not computed by the code under test.
+@pytest.mark.parametrize("fwhm,sigma,radius,tso_text,tSS", [
+    (1.0, 0.424809, 1.5, "0.849617661", 0.36093), (1.5, 0.637213, 2.0, "1.27442651", 0.81208),
+    (2.0, 0.849618, 2.5, "1.69923531", 1.44370), (2.5, 1.062022, 3.0, "2.12404421", 2.25578),
+    (3.0, 1.274426, 3.5, "2.5488531", 3.24833)])
+def test_bic_constants(O, pkg, fwhm, sigma, radius, tso_text, tSS):
     k = O.bic_constants(fwhm)
     s = f32(fwhm) / f32(2.354)
     assert k["sigma"] == float(s)
     assert k["radius"] == float(f32(int(f32(4.0) * s)) / f32(2) + f32(1))
     assert k["tSS"] == float(f32(f32(2) * s) * s)
-    assert k["tSO"] == float(f32(f32(2) * s) + f32(1))
     assert abs(k["sigma"] - sigma) < 1e-6 and k["radius"] == radius
-    assert abs(k["tSO"] - tSO) < 1e-5 and abs(k["tSS"] - tSS) < 1e-5
+    assert abs(k["tSS"] - tSS) < 1e-5
+    # (float) of the double literal, as `(float) tSO` (liveSRRF.cl:291) reads it
+    assert k["tSO"] == float(f32(float(tso_text)))
+    assert O.java_float_to_string(f32(2) * s) + "1" == tso_text
+    assert pkg._capi.bic_tso_text(fwhm) == tso_text
+    assert pkg._capi.bic_constants(fwhm)["tSO"] == k["tSO"]
+
+
+# Float.toString cases fixed by the Java SE specification (decimal form for 1e-3 <= |x| <
+# 1e7, computerized scientific notation otherwise, one- and two-digit candidates when one
+# digit suffices, at least one fraction digit).
+@pytest.mark.parametrize("x,text", [
+    (2.0, "2.0"), (100.0, "100.0"), (0.1, "0.1"), (0.5, "0.5"), (1e-3, "0.001"),
+    (1e-4, "1.0E-4"), (1e7, "1.0E7"), (9999999.0, "9999999.0"), (1234567.8, "1234567.8"),
+    (1.4e-45, "1.4E-45"), (3.4028235e38, "3.4028235E38"), (1.17549435e-38, "1.1754944E-38"),
+    (-2.5e-5, "-2.5E-5"), (0.0, "0.0"), (-0.0, "-0.0"), (1.0 / 3.0, "0.33333334")])
+def test_java_float_to_string_cases(O, x, text):
+    assert O.java_float_to_string(f32(x)) == text
+
+
+def _java_text_numpy(x):
+    """Float.toString through numpy's Dragon4 shortest digits -- an implementation
+    independent of the oracle's and the engine's -- with Java's layout and its rule for
+    one-digit decimals."""
+    from fractions import Fraction
+    x = f32(x)
+    sci = np.format_float_scientific(x, unique=True, trim="-", exp_digits=1)
+    if len(sci.split("e")[0].lstrip("-").replace(".", "")) == 1:
+        two = np.format_float_scientific(x, precision=1, unique=False, exp_digits=1)
+        if f32(two) == x and abs(Fraction(two) - Fraction(float(x))) < \
+                abs(Fraction(sci) - Fraction(float(x))):
+            sci = two
+    mant, ex = sci.split("e")
+    sign = "-" if mant.startswith("-") else ""
+    dg = mant.lstrip("-").replace(".", "")
+    ex = int(ex)
+    if -3 <= ex <= 6:
+        if ex >= 0:
+            return f"{sign}{dg[:ex + 1].ljust(ex + 1, '0')}.{dg[ex + 1:] or '0'}"
+        return f"{sign}0.{'0' * (-ex - 1)}{dg}"
+    return f"{sign}{dg[0]}.{dg[1:] or '0'}E{ex}"
+
+
+def test_java_float_to_string_matches_dragon4(O, pkg):
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([
+        (f32(2) * (rng.uniform(0.3, 12.0, 3000).astype(f32) / f32(2.354))).astype(f32),
+        f32(10.0) ** rng.uniform(-44, 38, 3000).astype(f32),
+        np.array([f32(2.0) ** k for k in range(-149, 128)], f32)])
+    for x in xs:
+        if not np.isfinite(x) or x == 0:
+            continue
+        assert O.java_float_to_string(x) == _java_text_numpy(x), x
+    # the engine's restatement, through the fwhm it is asked about
+    for fwhm in np.round(np.arange(0.5, 6.0, 0.01), 2):
+        s = f32(fwhm) / f32(2.354)
+        txt = _java_text_numpy(f32(2) * s) + "1"
+        assert pkg._capi.bic_tso_text(float(fwhm)) == txt, fwhm
+        assert pkg._capi.bic_constants(float(fwhm))["tSO"] == float(f32(float(txt))), fwhm
+
+
+def test_bic_gated_samples_production_geometry(O):
+    """The production geometry (256x256, M=5, fwhm 1.5): with tSO = 1.27442651 the gate of
+    liveSRRF.cl:291 keeps about 20.3 samples per output pixel (3.33e7 per frame), against
+    64.6 for a gate of 2*sigma+1 -- counted by brute force over the reference's window."""
+    k = O.bic_constants(1.5)
+    M, w, h = 5, 256, 256
+    n = 0
+    r2 = int(f32(2) * f32(k["radius"]))
+    r2hi = int(f32(2) * f32(k["radius"]) + f32(1))
+    # phase geometry of interior pixels (every vx - xc is exact for X >= 8)
+    for sy in range(M):
+        for sx in range(M):
+            xM, yM = 16 * M + sx, 16 * M + sy
+            xc = f32((xM + 0.5) / M)
+            yc = f32((yM + 0.5) / M)
+            bx, by = int(f32(2) * xc), int(f32(2) * yc)
+            for j in range(-r2, r2hi + 1):
+                vy = (f32(by) + f32(j)) / f32(2)
+                for i in range(-r2, r2hi + 1):
+                    vx = (f32(bx) + f32(i)) / f32(2)
+                    dx, dy = f32(vx - xc), f32(vy - yc)
+                    d = np.sqrt(f32(f32(dx * dx) + f32(dy * dy)))
+                    if d != 0 and d <= f32(k["tSO"]):
+                        n += 1
+    per_px = n / (M * M)
+    assert 20.0 < per_px < 20.6, per_px
+    assert abs(per_px * w * h * M * M - 3.33e7) < 0.01e7
 
 
 def test_fht_constants(O):
