@@ -4,9 +4,12 @@ Headline (the JSON line's `value`): the production 2-D workload -- 1000 frames o
 uint16 (synthetic blinking filaments, 2000 emitters on 40 filaments, background 200),
 magnification 5, fwhm ("radius") 1.5, sensitivity 2, 1000 frames per SR frame, intensity
 weighting and macro-pixel correction on, BIC interpolation -> one 1280x1280 float32 SR image
-(AVG, VAR, TAC2, INT).  A step = resetFramePosition + calculateSRRF over the 1000 frames
-already resident in HBM + readSRRFbuffer's finalisation (VAR, MP map, MP correction, NaN
-scrub) into a device buffer.  Metric: output Mpx/s = 1280*1280*1000 / step time / 1e6.
+(AVG, VAR, TAC2, INT).  A step = resetFramePosition + calculateSRRF on the 1000 uint16
+frames in (pinned) host memory -- the upload overlapped with the kernels -- + readSRRFbuffer
+(VAR, MP map, MP correction, NaN scrub, copy to host memory): the time from the first
+host-to-device copy to the result's device-to-host copy.  Metric: output Mpx/s =
+1280*1280*1000 / step time / 1e6.  Also reported: the same step with frames and result
+resident in HBM, and with pageable host memory.
 
 `roofline` covers the RGC kernel: algorithmic FLOPs (14 per gated RGC sample + 42 per output
 pixel-frame) over its HIP-event-timed duration on the launch stream; `traffic` is its HBM
@@ -45,10 +48,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=NF, help="frames per SR frame (default 1000)")
-    ap.add_argument("--cpu-sample-frames", type=int, default=96)
+    ap.add_argument("--cpu-sample-frames", type=int, default=240)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-rse", action="store_true")
+    ap.add_argument("--upload-chunks", type=int, default=0,
+                    help="esrrf_set_upload_chunks for the headline (0: time 1, 2, 4, 8 and "
+                         "keep the fastest)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default RGC launch shape instead of esrrf_autotune's")
     ap.add_argument("--mode", choices=("bic", "fht", "3d", "rolling"), default="bic",
@@ -74,12 +80,25 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 3}))
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(stack, nframes):
-    """Oracle (C restatement of liveSRRF.cl, OpenMP) on the first `nframes` frames."""
+    """Oracle (C restatement of liveSRRF.cl, OpenMP) on the first `nframes` frames of the
+    headline stack, on every host thread this process may use (OMP_NUM_THREADS: the CPU
+    share of one GPU on a shared node)."""
     import numpy as np
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc))
     o = O.OracleLiveSRRF(W, H, M, FWHM, SENS, nframes, nframes, True, True, False,
                          nthreads=threads)
     sub = stack[:nframes].astype(np.float32)
@@ -88,9 +107,12 @@ def cpu_baseline(stack, nframes):
     o.readSRRFbuffer()
     dt = time.perf_counter() - t0
     return {"value": round(W * M * H * M * nframes / dt / 1e6, 3), "unit": "Mpx/s",
-            "cores": threads, "kind": "port",
-            "sample": f"{nframes} of the 1000 frames (256x256, M=5, fwhm 1.5, S=2, BIC, MP on); "
-                      f"{dt:.1f} s on {threads} host threads"}
+            "cores": threads, "kind": "port", "label": "restatement, not reference",
+            "nproc": nproc, "cpu_model": cpu_model(),
+            "sample": f"frames 0-{nframes - 1} of the 1000-frame headline stack (256x256, "
+                      f"M=5, fwhm 1.5, S=2, BIC, VAR, MP on), upload excluded; {dt:.1f} s on "
+                      f"{threads} OpenMP threads; per-frame cost is constant, so the rate "
+                      f"holds for the full stack"}
 
 
 def load_traffic():
@@ -419,7 +441,23 @@ def main():
         eng.check(L.esrrf_calculate_device(h, fptr, C.ESRRF_U16, nf, None, None))
         eng.check(L.esrrf_read_result_device(h, optr))
 
+    # the headline step: host uint16 frames (pinned) -> esrrf_calculate (upload overlapped
+    # with the kernels) -> esrrf_read_result into host memory (synchronous)
+    h_frames = C.HostBuffer(stack.nbytes)          # esrrf_host_alloc: page-locked
+    h_frames.array(np.uint16, stack.shape)[...] = stack
+    h_out = C.HostBuffer(4 * 4 * W * M * H * M)
+    hptr = h_frames.ptr
+    houtp = ctypes.cast(h_out.ptr, ctypes.POINTER(ctypes.c_float))
+    pageable = np.ascontiguousarray(stack)
+    pg_out = np.empty(4 * W * M * H * M, np.float32)
+
+    def step_host(frames_ptr=hptr, out_ptr=houtp):
+        eng.check(L.esrrf_reset_frame_position(h))
+        eng.check(L.esrrf_calculate(h, frames_ptr, C.ESRRF_U16, nf, None, None))
+        eng.check(L.esrrf_read_result(h, out_ptr))
+
     for _ in range(a.warmup):
+        step_host()
         step()
     torch.cuda.synchronize()
     # pick the RGC launch shape on this load (esrrf_autotune), as a deployment would once
@@ -439,8 +477,9 @@ def main():
                  "taps_in_lds": sh[2].value, "frames_per_pass": sh[3].value}
         if tuned:
             shape.update(tuned)
+
+    # device-resident rate and the kernel timings (HIP events on the launch stream)
     eng.check(L.esrrf_profile_enable(h, 1))
-    t0 = time.perf_counter()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
@@ -448,8 +487,7 @@ def main():
         step()
     ev1.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = ev0.elapsed_time(ev1)
+    ms_dev = ev0.elapsed_time(ev1) / a.steps
     rgc_ms, rgc_n = ctypes.c_double(), ctypes.c_long()
     eng.check(L.esrrf_profile_read(h, 0, ctypes.byref(rgc_ms), ctypes.byref(rgc_n)))
     prep_ms, prep_n = ctypes.c_double(), ctypes.c_long()
@@ -459,7 +497,38 @@ def main():
     eng.check(L.esrrf_profile_enable(h, 0))
     kname = eng.rgc_kernel()
 
-    ms_step = ms_total / a.steps
+    # upload pipelining depth on this box (a few steps each), then the headline with the
+    # fastest
+    chunk_ms = {}
+    if a.upload_chunks > 0:
+        eng.check(L.esrrf_set_upload_chunks(h, a.upload_chunks))
+    elif not fht:
+        for ch in (1, 2, 4, 8):
+            eng.check(L.esrrf_set_upload_chunks(h, ch))
+            step_host()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                step_host()
+            chunk_ms[ch] = round((time.perf_counter() - t0) * 1e3 / 2, 3)
+        best_ch = min(chunk_ms, key=chunk_ms.get)
+        eng.check(L.esrrf_set_upload_chunks(h, best_ch))
+    step_host()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step_host()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / a.steps
+    # pageable host memory (a plain Java/numpy array), two steps
+    step_host(pageable.ctypes.data_as(ctypes.c_void_p),
+              pg_out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    t0 = time.perf_counter()
+    for _ in range(2):
+        step_host(pageable.ctypes.data_as(ctypes.c_void_p),
+                  pg_out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    ms_pageable = (time.perf_counter() - t0) * 1e3 / 2
+
     outpx = W * M * H * M
     value = outpx * nf / (ms_step / 1e3) / 1e6
     rgc_avg = rgc_ms.value / max(rgc_n.value, 1)
@@ -499,8 +568,20 @@ def main():
                      if fht else None,
                      "launch_shape": shape},
         "wall_s": round(wall, 3),
+        "timed_region": "per step: resetFramePosition, esrrf_calculate from page-locked "
+                        "host uint16 frames (esrrf_host_alloc; H2D overlapped with the "
+                        "kernels), esrrf_read_result into page-locked host memory (D2H, "
+                        "synchronous)",
+        "device_resident": {"value": round(outpx * nf / (ms_dev / 1e3) / 1e6, 1),
+                            "ms_per_step": round(ms_dev, 3),
+                            "note": "frames already in HBM, result left in HBM"},
+        "pageable_host": {"value": round(outpx * nf / (ms_pageable / 1e3) / 1e6, 1),
+                          "ms_per_step": round(ms_pageable, 3)},
+        "upload_chunks_ms_per_step": chunk_ms or None,
     }
     eng.close()
+    h_frames.close()
+    h_out.close()
     del d_frames, d_out
     if not a.no_cpu_baseline and not fht:
         res["cpu_baseline"] = cpu_baseline(stack, min(a.cpu_sample_frames, nf))

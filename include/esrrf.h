@@ -13,6 +13,8 @@
 #ifndef ESRRF_H
 #define ESRRF_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -109,6 +111,17 @@ int esrrf_finish(esrrf_engine *e);
 
 /* Run on the caller's HIP stream (NULL restores the engine's own stream). */
 int esrrf_set_stream(esrrf_engine *e, void *hip_stream);
+/* esrrf_calculate (BIC mode, no cancel flag) splits a load of host frames into up to
+ * `chunks` (1..16) parts of at least 64 frames and copies part c+1 while part c is
+ * computed.  No counterpart: the reference queues one write of the whole load ahead of
+ * the kernels on its in-order queue (LiveSRRF_CL.java:655-661).  1 = one copy, then the
+ * kernels.  Default 4. */
+int esrrf_set_upload_chunks(esrrf_engine *e, int chunks);
+/* Page-locked host memory from the runtime the engine copies with, for frame and result
+ * buffers that upload and read back at the link's full rate (no counterpart: the reference
+ * copies from the JVM heap).  Pageable memory works everywhere, more slowly. */
+int esrrf_host_alloc(size_t bytes, void **ptr);
+int esrrf_host_free(void *ptr);
 /* Kernel timing with HIP events on the launch stream (NanoJProfiler equivalent,
  * LiveSRRF_CL.java:67).  which: 0 = RGC kernel, 1 = gradient prep, 2 = upsampling. */
 int esrrf_profile_enable(esrrf_engine *e, int enable);

@@ -26,6 +26,7 @@ EXPORTS = [
     "esrrf_calculate_device", "esrrf_read_result", "esrrf_read_result_device",
     "esrrf_read_gradients", "esrrf_read_aligned", "esrrf_read_mp_maps",
     "esrrf_read_frame_counters", "esrrf_get_dims", "esrrf_finish", "esrrf_set_stream",
+    "esrrf_set_upload_chunks", "esrrf_host_alloc", "esrrf_host_free",
     "esrrf_profile_enable", "esrrf_profile_read", "esrrf_bic_constants",
     "esrrf_bic_tso_text", "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
     "esrrf_last_error", "esrrf_version", "esrrf_autotune", "esrrf_launch_shape",
@@ -152,6 +153,9 @@ def lib():
         "esrrf_get_dims": [vp, ip, ip, ip, ip, ip, ip],
         "esrrf_finish": [vp],
         "esrrf_set_stream": [vp, vp],
+        "esrrf_set_upload_chunks": [vp, c_int],
+        "esrrf_host_alloc": [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
+        "esrrf_host_free": [vp],
         "esrrf_profile_enable": [vp, c_int],
         "esrrf_profile_read": [vp, c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_long)],
@@ -246,6 +250,33 @@ def bic_constants(fwhm: float):
     out = [ctypes.c_float() for _ in range(4)]
     check(lib().esrrf_bic_constants(fwhm, *[ctypes.byref(o) for o in out]))
     return dict(zip(("sigma", "radius", "tSS", "tSO"), (o.value for o in out)))
+
+
+class HostBuffer:
+    """Page-locked host memory of `nbytes` from the engine's runtime (esrrf_host_alloc),
+    viewable as a numpy array."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = ctypes.c_void_p()
+        check(lib().esrrf_host_alloc(nbytes, ctypes.byref(self.ptr)))
+        self.nbytes = nbytes
+
+    def array(self, dtype, shape) -> np.ndarray:
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        assert n <= self.nbytes
+        raw = (ctypes.c_char * n).from_address(self.ptr.value)
+        return np.frombuffer(raw, dtype=dtype).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            lib().esrrf_host_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bic_tso_text(fwhm: float) -> str:

@@ -31,6 +31,9 @@ namespace {
 
 enum Mode { MODE_BIC = 0, MODE_FHT = 1, MODE_3D = 2 };
 
+// Most upload chunks one load of host frames is split into.
+constexpr int kMaxUploadChunks = 16;
+
 struct ProfSlot {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double total_ms = 0.0;
@@ -45,6 +48,12 @@ struct esrrf_engine {
     int dev = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
+    // host-frame upload overlapped with compute: chunks go up on copy_stream, each with an
+    // event the compute stream waits for (BIC mode)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_free = nullptr;                 // staging buffer free to overwrite
+    std::vector<hipEvent_t> ev_chunk;             // chunk c resident
+    int upload_chunks = 4;
     Mode mode = MODE_BIC;
     esrrf_params p{};
     // sizes
@@ -129,6 +138,10 @@ void free_all(esrrf_engine *e)
             (void)hipEventDestroy(ev.first);
             (void)hipEventDestroy(ev.second);
         }
+    for (hipEvent_t ev : e->ev_chunk)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->ev_free) (void)hipEventDestroy(e->ev_free);
+    if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
 }
 
@@ -235,6 +248,53 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
     return ESRRF_OK;
 }
 
+// Smallest upload chunk worth its own RGC launch (frames).
+constexpr int kMinUploadChunk = 64;
+
+// calculateSRRF for host frames in BIC mode with the upload overlapped: chunk c of the load
+// is copied on copy_stream while the compute stream converts, preps and accumulates chunk
+// c-1.  Each chunk's RGC launch continues the cumulants the previous one left in d_out /
+// d_prev, as successive loads of one SR frame do, so the result equals one launch over the
+// whole load bit for bit.  Chunks are issued copy-then-compute, so pageable host memory
+// (whose copies return only when done) still overlaps the previous chunk's kernels.
+int calculate_bic_pipelined(esrrf_engine *e, const void *frames, int dtype, int n)
+{
+    const size_t esz = dtype == ESRRF_U16 ? 2 : 4;
+    const long npx = (long)e->W * e->H;
+    const int nch = std::min(e->upload_chunks, n / kMinUploadChunk);
+    int csz = (n + nch - 1) / nch;
+    csz += csz & 1;   // even chunks keep the frame-pair kernel on pairs
+    // the staging buffer may still be read by work queued earlier on the compute stream
+    HIPCHK(e, hipEventRecord(e->ev_free, e->stream));
+    HIPCHK(e, hipStreamWaitEvent(e->copy_stream, e->ev_free, 0));
+    e->counters[1] = 0;
+    e->loaded = n;
+    int c = 0;
+    for (int f0 = 0; f0 < n; f0 += csz, c++) {
+        const int k = std::min(csz, n - f0);
+        const char *hsrc = (const char *)frames + esz * npx * f0;
+        char *dst = (char *)e->d_in + esz * npx * f0;
+        HIPCHK(e, hipMemcpyAsync(dst, hsrc, esz * npx * k, hipMemcpyHostToDevice,
+                                 e->copy_stream));
+        HIPCHK(e, hipEventRecord(e->ev_chunk[c], e->copy_stream));
+        HIPCHK(e, hipStreamWaitEvent(e->stream, e->ev_chunk[c], 0));
+        HIPCHK(e, launch_convert_frames(dst, dtype, e->d_px + npx * f0, npx * k, e->stream));
+        {
+            ProfScope ps(e, 1);
+            HIPCHK(e, launch_bic_prep(e->d_px + npx * f0, e->d_U + 4 * npx * f0, e->w, e->h, k,
+                                      e->stream));
+        }
+        int rc = run_bic_rgc(e, f0, e->counters[0], k);
+        if (rc != ESRRF_OK) return rc;
+        e->counters[0] += k;
+        e->counters[1] += k;
+    }
+    // the caller may reuse its frames on return, as after LiveSRRF_CL.java:711's finish;
+    // the kernels keep running
+    HIPCHK(e, hipStreamSynchronize(e->copy_stream));
+    return ESRRF_OK;
+}
+
 int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtype, int n,
                    const volatile int *cancel, int *cancelled)
 {
@@ -253,9 +313,13 @@ int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtyp
     const size_t esz = dtype == ESRRF_U16 ? 2 : 4;
     const long npx = (long)e->W * e->H;
     const void *src = frames;
+    if (!on_device && e->mode == MODE_BIC && !cancel && e->upload_chunks > 1 &&
+        n >= 2 * kMinUploadChunk)
+        return calculate_bic_pipelined(e, frames, dtype, n);
     if (!on_device) {
         HIPCHK(e, hipMemcpyAsync(e->d_in, frames, esz * npx * n, hipMemcpyHostToDevice,
                                  e->stream));
+        HIPCHK(e, hipEventRecord(e->ev_chunk[0], e->stream));
         src = e->d_in;
     }
     e->counters[1] = 0;   // the gradient kernel resets the load counter (liveSRRF.cl:207)
@@ -305,9 +369,11 @@ int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtyp
         e->counters[1] += k;
         if (cancel && *cancel && f0 + k < n) {
             if (cancelled) *cancelled = 1;
-            return ESRRF_OK;
+            break;
         }
     }
+    // the host frames are free for the caller on return (LiveSRRF_CL.java:711)
+    if (!on_device) HIPCHK(e, hipEventSynchronize(e->ev_chunk[0]));
     return ESRRF_OK;
 }
 
@@ -446,6 +512,13 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
     if (hipSetDevice(dev) != hipSuccess) return bad(ESRRF_ERR_HIP);
     if (hipStreamCreate(&e->own_stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
     e->stream = e->own_stream;
+    if (hipStreamCreate(&e->copy_stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
+    if (hipEventCreateWithFlags(&e->ev_free, hipEventDisableTiming) != hipSuccess)
+        return bad(ESRRF_ERR_HIP);
+    e->ev_chunk.assign(kMaxUploadChunks, nullptr);
+    for (hipEvent_t &ev : e->ev_chunk)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            return bad(ESRRF_ERR_HIP);
     const bool do3d = p->n_planes > 0;
     e->mode = do3d ? MODE_3D : (p->do_fht_interpolation ? MODE_FHT : MODE_BIC);
     e->M = p->magnification;
@@ -716,6 +789,32 @@ int esrrf_set_stream(esrrf_engine *e, void *hip_stream)
     (void)hipSetDevice(e->dev);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->stream = hip_stream ? (hipStream_t)hip_stream : e->own_stream;
+    return ESRRF_OK;
+}
+
+int esrrf_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr || bytes == 0) return ESRRF_ERR_INVALID;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        (void)hipGetLastError();
+        return ESRRF_ERR_NOMEM;
+    }
+    return ESRRF_OK;
+}
+
+int esrrf_host_free(void *ptr)
+{
+    if (!ptr) return ESRRF_OK;
+    return hipHostFree(ptr) == hipSuccess ? ESRRF_OK : ESRRF_ERR_HIP;
+}
+
+int esrrf_set_upload_chunks(esrrf_engine *e, int chunks)
+{
+    if (!e || chunks < 1 || chunks > kMaxUploadChunks) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->upload_chunks = chunks;
     return ESRRF_OK;
 }
 

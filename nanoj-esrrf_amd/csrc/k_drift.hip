@@ -137,14 +137,23 @@ __global__ void k_drift_peak(const float2 *__restrict__ buf, int N, int batch,
             bidx = i;
         }
     }
+    // a map without a finite maximum (a frame holding +-Inf makes it NaN or Inf) leaves
+    // bidx unset: no shift, and no read through it
+    if (bidx < 0 || bidx >= N * N || !isfinite(best)) {
+        sx[b] = 0.0f;
+        sy[b] = 0.0f;
+        return;
+    }
     const float2 *c = buf + (long)N * N * b;
     const int py = bidx / N, px = bidx - py * N;
     const float c0 = best;
     const float xm = c[py * N + ((px - 1) & (N - 1))].x, xp = c[py * N + ((px + 1) & (N - 1))].x;
     const float ym = c[((py - 1) & (N - 1)) * N + px].x, yp = c[((py + 1) & (N - 1)) * N + px].x;
     const float dx_den = xm - 2.0f * c0 + xp, dy_den = ym - 2.0f * c0 + yp;
-    const float fx = dx_den != 0.0f ? 0.5f * (xm - xp) / dx_den : 0.0f;
-    const float fy = dy_den != 0.0f ? 0.5f * (ym - yp) / dy_den : 0.0f;
+    float fx = dx_den != 0.0f ? 0.5f * (xm - xp) / dx_den : 0.0f;
+    float fy = dy_den != 0.0f ? 0.5f * (ym - yp) / dy_den : 0.0f;
+    if (!isfinite(fx)) fx = 0.0f;   // a non-finite neighbour: integer peak on that axis
+    if (!isfinite(fy)) fy = 0.0f;
     const int lx = px < N / 2 ? px : px - N, ly = py < N / 2 ? py : py - N;
     sx[b] = (float)lx + fx;
     sy[b] = (float)ly + fy;

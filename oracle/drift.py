@@ -47,15 +47,22 @@ def cross_correlation(ref: np.ndarray, frame: np.ndarray, N: int) -> np.ndarray:
 
 
 def peak_shift(cc: np.ndarray):
-    """(dx, dy) of the first maximum, refined by a parabola per axis."""
+    """(dx, dy) of the first maximum (NaN values never win), refined by a parabola per
+    axis.  A map whose maximum is not finite -- a frame holding +-Inf -- gives no shift;
+    a parabola through a non-finite neighbour leaves that axis on the integer peak."""
     N = cc.shape[0]
-    i = int(np.argmax(cc))
+    nn = np.where(np.isnan(cc), -np.inf, cc)
+    i = int(np.argmax(nn))
     py, px = divmod(i, N)
     c0 = cc[py, px]
+    if not np.isfinite(c0):
+        return 0.0, 0.0
 
     def fit(cm, cp):
         den = cm - 2.0 * c0 + cp
-        return 0.5 * (cm - cp) / den if den != 0.0 else 0.0
+        with np.errstate(invalid="ignore", divide="ignore"):
+            f = 0.5 * (cm - cp) / den if den != 0.0 else 0.0
+        return f if np.isfinite(f) else 0.0
 
     dx = fit(cc[py, (px - 1) % N], cc[py, (px + 1) % N])
     dy = fit(cc[(py - 1) % N, px], cc[(py + 1) % N, px])

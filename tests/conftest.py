@@ -37,3 +37,19 @@ def normalised_close(got, exp, rel=1e-4, abs_=1e-5):
     excess = np.abs(a - b) - (abs_ + rel * np.abs(b))
     i = int(np.argmax(excess))
     return bool(excess.flat[i] <= 0), float(excess.flat[i]), np.unravel_index(i, excess.shape)
+
+
+def hip_runtime():
+    """The HIP runtime libesrrf.so runs on, through ctypes: device and pinned host buffers
+    for the tests of the *_device entry points (torch ships a second HIP runtime of its
+    own, which tests keep out of the process)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
+                                  ctypes.c_uint]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipHostFree.argtypes = [ctypes.c_void_p]
+    hip.hipDeviceSynchronize.argtypes = []
+    return hip

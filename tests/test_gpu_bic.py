@@ -297,3 +297,69 @@ def test_bic_prep_shared_row_sums_match_per_point_stencils(pkg, monkeypatch, w, 
         res[v], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
                                   loads=loads)
     assert np.array_equal(res["1"], res["2"])
+
+
+@pytest.mark.parametrize("n,chunks", [(130, 1), (130, 2), (301, 4), (301, 16), (257, 3)])
+def test_pipelined_host_upload_bit_identical(pkg, n, chunks):
+    """esrrf_calculate splits a load of host frames into chunks whose upload overlaps the
+    previous chunk's kernels; each chunk's RGC launch continues the cumulants of the last.
+    The result, counters included, equals the one-copy path and the device-resident path
+    bit for bit, for pageable and pinned host memory and odd chunk boundaries."""
+    import ctypes
+    from conftest import hip_runtime
+    hip = hip_runtime()
+    C = pkg._capi
+    w, h, M = 40, 36, 5
+    st = stack(n, w, h, seed=900 + n).astype(np.uint16)
+    p = C.Params()
+    p.width, p.height, p.magnification, p.fwhm, p.sensitivity = w, h, M, 1.5, 2.0
+    p.n_frames_on_gpu, p.n_frame_for_srrf, p.block_length, p.device = n, n, 20000, 0
+    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, 1, 0
+    buf = ctypes.c_void_p()
+
+    def run(mode):
+        eng = pkg.Engine(p)
+        L, hd = eng.L, eng.h
+        out = np.zeros(4 * w * M * h * M, np.float32)
+        eng.check(L.esrrf_set_upload_chunks(hd, 1 if mode == "single" else chunks))
+        eng.check(L.esrrf_reset_frame_position(hd))
+        if mode == "device":
+            assert hip.hipMalloc(ctypes.byref(buf), st.nbytes) == 0
+            assert hip.hipMemcpy(buf, st.ctypes.data, st.nbytes, 1) == 0   # host to device
+            eng.check(L.esrrf_calculate_device(hd, buf, C.ESRRF_U16, n, None, None))
+        elif mode == "pinned":
+            hb = C.HostBuffer(st.nbytes)
+            hb.array(np.uint16, st.shape)[...] = st
+            eng.check(L.esrrf_calculate(hd, hb.ptr, C.ESRRF_U16, n, None, None))
+        else:
+            eng.check(L.esrrf_calculate(hd, st.ctypes.data_as(ctypes.c_void_p), C.ESRRF_U16,
+                                        n, None, None))
+        eng.check(L.esrrf_read_result(hd, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        cnt = eng.counters()
+        eng.close()
+        if mode == "device":
+            hip.hipFree(buf)
+        elif mode == "pinned":
+            hb.close()
+        return out, cnt
+
+    ref, rcnt = run("single")
+    assert rcnt == [n, n]
+    for mode in ("pageable", "pinned", "device"):
+        got, cnt = run(mode)
+        assert cnt == rcnt, mode
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), mode
+
+
+def test_upload_chunks_refuses_out_of_range(pkg):
+    C = pkg._capi
+    p = C.Params()
+    p.width, p.height, p.magnification, p.fwhm, p.sensitivity = 16, 16, 2, 1.5, 1.0
+    p.n_frames_on_gpu, p.n_frame_for_srrf, p.block_length, p.device = 4, 4, 20000, 0
+    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, 1, 0
+    eng = pkg.Engine(p)
+    ESRRF_ERR_INVALID = 1
+    assert eng.L.esrrf_set_upload_chunks(eng.h, 0) == ESRRF_ERR_INVALID
+    assert eng.L.esrrf_set_upload_chunks(eng.h, 17) == ESRRF_ERR_INVALID
+    assert eng.L.esrrf_set_upload_chunks(eng.h, 16) == C.ESRRF_OK
+    eng.close()

@@ -88,3 +88,17 @@ def test_vibration_correction_feeds_the_engine(pkg):
     calc.close()
     for k in got:
         assert np.array_equal(got[k], exp[k])
+
+
+def test_inf_pixel_frame_gets_no_shift(pkg):
+    """A frame holding Inf makes its correlation map NaN; the device keeps the peak index
+    in bounds and reports no shift for it, as the oracle does, and the other frames of the
+    batch are unaffected."""
+    st, _ = drifted_stack(6, 32, 32, seed=11)
+    st[3, 5, 7] = np.inf
+    sx, sy = run(pkg, st)
+    ox, oy = D.shift_array(st, 0, 6)
+    assert sx[3] == 0.0 and sy[3] == 0.0 and ox[3] == 0.0 and oy[3] == 0.0
+    keep = [0, 1, 2, 4, 5]
+    np.testing.assert_allclose(sx[keep], ox[keep], atol=2e-3)
+    np.testing.assert_allclose(sy[keep], oy[keep], atol=2e-3)

@@ -10,6 +10,8 @@ import zipfile
 import numpy as np
 import pytest
 
+from conftest import hip_runtime
+
 from conftest import normalised_close
 
 pytestmark = pytest.mark.gpu
@@ -60,17 +62,6 @@ def test_rolling_matches_engine_per_window(pkg, n, w, h, M, fwhm, S, nF, gap, ch
     with pytest.raises(pkg._capi.EsrrfError):
         r.next()
     r.close()
-
-
-def hip_runtime():
-    """The HIP runtime libesrrf.so runs on (device buffers for the *_device entry points)."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    hip.hipFree.argtypes = [ctypes.c_void_p]
-    hip.hipDeviceSynchronize.argtypes = []
-    return hip
 
 
 def test_rolling_device_stack_and_reuse(pkg):

@@ -52,3 +52,27 @@ def test_nan_pixels_read_as_zero():
     frame[0, 0] = np.nan
     gx, gy = D.peak_shift(D.cross_correlation(ref, frame, 32))
     assert abs(gx - 2) < 0.01 and abs(gy) < 0.01
+
+
+def test_inf_pixel_gives_no_shift():
+    """A frame holding +-Inf has an infinite mean, so its padded frame and the whole
+    correlation map are NaN: no finite maximum, so no shift (and no read through an unset
+    peak index)."""
+    ref = gaussian(32, 15.0, 16.0, 2.0)
+    frame = np.roll(ref, (1, 2), axis=(0, 1)).copy()
+    frame[3, 4] = np.inf
+    cc = D.cross_correlation(ref, frame, 32)
+    assert np.isnan(cc).all()
+    assert D.peak_shift(cc) == (0.0, 0.0)
+    frame[3, 4] = -np.inf
+    assert D.peak_shift(D.cross_correlation(ref, frame, 32)) == (0.0, 0.0)
+
+
+def test_non_finite_neighbour_keeps_integer_peak():
+    cc = np.zeros((8, 8))
+    cc[2, 3] = 5.0
+    cc[2, 4] = np.nan      # right neighbour along x
+    cc[1, 3], cc[3, 3] = 1.0, 3.0
+    dx, dy = D.peak_shift(cc)
+    assert dx == 3.0
+    assert abs(dy - (2 + 0.5 * (1.0 - 3.0) / (1.0 - 10.0 + 3.0))) < 1e-12
