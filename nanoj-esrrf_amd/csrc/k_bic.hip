@@ -659,14 +659,18 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
                     const float Cd = (P.wsumd + acc2) * invw[p] - C0 * rwd[p];
                     float t, dtdC, dtdS;
                     sens_derivs(C0, a.sens, a.sens_int, v, intw, t, dtdC, dtdS);
+                    // a pixel without gated samples (sum(W) = 0) has C0 = 0/0 and the
+                    // constant output 0 (liveSRRF.cl:358-360): no derivative, rather than
+                    // 0 x NaN
+                    const float dtdtss = dtdC != 0.0f ? dtdC * Cd : 0.0f;
                     if (first) {
                         s_avg[p] = t * rn;
-                        s_e2[p] = (dtdC * Cd) * rn;
+                        s_e2[p] = dtdtss * rn;
                         s_tac[p] = dtdS * rn;
                         s_int[p] = v * rn;
                     } else {
                         s_avg[p] = s_avg[p] + t * rn;
-                        s_e2[p] = s_e2[p] + (dtdC * Cd) * rn;
+                        s_e2[p] = s_e2[p] + dtdtss * rn;
                         s_tac[p] = s_tac[p] + dtdS * rn;
                         s_int[p] = s_int[p] + v * rn;
                     }

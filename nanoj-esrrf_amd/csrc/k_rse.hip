@@ -80,9 +80,17 @@ __global__ void k_rse_prep(const float *__restrict__ out4, const double *__restr
             const float m = (float)stats[s];
             const float mt = (float)stats[M * M + s], ms = (float)stats[2 * M * M + s];
             const float sr = avg / m;
-            SR[i] = sr;
-            Dt[i] = (gt - sr * mt) / m;
-            DS[i] = (gs - sr * ms) / m;
+            if (isnan(sr)) {
+                // a sub-pixel position without gated samples has m = 0: readSRRFbuffer
+                // hands the 0/0 over as 0 (LiveSRRF_CL.java:781), constant in the parameters
+                SR[i] = 0.0f;
+                Dt[i] = 0.0f;
+                DS[i] = 0.0f;
+            } else {
+                SR[i] = sr;
+                Dt[i] = (gt - sr * mt) / m;
+                DS[i] = (gs - sr * ms) / m;
+            }
         } else {
             SR[i] = avg;
             Dt[i] = gt;

@@ -111,3 +111,25 @@ def test_descent_reaches_grid_best(rse_obj):
     print("descent trajectory (fwhm, S, sigma, rse):\n", traj)
     assert traj[-1, 3] < traj[0, 3]
     assert traj[:, 3].min() <= 1.01 * best[0], (traj[:, 3].min(), best)
+
+
+@pytest.mark.parametrize("fwhm", [0.5, 0.6])
+def test_rse_with_sample_free_phases(pkg, O, data, fwhm):
+    """At M=5 and fwhm below ~0.7 the centre sub-pixel position has no gated sample (its
+    own grid point is at distance 0, the next at 0.5 px > tSO), so its MP-map entry is 0
+    and the corrected AVG is 0/0 there; readSRRFbuffer hands that over as 0
+    (LiveSRRF_CL.java:781).  The RSE and its gradient stay finite and match the oracle."""
+    M5 = 5
+    r = pkg.RSE(W, H, M5, NF, crop_size=CROP)
+    r.load(data)
+    got = r.value(fwhm, 1.5, 3.0)
+    v, g = r.value_and_grad(fwhm, 1.5, 3.0)
+    r.close()
+    o = O.OracleLiveSRRF(W, H, M5, fwhm, 1.5, NF, NF, True, True, False)
+    o.calculateSRRF(data.astype(np.float32))
+    out = o.readSRRFbuffer()
+    exp = R.rse(R.crop(out[3], M5, CROP), R.crop(out[0], M5, CROP), 3.0,
+                R.default_K(4.0 / 2.35482 * M5))
+    assert math.isfinite(got) and math.isfinite(v) and all(math.isfinite(x) for x in g)
+    assert math.isclose(got, exp, rel_tol=1e-4), (got, exp)
+    assert math.isclose(v, got, rel_tol=1e-5)
