@@ -327,6 +327,19 @@ int esrrf_frc_run_device(esrrf_frc *q, const float *d_a, const float *d_b, doubl
 int esrrf_frc_run(esrrf_frc *q, const float *a, const float *b, double *resolution,
                   float *curve);
 const char *esrrf_frc_last_error(esrrf_frc *q);
+/* No counterpart in the reference: esrrf_frc_run_device for callers that cannot allocate
+ * device memory (code under stream capture that must not call hipMalloc, tight serving
+ * loops, harnesses that forbid allocation).  No handle and no allocation: the spectra, ring
+ * sums, window and twiddles live in one static device array of about 76 MiB per GPU, held
+ * from the library's first use on that GPU to the end of the process.  Transform size N
+ * (the power of two >= min(width, height)) at most ESRRF_FRC_NOALLOC_MAX_N, otherwise
+ * ESRRF_ERR_UNSUPPORTED.  device < 0: the current device; hip_stream NULL: the null stream.
+ * curve (host) receives N/2 values when given, *n_rings N/2.  Blocking.  Calls that overlap
+ * on one GPU share the array and race: serialise them. */
+#define ESRRF_FRC_NOALLOC_MAX_N 2048
+int esrrf_frc_run_device_noalloc(int device, void *hip_stream, const float *d_a,
+                                 const float *d_b, int width, int height, double *resolution,
+                                 float *curve, int *n_rings);
 /* FRC resolution (SR pixels) of every configuration of two sweeps with identical parameters
  * run on the odd and the even frames (:403-447 with :808-816's frame mapping), for channel 0
  * AVG, 1 VAR or 2 TAC2: [n_magnifications][n_n_frames][n_sensitivity][n_fwhm]. */

@@ -43,7 +43,8 @@ EXPORTS = [
     "esrrf_rolling_begin", "esrrf_rolling_begin_device", "esrrf_rolling_next",
     "esrrf_rolling_next_device", "esrrf_rolling_last_error",
     "esrrf_frc_create", "esrrf_frc_destroy", "esrrf_frc_set_stream", "esrrf_frc_rings",
-    "esrrf_frc_run_device", "esrrf_frc_run", "esrrf_frc_last_error", "esrrf_sweep_frc",
+    "esrrf_frc_run_device", "esrrf_frc_run", "esrrf_frc_last_error",
+    "esrrf_frc_run_device_noalloc", "esrrf_sweep_frc",
     "esrrf_drift_create", "esrrf_drift_destroy", "esrrf_drift_set_stream",
     "esrrf_drift_run_device", "esrrf_drift_run", "esrrf_drift_last_error",
 ]
@@ -212,6 +213,7 @@ def lib():
         "esrrf_frc_set_stream": [vp, vp],
         "esrrf_frc_rings": [vp, ip, ip],
         "esrrf_frc_run_device": [vp, vp, vp, dp, fp],
+        "esrrf_frc_run_device_noalloc": [c_int, vp, vp, vp, c_int, c_int, dp, fp, ip],
         "esrrf_frc_run": [vp, fp, fp, dp, fp],
         "esrrf_frc_last_error": [vp],
         "esrrf_drift_create": [c_int, c_int, c_int, ctypes.POINTER(vp)],

@@ -76,6 +76,31 @@ std::vector<float> tukey(int n)
     return w;
 }
 
+}  // namespace
+
+double esrrf::frc_resolution(const double *rings, int N, float *curve)
+{
+    const int R = N / 2;
+    const double *num = rings, *d1 = rings + R, *d2 = rings + 2 * R;
+    const double t = 1.0 / 7.0;
+    double res = std::numeric_limits<double>::quiet_NaN(), prev = 0.0;
+    bool found = false;
+    for (int r = 0; r < R; r++) {
+        const double den = std::sqrt(d1[r] * d2[r]);
+        const double c = den > 0.0 ? num[r] / den : 0.0;
+        if (curve) curve[r] = (float)c;
+        if (!found && r >= 1 && c < t) {
+            const double rs = prev >= t ? (r - 1) + (prev - t) / (prev - c) : r;
+            res = (double)N / rs;
+            found = true;
+        }
+        prev = c;
+    }
+    return res;
+}
+
+namespace {
+
 int run_impl(esrrf_frc *q, const float *d_a, const float *d_b, double *resolution, float *curve,
              int *n_rings)
 {
@@ -85,22 +110,7 @@ int run_impl(esrrf_frc *q, const float *d_a, const float *d_b, double *resolutio
     FCHK(q, hipMemcpyAsync(q->h_rings.data(), p.rings, sizeof(double) * 3 * R,
                            hipMemcpyDeviceToHost, q->stream));
     FCHK(q, hipStreamSynchronize(q->stream));
-    const double *num = q->h_rings.data(), *d1 = num + R, *d2 = num + 2 * R;
-    const double t = 1.0 / 7.0;
-    std::vector<double> c(R);
-    for (int r = 0; r < R; r++) {
-        const double den = std::sqrt(d1[r] * d2[r]);
-        c[r] = den > 0.0 ? num[r] / den : 0.0;
-        if (curve) curve[r] = (float)c[r];
-    }
-    double res = std::numeric_limits<double>::quiet_NaN();
-    for (int r = 1; r < R; r++) {
-        if (c[r] < t) {
-            const double rs = c[r - 1] >= t ? (r - 1) + (c[r - 1] - t) / (c[r - 1] - c[r]) : r;
-            res = (double)p.N / rs;
-            break;
-        }
-    }
+    const double res = frc_resolution(q->h_rings.data(), p.N, curve);
     if (resolution) *resolution = res;
     if (n_rings) *n_rings = R;
     return ESRRF_OK;

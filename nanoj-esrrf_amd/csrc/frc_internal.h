@@ -20,6 +20,11 @@ struct FrcPlan {
 
 hipError_t launch_frc(const FrcPlan &p, const float *a, const float *b, hipStream_t s);
 
+// Resolution N / r* (pixels) at the first crossing r* of 1/7 after ring 0 of the curve
+// FRC(r) = num / sqrt(d1 d2) from the ring sums [3][N/2] (linear between rings; NaN when
+// the curve never crosses); the curve into `curve` (N/2 values) when given.
+double frc_resolution(const double *rings, int N, float *curve);
+
 // Forward 2-D FFT, in place, of `batch` N x N complex images at p.buf (p.N, p.log2N, p.tw).
 hipError_t launch_fft2d(const FrcPlan &p, int batch, hipStream_t s);
 
