@@ -159,6 +159,13 @@ int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_ld
  * kernel (FHT, 3-D, or a geometry outside the tiled kernel's range). */
 int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
                        int *frames_per_pass);
+/* Pin the tiled 2-D kernel's launch shape (no counterpart in the reference), e.g. one the
+ * autotuner chose earlier: nw waves per block (1..16, at most 8 sub-pixel phases per wave),
+ * tap tables in LDS (1) or through scalar loads (0), one or two frames per pass.
+ * ESRRF_ERR_UNSUPPORTED when the kernels cannot run that shape for this geometry (tables or
+ * tiles beyond their LDS budget, frame pairs with more than 4 phases per wave);
+ * ESRRF_ERR_STATE when the engine runs no tiled kernel. */
+int esrrf_set_launch_shape(esrrf_engine *e, int nw, int taps_in_lds, int frames_per_pass);
 /* Name of the RGC kernel the engine's last calculate launched, e.g. "k_bic_rgc2",
  * "k_fht_rgc2", "k3d_rgc3" ("" before the first launch).  No counterpart in the reference
  * (its single OpenCL kernel per mode needs none); for profiling and tests. */

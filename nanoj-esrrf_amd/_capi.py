@@ -30,6 +30,7 @@ EXPORTS = [
     "esrrf_profile_enable", "esrrf_profile_read", "esrrf_bic_constants",
     "esrrf_bic_tso_text", "esrrf_fht_constants", "esrrf_active_samples", "esrrf_plan_rolling", "esrrf_plan_load",
     "esrrf_last_error", "esrrf_version", "esrrf_autotune", "esrrf_launch_shape",
+    "esrrf_set_launch_shape",
     "esrrf_rgc_kernel",
     "esrrf_sweep_create", "esrrf_sweep_destroy", "esrrf_sweep_set_stream", "esrrf_sweep_run",
     "esrrf_sweep_run_device", "esrrf_sweep_read", "esrrf_sweep_read_device",
@@ -170,6 +171,7 @@ def lib():
         "esrrf_version": [],
         "esrrf_autotune": [vp, c_int, ip, ip, ip, fp],
         "esrrf_launch_shape": [vp, ip, ip, ip, ip],
+        "esrrf_set_launch_shape": [vp, c_int, c_int, c_int],
         "esrrf_rgc_kernel": [vp, ctypes.c_char_p, c_int],
         "esrrf_sweep_create": [ctypes.POINTER(SweepParams), ctypes.POINTER(vp)],
         "esrrf_sweep_destroy": [vp],

@@ -85,6 +85,15 @@ BicShape bic_default_shape(const BicTables &t, int M, bool adj)
     return s;
 }
 
+bool bic_shape_supported(const BicTables &t, int M, const BicShape &s, bool adj)
+{
+    const int MM = M * M;
+    if (s.nw < 1 || s.nw > 16 || s.ppw != (MM + s.nw - 1) / s.nw || s.ppw > 8) return false;
+    if (s.taps_in_lds && !tlds_fits(t, adj)) return false;
+    if (s.fp == 2) return !adj && s.ppw <= 4 && pair_fits(t, s.taps_in_lds == 0);
+    return s.fp == 1;
+}
+
 std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj)
 {
     const int MM = M * M;

@@ -44,6 +44,9 @@ BicShape bic_default_shape(const BicTables &t, int M, bool adj = false);
 // All shapes the kernel supports for M (the autotuner's candidates), frame-pair shapes
 // included where their LDS tiles and tables fit in 64 KiB (not for the adjoint).
 std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj = false);
+// Whether the kernels run shape s (any wave count, not only the candidates) for this
+// geometry.
+bool bic_shape_supported(const BicTables &t, int M, const BicShape &s, bool adj = false);
 
 // Arguments common to every launch over unshifted frames (out, prev, frames, cumulant
 // constants and mode are filled by the caller).

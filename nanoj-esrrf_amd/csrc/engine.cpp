@@ -24,6 +24,7 @@
 #include "geometry.h"
 #include "java_text.h"
 #include "modes.h"
+#include "variants.h"
 
 using namespace esrrf;
 
@@ -80,6 +81,7 @@ struct esrrf_engine {
     bool tiled = false;          // tiled BIC kernel usable
     int stage = 0;
     int pair_direct = 0;
+    KernelVariants var;          // alternative kernels (read once at create)
     std::vector<float> h_drift;  // host copy of the drift buffer (x M)
     ModeState ms;                // FHT / 3-D state
     // bookkeeping
@@ -282,7 +284,7 @@ int calculate_bic_pipelined(esrrf_engine *e, const void *frames, int dtype, int 
         {
             ProfScope ps(e, 1);
             HIPCHK(e, launch_bic_prep(e->d_px + npx * f0, e->d_U + 4 * npx * f0, e->w, e->h, k,
-                                      e->stream));
+                                      e->stream, e->var.bic_prep_per_point));
         }
         int rc = run_bic_rgc(e, f0, e->counters[0], k);
         if (rc != ESRRF_OK) return rc;
@@ -329,7 +331,8 @@ int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtyp
         HIPCHK(e, launch_convert_frames(src, dtype, e->d_px, npx * n, e->stream));
         {
             ProfScope ps(e, 1);
-            HIPCHK(e, launch_bic_prep(e->d_px, e->d_U, e->w, e->h, n, e->stream));
+            HIPCHK(e, launch_bic_prep(e->d_px, e->d_U, e->w, e->h, n, e->stream,
+                                      e->var.bic_prep_per_point));
         }
     } else {
         ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,
@@ -543,6 +546,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
     e->hM = e->h * e->M;
     e->nvox = (long)e->wM * e->hM * (do3d ? (long)e->nP * e->M : 1);
     e->sens_int = sens_exact_int(p->sensitivity);
+    e->var = kernel_variants_from_env();
     if (e->mode == MODE_FHT)
         esrrf_fht_constants(p->fwhm, e->M, &e->sigma, &e->radius, &e->tSS, &e->tSO);
     else
@@ -576,20 +580,8 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         e->tiled = e->tabs.ok;
         if (e->tiled) {
             e->shape = bic_default_shape(e->tabs, e->M);
-            const int MM = e->M * e->M;
-            if (const char *ov = getenv("ESRRF_BIC_NW")) {
-                const int v = atoi(ov);
-                if (v >= 1 && v <= 16 && (MM + v - 1) / v <= 8) {
-                    e->shape.nw = v;
-                    e->shape.ppw = (MM + v - 1) / v;
-                }
-            }
-            if (const char *ov = getenv("ESRRF_BIC_TLDS")) e->shape.taps_in_lds = atoi(ov) != 0;
-            if (const char *ov = getenv("ESRRF_BIC_FP")) e->shape.fp = atoi(ov) == 2 ? 2 : 1;
-            e->stage = 0;
-            if (const char *ov = getenv("ESRRF_BIC_STAGE")) e->stage = atoi(ov) != 0;
-            e->pair_direct = 0;
-            if (const char *ov = getenv("ESRRF_BIC_PAIR_DIRECT")) e->pair_direct = atoi(ov) != 0;
+            e->stage = e->var.bic_stage;
+            e->pair_direct = e->var.bic_pair_direct;
         }
     } else {
         ModeCreate mc{};
@@ -605,6 +597,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         mc.tSS = e->tSS;
         mc.tSO = e->tSO;
         mc.p = p;
+        mc.var = e->var;
         rc = mode_create(mc, e->ms, &e->d_px, &e->d_U, &e->d_aux, &e->d_aux_bytes);
         if (rc != ESRRF_OK) {
             fail(e, rc, "%s", e->ms.err.c_str());
@@ -891,6 +884,25 @@ int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_ld
     if (ppw) *ppw = best.ppw;
     if (taps_in_lds) *taps_in_lds = best.taps_in_lds;
     if (ms) *ms = best.ms;
+    return ESRRF_OK;
+}
+
+int esrrf_set_launch_shape(esrrf_engine *e, int nw, int taps_in_lds, int frames_per_pass)
+{
+    if (!e) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->mode != MODE_BIC || !e->tiled)
+        return fail(e, ESRRF_ERR_STATE, "no tiled 2-D kernel in this mode or geometry");
+    BicShape s;
+    s.nw = nw;
+    s.ppw = nw >= 1 ? (e->M * e->M + nw - 1) / nw : 0;
+    s.taps_in_lds = taps_in_lds ? 1 : 0;
+    s.fp = frames_per_pass;
+    if (!bic_shape_supported(e->tabs, e->M, s))
+        return fail(e, ESRRF_ERR_UNSUPPORTED,
+                    "launch shape nw=%d taps_in_lds=%d frames_per_pass=%d not runnable here", nw,
+                    taps_in_lds, frames_per_pass);
+    e->shape = s;
     return ESRRF_OK;
 }
 

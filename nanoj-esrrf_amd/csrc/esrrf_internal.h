@@ -120,7 +120,8 @@ struct Rgc3Tables {
 
 // Host helpers implemented in the .hip files (all launches on `s`).
 hipError_t launch_convert_frames(const void *src, int dtype, float *dst, long n, hipStream_t s);
-hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s);
+hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s,
+                           bool per_point = false);
 hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h, int f,
                            hipStream_t s);
 // *kname (when given) receives the name of the kernel launched.

@@ -799,14 +799,9 @@ hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long 
     return hipGetLastError();
 }
 
-static int getenv_int(const char *name, int dflt)
-{
-    const char *v = getenv(name);
-    return v ? atoi(v) : dflt;
-}
-
 hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
-                        size_t lds_limit, hipStream_t s, const char **kname)
+                        size_t lds_limit, hipStream_t s, const char **kname, int kern,
+                        int split)
 {
     if (a.nframes <= 0) return hipSuccess;
     const long nl = (long)a.nframes * a.nP * a.wM * a.hM;
@@ -815,13 +810,12 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t
     const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.M);
     // (the fast kernel addresses one frame of U with 32-bit byte offsets)
     const bool off32 = (double)a.nP * 4.0 * a.w * a.h * sizeof(float4) < 4294967296.0;
-    const int kern = getenv_int("ESRRF_3D_KERNEL", 3);
     // the tiled kernel addresses one frame of U with 32-bit element offsets
     const bool off31 = (double)a.nP * 4.0 * a.w * a.h < 2147483648.0;
     if (drift_zero && t && kern == 3 && off31 && rgc3_lds_bytes(*t) <= lds_limit) {
         const int tiles = ((a.w + 7) >> 3) * ((a.h + 7) >> 3);
         const dim3 g3(tiles, (a.M * a.M + 3) / 4, a.nP * a.M);
-        if (getenv_int("ESRRF_3D_SPLIT", 2) == 1)
+        if (split == 1)
             hipLaunchKernelGGL(k3d_rgc3<1>, g3, dim3(256), rgc3_lds_bytes(*t), s, a, *t);
         else
             hipLaunchKernelGGL(k3d_rgc3<2>, g3, dim3(512), rgc3_lds_bytes(*t), s, a, *t);

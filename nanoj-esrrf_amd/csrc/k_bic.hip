@@ -275,11 +275,11 @@ __global__ void __launch_bounds__(256) k_bic_gint(const float *__restrict__ px,
     gyi[(long)y * 2 * w + x] = gint_at(lgy, PREP_G, gx0, gy0, w, h, x, y);
 }
 
-hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s)
+hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s,
+                           bool per_point)
 {
     if (nF <= 0) return hipSuccess;
-    const char *v = getenv("ESRRF_BIC_PREP");
-    if (v && atoi(v) == 1) {   // one 4x4 stencil per sample point (k_bic_prep)
+    if (per_point) {   // one 4x4 stencil per sample point (k_bic_prep)
         dim3 grid((2 * w + PREP_T - 1) / PREP_T, (2 * h + PREP_T - 1) / PREP_T, nF);
         hipLaunchKernelGGL(k_bic_prep, grid, dim3(PREP_T, PREP_T), 0, s, px, U, w, h);
         return hipGetLastError();

@@ -202,7 +202,7 @@ int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
     a.tSS = r.tSS;
     a.tSO = r.tSO;
     hipError_t e = launch3d_rgc(a, r.drift_zero, ms.tiled3 ? &ms.t3 : nullptr, ms.lds_limit,
-                                c.s, &ms.rgc_kernel);
+                                c.s, &ms.rgc_kernel, ms.var.k3d_kernel, ms.var.k3d_split);
     if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D RGC launch", e);
     return ESRRF_OK;
 }

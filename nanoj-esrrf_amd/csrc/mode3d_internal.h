@@ -37,8 +37,11 @@ hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long 
                          hipStream_t s);
 // The tiled kernel (k3d_rgc3) runs for frames without drift when `t` is given and its LDS
 // fits `lds_limit`; *kname receives the name of the kernel launched.
+// kern: 3 tiled (default), 2 / 1 the per-voxel kernels; split: waves per phase in the tiled
+// kernel (2 default)
 hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
-                        size_t lds_limit, hipStream_t s, const char **kname);
+                        size_t lds_limit, hipStream_t s, const char **kname, int kern = 3,
+                        int split = 2);
 size_t rgc3_lds_bytes(const Rgc3Tables &t);
 hipError_t launch3d_count(int w, int h, int nP, int M, float radius, float tSO,
                           unsigned long long *d_total, hipStream_t s);

@@ -107,6 +107,7 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
 {
     ms.mode = c.mode;
     ms.nF = c.nF;
+    ms.var = c.var;
     *aux_bytes = 0;
     *d_aux = nullptr;
     if (c.mode == 1) {
@@ -152,14 +153,11 @@ int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, 
             const size_t per = (size_t)(c.M + 1) * N * N * sizeof(float2);
             p.chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)c.nF, (1ull << 30) / per));
             p.chunk = std::min(p.chunk, 65535 / (c.M * c.M));   // grid.y = chunk * M^2
-            // last pass: row-tiled stores (default) or the pixel-by-pixel kernel
-            const char *sc = getenv("ESRRF_FHT_SCATTER");
-            p.scatter = (sc && atoi(sc) != 0) ? 1 : 0;
-            // column passes: tiles grouped per XCD (default) or in launch order
-            const char *xo = getenv("ESRRF_FHT_XCD");
-            p.xcd = (xo && atoi(xo) == 0) ? 0 : 1;
-            const char *pf = getenv("ESRRF_FHT_PREFETCH");
-            p.no_prefetch = (pf && atoi(pf) == 0) ? 1 : 0;
+            // last pass: row-tiled stores (default) or the pixel-by-pixel kernel; column
+            // passes: tiles grouped per XCD (default) or in launch order
+            p.scatter = c.var.fht_scatter;
+            p.xcd = c.var.fht_xcd;
+            p.no_prefetch = c.var.fht_prefetch ? 0 : 1;
             float2 *dtw = nullptr;
             if (!dmalloc(&p.F, sizeof(float2) * (size_t)N * N * p.chunk) ||
                 !dmalloc(&p.Y, sizeof(float2) * (size_t)N * N * c.M * p.chunk) ||
@@ -259,8 +257,7 @@ int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
         a.int_weighting = r.int_weighting;
         a.R = ms.fht_R;
         a.Rhi = ms.fht_Rhi;
-        a.pair = 1;
-        if (const char *ov = getenv("ESRRF_FHT_PAIR")) a.pair = atoi(ov) != 0;
+        a.pair = ms.var.fht_pair;
         a.TW = ms.fht_TW;
         a.TH = ms.fht_TW;
         a.pitch = ms.fht_pitch;

@@ -8,6 +8,7 @@
 #include "../../include/esrrf.h"
 #include "esrrf_internal.h"
 #include "fht_internal.h"
+#include "variants.h"
 
 namespace esrrf {
 
@@ -47,6 +48,7 @@ struct ModeState {
     Rgc3Tables t3{};
     bool tiled3 = false;
     size_t lds_limit = 0;         // dynamic LDS one block may use on the device
+    KernelVariants var;           // alternative kernels (read when the engine was created)
     const char *rgc_kernel = "";  // RGC kernel of the last launch
 };
 
@@ -64,6 +66,7 @@ struct ModeCreate {
     int w, h, W, H, nP, M, nF;
     float radius, tSS, tSO;
     const esrrf_params *p;
+    KernelVariants var;
 };
 
 struct ModeRgc {

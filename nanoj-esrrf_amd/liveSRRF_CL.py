@@ -267,6 +267,17 @@ class LiveSRRF_CL:
         return dict(nw=v[0].value, ppw=v[1].value, taps_in_lds=v[2].value,
                     frames_per_pass=v[3].value)
 
+    def setLaunchShape(self, nw=None, taps_in_lds=None, frames_per_pass=None):
+        """Pin the tiled 2-D kernel's launch shape (esrrf_set_launch_shape; no counterpart in
+        the reference); fields left None keep the shape in use."""
+        cur = self.launchShape()
+        e = self._e()
+        e.check(e.L.esrrf_set_launch_shape(
+            e.h, int(cur["nw"] if nw is None else nw),
+            int(cur["taps_in_lds"] if taps_in_lds is None else taps_in_lds),
+            int(cur["frames_per_pass"] if frames_per_pass is None else frames_per_pass)))
+        return self.launchShape()
+
     def frameCounters(self):
         """nCurrentFrame[0..1] (exact bookkeeping hook)."""
         return self._e().counters()

@@ -17,7 +17,8 @@ def stack(n, w, h, seed, emitters=None, fil=4):
 
 
 def run_engine(pkg, st, M, fwhm, S, nOnGPU=None, nFor=None, intW=True, mp=True, drift=None,
-               loads=None):
+               loads=None, shape=None):
+    """shape: launch-shape fields to pin (nw, taps_in_lds, frames_per_pass)."""
     n = st.shape[0]
     nOnGPU = nOnGPU or n
     nFor = nFor or n
@@ -25,6 +26,8 @@ def run_engine(pkg, st, M, fwhm, S, nOnGPU=None, nFor=None, intW=True, mp=True, 
     e.checkDevices()
     e.initialise(st.shape[2], st.shape[1], M, fwhm, S, nOnGPU, nFor, 20000,
                  e.allCLdevices[0], intW, mp, False, None, 0, 1)
+    if shape:
+        e.setLaunchShape(**shape)
     e.resetFramePosition()
     dx, dy = (np.zeros(nFor), np.zeros(nFor)) if drift is None else drift
     e.loadDriftXYGPUbuffer(dx, dy)
@@ -83,64 +86,62 @@ def test_bic_configs_parity(pkg, O, M, fwhm, S, w, h):
     (5, 40, 32, [(0, 5), (5, 9)]), (4, 27, 19, [(0, 7)]), (2, 24, 24, [(0, 1), (1, 4)]),
     (7, 16, 24, [(0, 6)]),
 ])
-def test_bic_frame_pair_kernel_matches_single_frame(pkg, monkeypatch, M, w, h, loads):
+def test_bic_frame_pair_kernel_matches_single_frame(pkg, M, w, h, loads):
     """The frame-pair kernel (two frames per LDS pass, the default) reproduces the
     single-frame kernel bit for bit: odd tails, one-frame loads and continuation across
     loads included."""
     n = loads[-1][1]
     st = stack(n, w, h, seed=40 + M)
     res = {}
-    for fp in ("1", "2"):
-        monkeypatch.setenv("ESRRF_BIC_FP", fp)
+    for fp in (1, 2):
         res[fp], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
-                                   nFor=n, loads=loads)
-    assert np.array_equal(res["1"], res["2"])
+                                   nFor=n, loads=loads, shape=dict(frames_per_pass=fp))
+    assert np.array_equal(res[1], res[2])
 
 
 @pytest.mark.parametrize("M,w,h,loads", [
     (5, 40, 32, [(0, 5), (5, 9)]), (4, 27, 19, [(0, 7)]), (3, 64, 48, [(0, 4)]),
 ])
-def test_bic_frame_pair_taps_from_scalar_loads(pkg, monkeypatch, M, w, h, loads):
+def test_bic_frame_pair_taps_from_scalar_loads(pkg, M, w, h, loads):
     """The frame-pair kernel reading its tap tables with scalar loads from global memory
     reproduces the variant that stages them in LDS bit for bit (same operations, the tap
     values only come from SGPRs instead of VGPRs)."""
     n = loads[-1][1]
     st = stack(n, w, h, seed=70 + M)
     res = {}
-    monkeypatch.setenv("ESRRF_BIC_FP", "2")
-    for v in ("0", "1"):
-        monkeypatch.setenv("ESRRF_BIC_TLDS", v)
+    for v in (0, 1):
         res[v], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
-                                  nFor=n, loads=loads)
-    assert np.array_equal(res["0"], res["1"])
+                                  nFor=n, loads=loads,
+                                  shape=dict(frames_per_pass=2, taps_in_lds=v))
+    assert np.array_equal(res[0], res[1])
 
 
-@pytest.mark.parametrize("M,fwhm,w,h,nw", [(5, 1.5, 40, 32, "8"), (4, 2.0, 27, 19, "4"),
-                                            (2, 1.0, 24, 24, "4"), (5, 3.0, 16, 16, "12")])
-def test_bic_frame_pair_register_staging(pkg, monkeypatch, M, fwhm, w, h, nw):
+@pytest.mark.parametrize("M,fwhm,w,h,nw,tl", [(5, 1.5, 40, 32, "8", 1), (4, 2.0, 27, 19, "4", 1),
+                                               (2, 1.0, 24, 24, "4", 1),
+                                               (5, 3.0, 16, 16, "12", 0)])
+def test_bic_frame_pair_register_staging(pkg, monkeypatch, M, fwhm, w, h, nw, tl):
     """Staging the next frame pair's tiles in registers during the current pair (stored
     to LDS between the barriers) gives the same bits as filling them after it, odd frame
     tails and continuation across loads included."""
     st = stack(7, w, h, seed=140 + M)
-    monkeypatch.setenv("ESRRF_BIC_FP", "2")
-    monkeypatch.setenv("ESRRF_BIC_NW", nw)
     res = {}
     for sg in ("0", "1"):
         monkeypatch.setenv("ESRRF_BIC_PAIR_DIRECT", sg)
-        res[sg], _, _ = run_engine(pkg, st, M, fwhm, 2, nOnGPU=7, loads=[(0, 5), (5, 7)])
+        res[sg], _, _ = run_engine(pkg, st, M, fwhm, 2, nOnGPU=7, loads=[(0, 5), (5, 7)],
+                                   shape=dict(frames_per_pass=2, nw=int(nw), taps_in_lds=tl))
     assert np.array_equal(res["0"], res["1"])
 
 
 @pytest.mark.parametrize("nw", ["7", "9", "13"])
-def test_bic_frame_pair_balanced_wave_counts(pkg, monkeypatch, nw):
+def test_bic_frame_pair_balanced_wave_counts(pkg, nw):
     """Blocks of 7, 9 or 13 waves (the balanced shapes for 25 phases) give the same bits as
     the default 8-wave block: each (pixel, phase) is evaluated the same way by whichever
     wave holds it."""
     st = stack(6, 40, 32, seed=91)
     res = {}
     for v in ("8", nw):
-        monkeypatch.setenv("ESRRF_BIC_NW", v)
-        res[v], _, _ = run_engine(pkg, st, 5, 1.5, 2, nOnGPU=6, loads=[(0, 6)])
+        res[v], _, _ = run_engine(pkg, st, 5, 1.5, 2, nOnGPU=6, loads=[(0, 6)],
+                                  shape=dict(nw=int(v)))
     assert np.array_equal(res["8"], res[nw])
 
 

@@ -30,12 +30,14 @@ def production():
     return synth.production_256x256x1000()
 
 
-def engine(pkg, st, mp=True, loads=None, fht=False, M_=M):
+def engine(pkg, st, mp=True, loads=None, fht=False, M_=M, shape=None):
     n, h, w = st.shape
     e = pkg.LiveSRRF_CL()
     e.checkDevices()
     e.initialise(w, h, M_, FWHM, S, max(b - a for a, b in (loads or [(0, n)])), n, 20000,
                  e.allCLdevices[0], True, mp, fht, None, 0, 1)
+    if shape:
+        e.setLaunchShape(**shape)
     e.resetFramePosition()
     e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
     for a, b in (loads or [(0, n)]):
@@ -79,10 +81,8 @@ def test_full_size_mp_correction(pkg, production, full_nomp):
     assert np.array_equal(got[3], full_nomp[3])          # INT is not corrected (:486-505)
 
 
-def test_full_size_kernels_and_loads_bit_identical(pkg, monkeypatch, production, full_nomp):
-    monkeypatch.setenv("ESRRF_BIC_FP", "1")
-    single = engine(pkg, production, mp=False)[0]
-    monkeypatch.delenv("ESRRF_BIC_FP")
+def test_full_size_kernels_and_loads_bit_identical(pkg, production, full_nomp):
+    single = engine(pkg, production, mp=False, shape=dict(frames_per_pass=1))[0]
     assert np.array_equal(single, full_nomp)
     four = engine(pkg, production, mp=False, loads=[(0, 250), (250, 500), (500, 750),
                                                     (750, 1000)])[0]

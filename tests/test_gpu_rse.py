@@ -133,3 +133,72 @@ def test_rse_with_sample_free_phases(pkg, O, data, fwhm):
     assert math.isfinite(got) and math.isfinite(v) and all(math.isfinite(x) for x in g)
     assert math.isclose(got, exp, rel_tol=1e-4), (got, exp)
     assert math.isclose(v, got, rel_tol=1e-5)
+
+
+# --- the differentiable-selection workload itself: 500 frames of 128x128, M=5 -------------
+M5, CROP5 = 5, 3
+
+
+@pytest.fixture(scope="module")
+def rse_full(pkg):
+    from nanoj_esrrf_amd import synth
+    st = synth.rse_128x128x500()
+    r = pkg.RSE(128, 128, M5, st.shape[0], crop_size=CROP5)
+    r.load(st)
+    yield r
+    r.close()
+
+
+def gated_pairs(O, fwhm, M):
+    """The (phase, tap) pairs the tSO gate keeps for interior pixels (liveSRRF.cl:279-291),
+    by brute force over the reference's window."""
+    f32 = np.float32
+    k = O.bic_constants(fwhm)
+    r2lo = -int(f32(2) * f32(k["radius"]))
+    r2hi = int(f32(2) * f32(k["radius"]) + f32(1))
+    keep = set()
+    for sy in range(M):
+        for sx in range(M):
+            xc, yc = f32((16 * M + sx + 0.5) / M), f32((16 * M + sy + 0.5) / M)
+            bx, by = int(f32(2) * xc), int(f32(2) * yc)
+            for j in range(r2lo, r2hi + 1):
+                for i in range(r2lo, r2hi + 1):
+                    dx = f32((f32(bx) + f32(i)) / f32(2) - xc)
+                    dy = f32((f32(by) + f32(j)) / f32(2) - yc)
+                    d = np.sqrt(f32(f32(dx * dx) + f32(dy * dy)))
+                    if d != 0 and d <= f32(k["tSO"]):
+                        keep.add((sy, sx, j, i))
+    return keep, (r2lo, r2hi)
+
+
+@pytest.mark.parametrize("x", [(1.5, 2.0, 4.0), (2.2, 1.5, 3.0)])
+def test_rse_gradient_full_size_matches_central_differences(O, rse_full, x):
+    """Adjoint gradient at 128x128x500, M=5 against central differences (h = 1e-3) of the
+    device forward: relative error < 1e-3 of the gradient's norm.  The gate and the window
+    are piecewise constant in fwhm (the adjoint holds them fixed), so the points are chosen,
+    and checked here, to keep the same gated samples over [fwhm - h, fwhm + h]."""
+    h = 1e-3
+    g0, w0 = gated_pairs(O, x[0], M5)
+    for fx in (x[0] - h, x[0] + h):
+        assert gated_pairs(O, fx, M5) == (g0, w0)
+    _, g = rse_full.value_and_grad(*x)
+    fd = fd_grad(rse_full.value, x, h)
+    err = np.abs(g - fd).max() / np.abs(fd).max()
+    assert err < 1e-3, (g, fd, err)
+
+
+def test_descent_reaches_grid_best_full_size(rse_full):
+    """30 projected Adam steps from (fwhm 1, S 1, sigma 1.5 raw px) reach the best RSE of the
+    25 x 25 (fwhm, S) grid with the RSF sigma by Brent at every point, within 1 %."""
+    smax = 4.0 / 2.35482 * M5
+    lo, hi = (0.5, 0.5, 0.01), (3.0, 4.0, smax)
+    best = None
+    for f in np.linspace(lo[0], hi[0], 25):
+        for s in np.linspace(lo[1], hi[1], 25):
+            sg, v, _ = rse_full.best_sigma(float(f), float(s))
+            assert math.isfinite(v)
+            if best is None or v < best[0]:
+                best = (v, float(f), float(s), sg)
+    traj = rse_full.descend((1.0, 1.0, 1.5 * M5), steps=30, lo=lo, hi=hi)
+    assert np.isfinite(traj).all()
+    assert traj[:, 3].min() <= 1.01 * best[0], (traj[:, 3].min(), best)

@@ -145,3 +145,77 @@ def test_engine_autotune_keeps_results(pkg):
     after = e.readSRRFbuffer()
     e.release()
     assert_parity(after, before, "after autotune")
+
+
+# --- the batched-sweep workload's full parameter grid on a reduced stack -------------------
+C3_MAGS, C3_FWHM, C3_SENS, C3_NFS = [2, 4, 5], [1.0, 1.5, 2.0, 2.5, 3.0], [1.0, 2.0, 3.0, 4.0], \
+    [6, 15, 24]
+
+
+def test_sweep_full_grid_autotuned_matches_per_configuration_engine(pkg):
+    """The 180 configurations of magnification {2,4,5} x fwhm {1..3} x sensitivity {1..4}
+    x first {6,15,24} frames of a 40x32x24 stack, through the autotuned, graph-captured
+    fused sweep, against initialise/calculateSRRF/readSRRFbuffer per configuration."""
+    st = stack(24, 40, 32, seed=303)
+    sw = pkg.ParametersSweep(40, 32, C3_MAGS, C3_FWHM, C3_SENS, C3_NFS, autotune=True,
+                             use_graph=True)
+    sw.run(st)
+    shapes = set()
+    for im, M in enumerate(C3_MAGS):
+        for i_f, fwhm in enumerate(C3_FWHM):
+            sh = sw.shape(im, i_f)
+            shapes.add((sh["nw"], sh["taps_in_lds"], sh["frames_per_pass"]))
+            for iq, nf in enumerate(C3_NFS):
+                for i_s, S in enumerate(C3_SENS):
+                    exp = plain(pkg, st, M, fwhm, S, nf)
+                    assert_parity(sw.read(im, iq, i_s, i_f), exp,
+                                  f"M{M} nf{nf} S{S} fwhm{fwhm}")
+    sw.close()
+    print("autotuned shapes:", shapes)
+
+
+@pytest.mark.parametrize("M", C3_MAGS)
+@pytest.mark.parametrize("fwhm", C3_FWHM)
+def test_every_runnable_launch_shape_is_bit_identical(pkg, M, fwhm):
+    """Every launch shape the autotuner may pick (4 to 16 waves, tap tables in LDS or
+    through scalar loads, one or two frames per pass) gives the default shape's bits for
+    this (magnification, fwhm), on a 40x32 stack in two loads."""
+    st = stack(24, 40, 32, seed=404 + M)
+    ref, shape0 = None, None
+    ran = []
+    for nw in ([M * M] if M * M <= 4 else []) + [4, 8, 12, 16]:
+        for tl in (1, 0):
+            for fp in (1, 2):
+                e = pkg.LiveSRRF_CL()
+                e.checkDevices()
+                e.initialise(40, 32, M, fwhm, 2.0, 12, 24, 20000, e.allCLdevices[0], True, True,
+                             False, None, 0, 1)
+                if shape0 is None:
+                    shape0 = e.launchShape()
+                try:
+                    e.setLaunchShape(nw=nw, taps_in_lds=tl, frames_per_pass=fp)
+                except pkg._capi.EsrrfError:
+                    e.release()
+                    continue      # not runnable for this geometry (LDS budget)
+                e.resetFramePosition()
+                e.loadDriftXYGPUbuffer(np.zeros(24), np.zeros(24))
+                e.calculateSRRF(st[:12])
+                e.calculateSRRF(st[12:])
+                out = np.asarray(e.readSRRFbuffer()).copy()
+                e.release()
+                ran.append((nw, tl, fp))
+                if ref is None:
+                    e0 = pkg.LiveSRRF_CL()
+                    e0.checkDevices()
+                    e0.initialise(40, 32, M, fwhm, 2.0, 12, 24, 20000, e0.allCLdevices[0],
+                                  True, True, False, None, 0, 1)
+                    e0.resetFramePosition()
+                    e0.loadDriftXYGPUbuffer(np.zeros(24), np.zeros(24))
+                    e0.calculateSRRF(st[:12])
+                    e0.calculateSRRF(st[12:])
+                    ref = np.asarray(e0.readSRRFbuffer()).copy()
+                    e0.release()
+                assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), (nw, tl, fp)
+    assert len(ran) >= 4, ran
+    # the default shape is among the runnable ones
+    assert (shape0["nw"], shape0["taps_in_lds"], shape0["frames_per_pass"]) in ran
