@@ -129,11 +129,12 @@ hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s,
                           const char **kname = nullptr);
 size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm);   // LDS of the frame-pair kernel
 hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
-hipError_t launch_var(float *out, long nvox, hipStream_t s);
+// nbatch images `ostride` floats apart (MP maps `mstride` floats apart) in one launch
+hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch = 1, long ostride = 0);
 hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,
-                         hipStream_t s);
+                         hipStream_t s, int nbatch = 1, long ostride = 0, long mstride = 0);
 hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int dims,
-                             hipStream_t s);
+                             hipStream_t s, int nbatch = 1, long ostride = 0, long mstride = 0);
 hipError_t launch_nan_to_zero_copy(const float *src, float *dst, long n, hipStream_t s);
 
 }  // namespace esrrf

@@ -45,8 +45,10 @@ hipError_t launch_convert_frames(const void *src, int dtype, float *dst, long n,
 }
 
 // VAR = E[x^2] - AVG^2 and TAC2 -= AVG^2, in place on channels 1 and 2.
-__global__ void k_var(float *__restrict__ out, long nvox)
+// blockIdx.y: image of a batch at a stride of `ostride` floats
+__global__ void k_var(float *__restrict__ out, long nvox, long ostride)
 {
+    out += ostride * blockIdx.y;
     long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
     for (; o < nvox; o += stride) {
@@ -57,19 +59,23 @@ __global__ void k_var(float *__restrict__ out, long nvox)
     }
 }
 
-hipError_t launch_var(float *out, long nvox, hipStream_t s)
+hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch, long ostride)
 {
-    hipLaunchKernelGGL(k_var, dim3(grid_for(nvox, 256)), dim3(256), 0, s, out, nvox);
+    hipLaunchKernelGGL(k_var, dim3(grid_for(nvox, 256), nbatch), dim3(256), 0, s, out, nvox,
+                       ostride);
     return hipGetLastError();
 }
 
 // One block per MP-map entry (channel, [z,] y, x sub-position): mean of that sub-position
 // over every macro-pixel (macro-voxel).  The reference sums sequentially in float; this
-// tree-reduces in double, which is at least as accurate.
+// tree-reduces in double, which is at least as accurate.  blockIdx.y: image of a batch
+// (images `ostride` floats apart, maps `mstride` floats apart).
 __global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
                                                 float *__restrict__ mp, int w, int h, int nP,
-                                                int M, int dims)
+                                                int M, int dims, long ostride, long mstride)
 {
+    out += ostride * blockIdx.y;
+    mp += mstride * blockIdx.y;
     const int o = blockIdx.x;
     const int MM = M * M;
     const int Md = dims == 3 ? MM * M : MM;
@@ -103,8 +109,10 @@ __global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
 }
 
 __global__ void k_mp_correct(float *__restrict__ out, const float *__restrict__ mp, int w,
-                             int h, int nP, int M, int dims)
+                             int h, int nP, int M, int dims, long ostride, long mstride)
 {
+    out += ostride * blockIdx.y;
+    mp += mstride * blockIdx.y;
     const long wM = (long)w * M, hM = (long)h * M;
     const long whM = wM * hM;
     const long zMn = dims == 3 ? (long)nP * M : 1;
@@ -128,19 +136,20 @@ __global__ void k_mp_correct(float *__restrict__ out, const float *__restrict__ 
 }
 
 hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,
-                         hipStream_t s)
+                         hipStream_t s, int nbatch, long ostride, long mstride)
 {
     int nentries = 3 * M * M * (dims == 3 ? M : 1);
-    hipLaunchKernelGGL(k_mp_map, dim3(nentries), dim3(256), 0, s, out, mp, w, h, nP, M, dims);
+    hipLaunchKernelGGL(k_mp_map, dim3(nentries, nbatch), dim3(256), 0, s, out, mp, w, h, nP, M,
+                       dims, ostride, mstride);
     return hipGetLastError();
 }
 
 hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int dims,
-                             hipStream_t s)
+                             hipStream_t s, int nbatch, long ostride, long mstride)
 {
     long n = 3L * w * M * h * M * (dims == 3 ? (long)nP * M : 1);
-    hipLaunchKernelGGL(k_mp_correct, dim3(grid_for(n, 256)), dim3(256), 0, s, out, mp, w, h,
-                       nP, M, dims);
+    hipLaunchKernelGGL(k_mp_correct, dim3(grid_for(n, 256), nbatch), dim3(256), 0, s, out, mp,
+                       w, h, nP, M, dims, ostride, mstride);
     return hipGetLastError();
 }
 

@@ -150,18 +150,19 @@ int enqueue_run(esrrf_sweep *s, int dtype)
                 c.sens_int[k] = sens_exact_int(s->sens[k]);
             }
             SCHK(s, launch_sweep_acc(c, st));
-            // VAR and MP correction of each configuration (LiveSRRF_CL.java:753-770)
-            for (int q = 0; q < nQ; q++)
-                for (int k = 0; k < nS; k++) {
-                    const long ci = ((long)q * nS + k) * nF + fi;
-                    float *out = s->d_out[im] + ci * 4 * whM;
-                    SCHK(s, launch_var(out, whM, st));
-                    if (s->do_mp) {
-                        float *mp = s->d_mp[im] + ci * 3 * M * M;
-                        SCHK(s, launch_mp_map(out, mp, s->w, s->h, 1, M, 2, st));
-                        SCHK(s, launch_mp_correct(out, mp, s->w, s->h, 1, M, 2, st));
-                    }
-                }
+            // VAR and MP correction of each configuration (LiveSRRF_CL.java:753-770), the
+            // nQ x nS configurations of this fwhm in one launch each: configuration
+            // c = q nS + k sits at index c nF + fi
+            const int nb = nQ * nS;
+            float *out = s->d_out[im] + (long)fi * 4 * whM;
+            SCHK(s, launch_var(out, whM, st, nb, (long)nF * 4 * whM));
+            if (s->do_mp) {
+                float *mp = s->d_mp[im] + (long)fi * 3 * M * M;
+                SCHK(s, launch_mp_map(out, mp, s->w, s->h, 1, M, 2, st, nb, (long)nF * 4 * whM,
+                                      (long)nF * 3 * M * M));
+                SCHK(s, launch_mp_correct(out, mp, s->w, s->h, 1, M, 2, st, nb,
+                                          (long)nF * 4 * whM, (long)nF * 3 * M * M));
+            }
         }
     }
     return ESRRF_OK;
