@@ -175,8 +175,9 @@ int esrrf_rgc_kernel(esrrf_engine *e, char *buf, int len);
  * first nf frames of one stack, 2-D BIC, no drift; for each the four channels that
  * readSRRFbuffer returns after that configuration's initialise/calculateSRRF/readSRRFbuffer
  * (LiveSRRF_CL.java:159,636,749), computed from one upload, one gradient pass, one RGC
- * launch per (magnification, fwhm) and one accumulation launch for all frame counts and
- * sensitivities.  The reference loops nf -> S -> fwhm and rebuilds its program per
+ * launch per magnification and group of up to five fwhm values (the fwhm only changes the
+ * weights and the gate of a sample, so one angular term serves the group) and one
+ * accumulation launch per fwhm for all frame counts and sensitivities.  The reference loops nf -> S -> fwhm and rebuilds its program per
  * configuration (:328-380, :452). */
 typedef struct esrrf_sweep esrrf_sweep;
 typedef struct esrrf_sweep_params {
@@ -207,8 +208,9 @@ int esrrf_sweep_run_device(esrrf_sweep *s, const void *d_frames, int dtype, int 
 int esrrf_sweep_read(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm, float *out);
 int esrrf_sweep_read_device(esrrf_sweep *s, int i_mag, int i_nf, int i_sens, int i_fwhm,
                             float *d_out);
-/* Launch shape used for a (magnification, fwhm) (autotuned or default): waves per block,
- * phases per wave, tap tables in LDS, frames per pass, autotuner time (0: not tuned). */
+/* Launch shape of the emit launch that serves a (magnification, fwhm) (autotuned or
+ * default): waves per block, phases per wave, tap tables in LDS, frames per pass, autotuner
+ * time (0: not tuned). */
 int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int i_fwhm, int *nw, int *ppw, int *taps_in_lds,
                       int *frames_per_pass, float *ms);
 const char *esrrf_sweep_last_error(esrrf_sweep *s);

@@ -195,4 +195,149 @@ hipError_t bic_autotune(const BicTables &t, BicRgcArgs a, int M, int reps, hipSt
     return e;
 }
 
+void bic_multi_tables_free(BicTablesMulti &t)
+{
+    void *ptrs[] = {t.d_taps, t.d_phases, t.d_tapk};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    t.d_taps = nullptr;
+    t.d_phases = nullptr;
+    t.d_tapk = nullptr;
+    t.ok = false;
+}
+
+hipError_t bic_multi_tables_create(int w, int h, int M, const float *fwhm, int nfg,
+                                   BicTablesMulti &t)
+{
+    float radius[SWEEP_NFG], tss[SWEEP_NFG], tso[SWEEP_NFG];
+    for (int f = 0; f < nfg; f++) {
+        float sigma = 0.0f;
+        esrrf_bic_constants(fwhm[f], &sigma, &radius[f], &tss[f], &tso[f]);
+    }
+    t.nfg = nfg;
+    t.ok = M <= 8 && nfg >= 1 && nfg <= SWEEP_NFG &&
+           build_bic_geometry_multi(w, h, M, radius, tss, tso, nfg, t.geo);
+    if (!t.ok) return hipSuccess;
+    const size_t nt = t.geo.taps.size(), np = t.geo.phases.size();
+    hipError_t e = hipMalloc((void **)&t.d_taps, sizeof(TapM) * nt);
+    if (e == hipSuccess) e = hipMalloc((void **)&t.d_phases, sizeof(PhaseM) * np);
+    if (e == hipSuccess) e = hipMalloc((void **)&t.d_tapk, sizeof(int) * nt);
+    if (e == hipSuccess)
+        e = hipMemcpy(t.d_taps, t.geo.taps.data(), sizeof(TapM) * nt, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(t.d_phases, t.geo.phases.data(), sizeof(PhaseM) * np,
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(t.d_tapk, t.geo.tapk.data(), sizeof(int) * nt, hipMemcpyHostToDevice);
+    if (e != hipSuccess) bic_multi_tables_free(t);
+    return e;
+}
+
+// longest tap run of one block's phases [g PG, (g+1) PG) over every x/y class
+static int group_taps(const BicTablesMulti &t, int M, int pg)
+{
+    const int MM = M * M;
+    int best = 0;
+    const int ncls = t.geo.ncls_x * t.geo.ncls_y;
+    for (int c = 0; c < ncls; c++) {
+        const PhaseM *P = t.geo.phases.data() + (size_t)c * MM;
+        for (int p0 = 0; p0 < MM; p0 += pg) {
+            const int pl = std::min(p0 + pg, MM) - 1;
+            best = std::max(best, P[pl].first_tap + P[pl].ntaps - P[p0].first_tap);
+        }
+    }
+    return best;
+}
+
+BicRgcArgs bic_args_multi(const BicTablesMulti &t, const EmitShape &sh, const float2 *U,
+                          const float *px, int w, int h, int M)
+{
+    BicRgcArgs a{};
+    a.U = U;
+    a.px = px;
+    a.w = w;
+    a.h = h;
+    a.M = M;
+    a.wM = w * M;
+    a.hM = h * M;
+    a.G0 = t.geo.G0;
+    a.PP = t.geo.PP;
+    a.PH = t.geo.PH;
+    a.TW = t.geo.TW;
+    a.TH = t.geo.TH;
+    a.R2lo = t.geo.R2lo;
+    a.R2hi = t.geo.R2hi;
+    a.xs_log2 = t.geo.xs_log2;
+    a.ncls_x = t.geo.ncls_x;
+    a.ncls_y = t.geo.ncls_y;
+    a.tiles_x = (w + 7) / 8;
+    a.tiles_y = (h + 7) / 8;
+    a.nw = sh.nw;
+    a.mode = 1;
+    return a;
+}
+
+std::vector<EmitShape> emit_candidate_shapes(const BicTablesMulti &t, int M)
+{
+    std::vector<EmitShape> out;
+    const int MM = M * M;
+    for (int ppw = 1; ppw <= 2; ppw++) {
+        for (int nw = 4; nw <= 16; nw += 4) {
+            if ((nw - 4) * ppw >= MM) break;   // smaller blocks already hold every phase
+            EmitShape s;
+            s.nw = nw;
+            s.ppw = ppw;
+            s.pg_taps = group_taps(t, M, nw * ppw);
+            BicRgcArgs a = bic_args_multi(t, s, nullptr, nullptr, 8, 8, M);
+            if (bic_emit_multi_lds_bytes(a, s.pg_taps) <= 64 * 1024) out.push_back(s);
+        }
+    }
+    if (out.empty()) {   // one phase per wave and block: the smallest footprint
+        EmitShape s;
+        s.nw = 1;
+        s.ppw = 1;
+        s.pg_taps = group_taps(t, M, 1);
+        out.push_back(s);
+    }
+    return out;
+}
+
+hipError_t emit_autotune(const BicTablesMulti &t, BicRgcArgs a, int M, long cstride, int reps,
+                         hipStream_t s, EmitShape *best)
+{
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return e;
+    }
+    const std::vector<EmitShape> cands = emit_candidate_shapes(t, M);
+    *best = cands[0];
+    bool have = false;
+    for (EmitShape c : cands) {
+        a.nw = c.nw;
+        e = launch_bic_emit_multi(a, c.ppw, t.d_taps, t.d_phases, t.d_tapk, t.nfg, c.pg_taps,
+                                  cstride, s);   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; r++)
+            e = launch_bic_emit_multi(a, c.ppw, t.d_taps, t.d_phases, t.d_tapk, t.nfg,
+                                      c.pg_taps, cstride, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) break;
+        c.ms = ms / (float)std::max(reps, 1);
+        if (!have || c.ms < best->ms) {
+            *best = c;
+            have = true;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return e;
+}
+
 }  // namespace esrrf

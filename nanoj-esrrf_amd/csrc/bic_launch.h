@@ -60,4 +60,32 @@ hipError_t bic_autotune(const BicTables &t, BicRgcArgs a, int M, int reps, hipSt
 
 int sens_exact_int(float s);
 
+// Union tables of up to SWEEP_NFG fwhm values of one magnification (the sweep's emit pass,
+// k_bic_emit_multi) and their device copy.
+struct BicTablesMulti {
+    BicGeometryMulti geo;
+    TapM *d_taps = nullptr;
+    PhaseM *d_phases = nullptr;
+    int *d_tapk = nullptr;
+    int nfg = 0;
+    bool ok = false;      // geometry fits the kernel
+};
+hipError_t bic_multi_tables_create(int w, int h, int M, const float *fwhm, int nfg,
+                                   BicTablesMulti &t);
+void bic_multi_tables_free(BicTablesMulti &t);
+
+// Launch shape of the emit pass: nw waves x ppw phases per block; pg_taps the longest tap
+// list of one block's phase group (LDS sizing).
+struct EmitShape {
+    int nw = 8, ppw = 1, pg_taps = 0;
+    float ms = 0.0f;
+};
+// Shapes whose LDS stays within 64 KiB (two blocks per CU), default first.
+std::vector<EmitShape> emit_candidate_shapes(const BicTablesMulti &t, int M);
+BicRgcArgs bic_args_multi(const BicTablesMulti &t, const EmitShape &sh, const float2 *U,
+                          const float *px, int w, int h, int M);
+// Time every candidate on `a` (reps launches after a warm-up) and keep the fastest.
+hipError_t emit_autotune(const BicTablesMulti &t, BicRgcArgs a, int M, long cstride, int reps,
+                         hipStream_t s, EmitShape *best);
+
 }  // namespace esrrf

@@ -21,6 +21,25 @@ struct Tap {
     float W;
 };
 
+// Taps of several fwhm values at once (the parameter sweep's emit pass): the union of their
+// gates in the reference's j-then-i order, each tap with the distance weight of every fwhm
+// of the group (0 where that fwhm's gate excludes it).
+constexpr int SWEEP_NFG = 5;   // fwhm values per union table
+struct TapM {
+    int off;
+    float a, b;
+    float W[SWEEP_NFG];
+};
+static_assert(sizeof(TapM) == 32, "TapM is two float4");
+
+struct PhaseM {
+    int first_tap, ntaps;    // into the TapM array; ntaps a multiple of 4
+    int du, dv;
+    float wx[4], wy[4];
+    int sx, sy, pad0, pad1;
+    float wsum[8];           // per fwhm: sum of W over its gated taps, reference order
+};
+
 // Geometry of one (x-class, y-class, sub-pixel phase) of a magnified pixel.
 struct Phase {
     int first_tap;  // index into the Tap array
@@ -127,7 +146,14 @@ hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h
 // *kname (when given) receives the name of the kernel launched.
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s,
                           const char **kname = nullptr);
-size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm);   // LDS of the frame-pair kernel
+size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm);
+// The sweep's emit pass over a union table of nfg <= SWEEP_NFG fwhm values (k_bic_emit_multi):
+// a.nw waves x ppw (1 or 2) phases per block, pg_taps = the largest tap count of one block's
+// phases (LDS sizing), CGLH of fwhm f at a.emitC + f * cstride.
+hipError_t launch_bic_emit_multi(const BicRgcArgs &a, int ppw, const TapM *taps,
+                                 const PhaseM *phases, const int *tapk, int nfg, int pg_taps,
+                                 long cstride, hipStream_t s);
+size_t bic_emit_multi_lds_bytes(const BicRgcArgs &a, int pg_taps);   // LDS of the frame-pair kernel
 hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
 // nbatch images `ostride` floats apart (MP maps `mstride` floats apart) in one launch
 hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch = 1, long ostride = 0);

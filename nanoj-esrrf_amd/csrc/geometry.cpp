@@ -48,6 +48,18 @@ static float cubic_host(float x)
     return z;
 }
 
+// distanceWeight = d*exp(-(d*d)/tSS); then w*w*w*w, left to right (liveSRRF.cl:296-297)
+static float tap_weight(float d, float tSS)
+{
+    volatile float e = -(d * d);
+    e = e / tSS;
+    volatile float w1 = d * std::exp((float)e);
+    volatile float W = w1 * w1;
+    W = W * w1;
+    W = W * w1;
+    return W;
+}
+
 static int class_of(int X, int xs_log2)
 {
     if (X < (1 << xs_log2)) return 0;
@@ -111,15 +123,8 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
                             d2 = d2 + dy2;
                             const float d = std::sqrt((float)d2);
                             if (!(d != 0.0f && d <= tSO)) continue;
-                            // distanceWeight = d*exp(-(d*d)/tSS); then w*w*w*w, left to
-                            // right (liveSRRF.cl:296-297)
-                            volatile float e = -(d * d);
-                            e = e / tSS;
-                            volatile float w1 = d * std::exp((float)e);
-                            volatile float W = w1 * w1;
-                            W = W * w1;
-                            W = W * w1;
-                            wsum = wsum + (float)W;
+                            const float W = tap_weight(d, tSS);
+                            wsum = wsum + W;
                             const int kxr = bx - 2 * Xr + i;
                             const int kyr = by - 2 * Yr + j;
                             const int gx = kxr + g.G0, gy = kyr + g.G0;
@@ -182,6 +187,117 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
         g.tapWd.push_back(0.0f);
     }
     // every lane (lx, ly in [0, 8)) must stay inside the 4-plane LDS tile
+    if (g.max_off + 7 * g.PP + 7 >= 4 * g.PP * g.PH) return false;
+    if (g.PW > g.PP) return false;
+    return true;
+}
+
+bool build_bic_geometry_multi(int w, int h, int M, const float *radius, const float *tSS,
+                              const float *tSO, int nf, BicGeometryMulti &g)
+{
+    if (nf < 1 || nf > SWEEP_NFG) return false;
+    float rmax = radius[0], tmax = tSO[0];
+    for (int f = 1; f < nf; f++) {
+        rmax = std::max(rmax, radius[f]);
+        tmax = std::max(tmax, tSO[f]);
+    }
+    g.nf = nf;
+    g.R2lo = -(int)(2.0f * rmax);
+    g.R2hi = (int)(2.0f * rmax + 1.0f);
+    if (g.R2hi - g.R2lo > 120) return false;
+    g.G0 = 2 * ((-g.R2lo + 1) / 2);
+    int TW = 16 + g.R2hi + g.G0;
+    if (TW & 1) TW++;
+    g.TW = g.TH = TW;
+    g.PW = g.PH = TW / 2;
+    g.PP = 24;
+    while (g.PP < g.PW) g.PP += 32;
+    int xs = 8;
+    while (xs < 1 - g.R2lo) xs *= 2;
+    int lg = 0;
+    while ((1 << lg) < xs) lg++;
+    g.xs_log2 = lg;
+    g.ncls_x = class_of(w > 0 ? w - 1 : 0, lg) + 1;
+    g.ncls_y = class_of(h > 0 ? h - 1 : 0, lg) + 1;
+    const int MM = M * M;
+    const int planesz = g.PP * g.PH;
+    g.taps.clear();
+    g.tapk.clear();
+    g.phases.assign((size_t)g.ncls_x * g.ncls_y * MM, PhaseM{});
+    g.max_off = 0;
+    g.max_phase_taps = 0;
+    for (int cy = 0; cy < g.ncls_y; cy++) {
+        for (int cx = 0; cx < g.ncls_x; cx++) {
+            const int Xr = rep_of(cx, lg), Yr = rep_of(cy, lg);
+            for (int sy = 0; sy < M; sy++) {
+                for (int sx = 0; sx < M; sx++) {
+                    PhaseM &P = g.phases[((size_t)cy * g.ncls_x + cx) * MM + sy * M + sx];
+                    const int xM = Xr * M + sx, yM = Yr * M + sy;
+                    const float xc = (float)((xM + 0.5) / M);
+                    const float yc = (float)((yM + 0.5) / M);
+                    const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc);
+                    P.first_tap = (int)g.taps.size();
+                    float wsum[SWEEP_NFG] = {};
+                    for (int j = g.R2lo; j <= g.R2hi; j++) {
+                        const float vy = ((float)by + (float)j) / 2.0f;
+                        for (int i = g.R2lo; i <= g.R2hi; i++) {
+                            const float vx = ((float)bx + (float)i) / 2.0f;
+                            volatile float dx = vx - xc;
+                            volatile float dy = vy - yc;
+                            volatile float d2 = dx * dx;
+                            volatile float dy2 = dy * dy;
+                            d2 = d2 + dy2;
+                            const float d = std::sqrt((float)d2);
+                            if (!(d != 0.0f && d <= tmax)) continue;
+                            TapM t{};
+                            for (int f = 0; f < nf; f++) {
+                                // the same window and gate as fwhm f's own tables
+                                const int lo = -(int)(2.0f * radius[f]);
+                                const int hi = (int)(2.0f * radius[f] + 1.0f);
+                                if (j < lo || j > hi || i < lo || i > hi || !(d <= tSO[f]))
+                                    continue;
+                                t.W[f] = tap_weight(d, tSS[f]);
+                                wsum[f] = wsum[f] + t.W[f];
+                            }
+                            const int kxr = bx - 2 * Xr + i;
+                            const int kyr = by - 2 * Yr + j;
+                            const int gx = kxr + g.G0, gy = kyr + g.G0;
+                            t.off = ((gy & 1) * 2 + (gx & 1)) * planesz + (gy >> 1) * g.PP + (gx >> 1);
+                            t.a = (float)dx / d;
+                            t.b = (float)dy / d;
+                            if (t.off > g.max_off) g.max_off = t.off;
+                            g.taps.push_back(t);
+                            g.tapk.push_back((kxr + 128) | ((kyr + 128) << 8));
+                        }
+                    }
+                    while (((int)g.taps.size() - P.first_tap) % 4 != 0) {
+                        g.taps.push_back(TapM{});
+                        g.tapk.push_back(128 | (128 << 8));
+                    }
+                    P.ntaps = (int)g.taps.size() - P.first_tap;
+                    g.max_phase_taps = std::max(g.max_phase_taps, P.ntaps);
+                    for (int f = 0; f < SWEEP_NFG; f++) P.wsum[f] = f < nf ? wsum[f] : 1.0f;
+                    P.sx = sx;
+                    P.sy = sy;
+                    volatile float xi = (float)xM / (float)M;
+                    xi = xi - 0.5f;
+                    volatile float yi = (float)yM / (float)M;
+                    yi = yi - 0.5f;
+                    const int u0 = (int)std::floor((float)xi), v0 = (int)std::floor((float)yi);
+                    P.du = u0 - Xr;
+                    P.dv = v0 - Yr;
+                    for (int k = 0; k < 4; k++) {
+                        P.wx[k] = cubic_host((float)xi - (float)(u0 - 1 + k));
+                        P.wy[k] = cubic_host((float)yi - (float)(v0 - 1 + k));
+                    }
+                }
+            }
+        }
+    }
+    for (int k = 0; k < TAP_SPARE; k++) {
+        g.taps.push_back(TapM{});
+        g.tapk.push_back(128 | (128 << 8));
+    }
     if (g.max_off + 7 * g.PP + 7 >= 4 * g.PP * g.PH) return false;
     if (g.PW > g.PP) return false;
     return true;

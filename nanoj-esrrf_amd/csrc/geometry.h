@@ -22,6 +22,22 @@ struct BicGeometry {
 };
 
 bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO, BicGeometry &g);
+
+// Union tables of nf <= SWEEP_NFG fwhm values of one magnification: the tile layout of the
+// widest window, the taps of the widest gate; tap t carries W[f] of fwhm f (0 outside its
+// gate) and phase wsum[f] sums fwhm f's weights in the same order as build_bic_geometry.
+struct BicGeometryMulti {
+    int R2lo = 0, R2hi = 0, G0 = 0, TW = 0, TH = 0, PW = 0, PH = 0, PP = 0;
+    int xs_log2 = 3, ncls_x = 1, ncls_y = 1;
+    int max_off = 0;
+    int max_phase_taps = 0;   // longest tap list of one phase
+    int nf = 0;
+    std::vector<TapM> taps;
+    std::vector<PhaseM> phases;
+    std::vector<int> tapk;
+};
+bool build_bic_geometry_multi(int w, int h, int M, const float *radius, const float *tSS,
+                              const float *tSO, int nf, BicGeometryMulti &g);
 long long count_active_samples_bic(int w, int h, int M, float radius, float tSO);
 
 }  // namespace esrrf

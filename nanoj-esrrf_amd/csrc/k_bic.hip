@@ -1047,6 +1047,254 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// k_bic_emit_multi: the parameter sweep's emit pass for up to SWEEP_NFG fwhm values of one
+// magnification at once.  A sample's angular term h = (1 - |sin|)[cos < 0] - 1 depends on
+// the tap's offset and direction only; the fwhm sets the weights and the gate.  The union
+// tables (build_bic_geometry_multi) list the widest gate's taps in the reference's order,
+// each with the weight of every fwhm of the group (0 outside its gate), so one h per tap
+// and frame feeds NFG accumulators: fwhm f's sum sees its own taps in its own order and
+// adds W * h = +0 for the others, which leaves it bit for bit what k_bic_rgc2 (mode 1)
+// computes from fwhm f's own tables.  The intensity is interpolated once per phase and
+// frame.  Emits CGLH of fwhm f to emitC + f * cstride and the intensity to emitV, both
+// phase-major [frame][M*M][h][w].  One block: an 8x8 macro-pixel tile and PG = nw * PPW
+// consecutive sub-pixel phases (grid.y: phase groups), whose taps alone are staged in LDS;
+// two frames per pass.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float tap_h_ab(float2 u, float ta, float tb)
+{
+    float c2 = fmaf(u.x, -ta, -1.0f);
+    c2 = fmaf(u.y, -tb, c2);
+    float s = u.y * ta;
+    s = fmaf(u.x, -tb, s);
+    return __builtin_amdgcn_fmed3f(-1.0f, -fabsf(s), c2);
+}
+
+size_t bic_emit_multi_lds_bytes(const BicRgcArgs &a, int pg_taps)
+{
+    return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float2) * RAW_SZ +
+           sizeof(TapM) * ((size_t)pg_taps + TAP_SPARE) +
+           sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
+}
+
+template <int PPW, int NFG>
+__global__ void __launch_bounds__(1024) k_bic_emit_multi(const BicRgcArgs a,
+                                                         const TapM *__restrict__ taps,
+                                                         const PhaseM *__restrict__ phases,
+                                                         const int *__restrict__ tapk,
+                                                         int pg_taps, long cstride)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4 *lU = reinterpret_cast<float4 *>(smem);
+    const int usz = 4 * a.PP * a.PH;
+    float2 *lR = reinterpret_cast<float2 *>(lU + usz);
+    float4 *lT = reinterpret_cast<float4 *>(lR + RAW_SZ);   // TapM as two float4 each
+    int *lPlan = reinterpret_cast<int *>(lT + 2 * (pg_taps + TAP_SPARE));
+
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lx = lane & 7, ly = lane >> 3;
+    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int X0 = tX << 3, Y0 = tY << 3;
+    const int X = X0 + lx, Y = Y0 + ly;
+    const int M = a.M, MM = M * M;
+    const int w = a.w, h = a.h;
+    const int ph0 = blockIdx.y * (a.nw * PPW);
+    const int phl = min(ph0 + a.nw * PPW, MM) - 1;
+    const PhaseM *PHS = phases + (axis_class(Y0, a.xs_log2) * a.ncls_x +
+                                  axis_class(X0, a.xs_log2)) * MM;
+    const int tb0 = PHS[ph0].first_tap;
+    const int tb1 = PHS[phl].first_tap + PHS[phl].ntaps;
+    const bool interior = (2 * X0 + a.R2lo >= 1) && (2 * (X0 + 7) + 1 + a.R2hi <= 2 * w - 1) &&
+                          (2 * Y0 + a.R2lo >= 1) && (2 * (Y0 + 7) + 1 + a.R2hi <= 2 * h - 1);
+    const bool inimg = (X < w) && (Y < h);
+    const long wh = (long)w * h;
+
+    float invw[PPW][NFG];
+    int ibase[PPW];
+#pragma unroll
+    for (int p = 0; p < PPW; p++) {
+        const int ph = ph0 + wave * PPW + p;
+        ibase[p] = -1;
+#pragma unroll
+        for (int f = 0; f < NFG; f++) invw[p][f] = 0.0f;
+        if (ph < MM) {
+            const PhaseM &P = PHS[ph];
+            float ws[NFG];
+#pragma unroll
+            for (int f = 0; f < NFG; f++) ws[f] = P.wsum[f];
+            if (!interior) {
+                // border: sum only the taps inside the 0 < v < size gates, reference order
+#pragma unroll
+                for (int f = 0; f < NFG; f++) ws[f] = 0.0f;
+                for (int t = 0; t < P.ntaps; t++) {
+                    const int k = tapk[P.first_tap + t];
+                    const int kx = 2 * X + (k & 255) - 128;
+                    const int ky = 2 * Y + ((k >> 8) & 255) - 128;
+                    if (kx >= 1 && kx <= 2 * w - 1 && ky >= 1 && ky <= 2 * h - 1) {
+#pragma unroll
+                        for (int f = 0; f < NFG; f++) ws[f] += taps[P.first_tap + t].W[f];
+                    }
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < NFG; f++) invw[p][f] = 1.0f / ws[f];
+            const int u0 = X + P.du, v0 = Y + P.dv;
+            if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2)
+                ibase[p] = (v0 - 1 - (Y0 - 2)) * RAW_P + (u0 - 1 - (X0 - 2));
+        }
+    }
+
+    const int wI = 2 * w, hI = 2 * h;
+    const long whI = (long)wI * hI;
+    const int kxo = 2 * X0 - a.G0, kyo = 2 * Y0 - a.G0;
+    const int planesz = a.PP * a.PH;
+    const int base = ly * a.PP + lx;
+    const int pw = a.TW >> 1;
+    const int nU = 4 * a.PH * pw;
+    for (int e = tid; e < nU; e += nthr) {
+        const int pl = e / (a.PH * pw);
+        const int r = (e - pl * a.PH * pw) / pw;
+        const int c = e - pl * a.PH * pw - r * pw;
+        const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+        lPlan[e] = (pl * planesz + r * a.PP + c) | (gy << 15) | (gx << 23);
+    }
+    {
+        const float4 *T4 = reinterpret_cast<const float4 *>(taps + tb0);
+        for (int e = tid; e < 2 * (tb1 - tb0 + TAP_SPARE); e += nthr) lT[e] = T4[e];
+    }
+    __syncthreads();
+
+    auto fill = [&](int f, bool two) {
+        const float2 *Ua = a.U + whI * f;
+        for (int e = tid; e < nU; e += nthr) {
+            const int pk = lPlan[e];
+            const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
+            const int kx = kxo + gx, ky = kyo + gy;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI) {
+                const long g = (long)ky * wI + kx;
+                const float2 ua = Ua[g];
+                const float2 ub = two ? Ua[g + whI] : make_float2(0.0f, 0.0f);
+                v = make_float4(ua.x, ua.y, ub.x, ub.y);
+            }
+            lU[pk & 0x7fff] = v;
+        }
+        const float *Pa = a.px + wh * f;
+        for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
+            const int r = e / RAW_T, c = e - r * RAW_T;
+            const long g = (long)min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+            lR[r * RAW_P + c] = make_float2(Pa[g], two ? Pa[g + wh] : 0.0f);
+        }
+    };
+
+    fill(a.frame0, a.nframes > 1);
+    __syncthreads();
+
+    for (int fi = 0; fi < a.nframes; fi += 2) {
+        const bool two = fi + 1 < a.nframes;
+#pragma unroll
+        for (int p = 0; p < PPW; p++) {
+            const int ph = ph0 + wave * PPW + p;
+            if (ph >= MM) continue;
+            const PhaseM &P = PHS[ph];
+            float acc[NFG][2];
+#pragma unroll
+            for (int f = 0; f < NFG; f++) acc[f][0] = acc[f][1] = 0.0f;
+            const float4 *Tl = lT + 2 * (P.first_tap - tb0);
+#pragma unroll 1
+            for (int t = 0; t < P.ntaps; t += 4) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float4 th = Tl[2 * (t + k)];
+                    const float4 tw = Tl[2 * (t + k) + 1];
+                    const float4 u = lU[base + __float_as_int(th.x)];
+                    const float h0 = tap_h_ab(make_float2(u.x, u.y), th.y, th.z);
+                    const float h1 = tap_h_ab(make_float2(u.z, u.w), th.y, th.z);
+                    const float W[5] = {th.w, tw.x, tw.y, tw.z, tw.w};
+#pragma unroll
+                    for (int f = 0; f < NFG; f++) {
+                        acc[f][0] = fmaf(W[f], h0, acc[f][0]);
+                        acc[f][1] = fmaf(W[f], h1, acc[f][1]);
+                    }
+                }
+            }
+            // interpolated intensity at (xM/M - 0.5, yM/M - 0.5) (liveSRRF.cl:361)
+            float va, vb;
+            if (ibase[p] >= 0) {
+                const float2 *row = lR + ibase[p];
+                float qa = 0.0f, qb = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float pa = 0.0f, pb = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float2 r2 = row[j * RAW_P + i];
+                        pa = fmaf(r2.x, P.wx[i], pa);
+                        pb = fmaf(r2.y, P.wx[i], pb);
+                    }
+                    qa = fmaf(pa, P.wy[j], qa);
+                    qb = fmaf(pb, P.wy[j], qb);
+                }
+                va = qa;
+                vb = qb;
+            } else {
+                const int xM = X * M + P.sx, yM = Y * M + P.sy;
+                const float xf = (float)xM / (float)M - 0.5f;
+                const float yf = (float)yM / (float)M - 0.5f;
+                const float *lRf = reinterpret_cast<const float *>(lR);
+                va = interp_tile_s<2>(lRf, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
+                vb = interp_tile_s<2>(lRf + 1, RAW_P, X0 - 2, Y0 - 2, w, h, xf, yf);
+            }
+            if (inimg) {
+                const long e = (((long)(a.frame0 + fi) * MM + ph) * h + Y) * w + X;
+                a.emitV[e] = va;
+                if (two) a.emitV[e + (long)MM * wh] = vb;
+#pragma unroll
+                for (int f = 0; f < NFG; f++) {
+                    float *Cf = a.emitC + cstride * f;
+                    Cf[e] = (P.wsum[f] + acc[f][0]) * invw[p][f];
+                    if (two) Cf[e + (long)MM * wh] = (P.wsum[f] + acc[f][1]) * invw[p][f];
+                }
+            }
+        }
+        if (fi + 2 < a.nframes) {
+            __syncthreads();   // every wave is done with this pair's tiles
+            fill(a.frame0 + fi + 2, fi + 3 < a.nframes);
+            __syncthreads();
+        }
+    }
+}
+
+template <int PPW>
+static hipError_t launch_emit_multi_p(const BicRgcArgs &a, const TapM *taps, const PhaseM *ph,
+                                      const int *tapk, int nfg, int pg_taps, long cstride,
+                                      hipStream_t s)
+{
+    const int MM = a.M * a.M;
+    const dim3 grid(a.tiles_x * a.tiles_y, (MM + a.nw * PPW - 1) / (a.nw * PPW));
+    const size_t lds = bic_emit_multi_lds_bytes(a, pg_taps);
+    switch (nfg) {
+    case 1: hipLaunchKernelGGL((k_bic_emit_multi<PPW, 1>), grid, dim3(64 * a.nw), lds, s, a, taps, ph, tapk, pg_taps, cstride); break;
+    case 2: hipLaunchKernelGGL((k_bic_emit_multi<PPW, 2>), grid, dim3(64 * a.nw), lds, s, a, taps, ph, tapk, pg_taps, cstride); break;
+    case 3: hipLaunchKernelGGL((k_bic_emit_multi<PPW, 3>), grid, dim3(64 * a.nw), lds, s, a, taps, ph, tapk, pg_taps, cstride); break;
+    case 4: hipLaunchKernelGGL((k_bic_emit_multi<PPW, 4>), grid, dim3(64 * a.nw), lds, s, a, taps, ph, tapk, pg_taps, cstride); break;
+    default: hipLaunchKernelGGL((k_bic_emit_multi<PPW, 5>), grid, dim3(64 * a.nw), lds, s, a, taps, ph, tapk, pg_taps, cstride); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bic_emit_multi(const BicRgcArgs &a, int ppw, const TapM *taps,
+                                 const PhaseM *phases, const int *tapk, int nfg, int pg_taps,
+                                 long cstride, hipStream_t s)
+{
+    if (a.nframes <= 0) return hipSuccess;
+    if (nfg < 1 || nfg > SWEEP_NFG || a.nw < 1 || a.nw > 16) return hipErrorInvalidValue;
+    return ppw == 2 ? launch_emit_multi_p<2>(a, taps, phases, tapk, nfg, pg_taps, cstride, s)
+                    : launch_emit_multi_p<1>(a, taps, phases, tapk, nfg, pg_taps, cstride, s);
+}
+
 size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm)
 {
     return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float2) * RAW_SZ +

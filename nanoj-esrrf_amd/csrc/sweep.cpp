@@ -6,8 +6,10 @@
 // (:452-458, :800-844).  Here:
 //   * the frames go to the device once and their unit tap gradients are computed once
 //     (they depend on neither magnification, fwhm nor sensitivity);
-//   * per (magnification, fwhm) one RGC launch over the largest frame count emits the
-//     normalised CGLH and intensity of every pixel and frame;
+//   * per magnification and group of up to five fwhm values one RGC launch over the largest
+//     frame count emits the normalised CGLH of every fwhm of the group and the intensity of
+//     every pixel and frame (one angular term per sample serves every fwhm: the fwhm only
+//     sets the weights and the gate, k_bic_emit_multi);
 //   * one accumulation launch turns them into the cumulants of every (frame count,
 //     sensitivity) configuration;
 //   * VAR and MP correction finish each configuration (liveSRRF.cl:427-505).
@@ -27,6 +29,7 @@
 #include "bic_launch.h"
 #include "esrrf_internal.h"
 #include "sweep_internal.h"
+#include "variants.h"
 
 using namespace esrrf;
 
@@ -45,10 +48,14 @@ struct esrrf_sweep {
     void *d_in = nullptr;          // frames as handed over (raw dtype), nfmax frames
     float *d_px = nullptr;         // [nfmax][h][w]
     float2 *d_U = nullptr;         // [nfmax][2h][2w]
-    float *d_C = nullptr, *d_V = nullptr;   // emitted CGLH / intensity, largest M
+    float *d_C = nullptr, *d_V = nullptr;   // emitted CGLH [fwhm of group][..] / intensity
+    long cstride = 0;              // floats between the CGLH planes of two fwhm
+    bool multi = true;             // one emit launch per group of fwhm (else per fwhm)
     float *d_dummy = nullptr;      // out/prev of the emit launches (never written)
     std::vector<std::vector<BicTables>> tabs;   // [iM][iF]
     std::vector<std::vector<BicShape>> shape;   // [iM][iFwhm]
+    std::vector<std::vector<BicTablesMulti>> mtabs;   // [iM][group of SWEEP_NFG fwhm]
+    std::vector<std::vector<EmitShape>> mshape;       // [iM][group]
     std::vector<float *> d_out;    // [iM] configurations [nf][S][fwhm][4][hM][wM]
     std::vector<float *> d_mp;     // [iM] MP maps [nf][S][fwhm][3][M][M]
     float *d_tmp = nullptr;        // read-back staging
@@ -90,6 +97,8 @@ void sweep_free(esrrf_sweep *s)
         if (p) (void)hipFree(p);
     for (auto &v : s->tabs)
         for (auto &t : v) bic_tables_free(t);
+    for (auto &v : s->mtabs)
+        for (auto &t : v) bic_multi_tables_free(t);
     for (float *p : s->d_out)
         if (p) (void)hipFree(p);
     for (float *p : s->d_mp)
@@ -111,24 +120,40 @@ int enqueue_run(esrrf_sweep *s, int dtype)
         const int M = s->Ms[im];
         const long whM = wh * M * M;
         for (int fi = 0; fi < nF; fi++) {
-            const BicTables &t = s->tabs[im][fi];
-            BicRgcArgs a = bic_args(t, s->shape[im][fi], s->d_U, s->d_px, s->w, s->h, M);
-            a.mode = 1;
-            a.emitC = s->d_C;
-            a.emitV = s->d_V;
-            a.out = s->d_dummy;
-            a.prev = s->d_dummy;
-            a.frame0 = 0;
-            a.nframes = s->nfmax;
-            a.first_init = 1;
-            a.rn = 1.0f;
-            a.rn1 = 1.0f;
-            a.sens = 1.0f;
-            a.sens_int = 1;
-            a.int_weighting = s->int_weighting;
-            SCHK(s, launch_bic_rgc(a, s->shape[im][fi].ppw, st));
+            const int slot = s->multi ? fi % SWEEP_NFG : 0;
+            if (s->multi && slot == 0) {
+                // CGLH of every fwhm of the group, intensity once
+                const int g = fi / SWEEP_NFG;
+                const BicTablesMulti &t = s->mtabs[im][g];
+                const EmitShape &sh = s->mshape[im][g];
+                BicRgcArgs a = bic_args_multi(t, sh, s->d_U, s->d_px, s->w, s->h, M);
+                a.emitC = s->d_C;
+                a.emitV = s->d_V;
+                a.frame0 = 0;
+                a.nframes = s->nfmax;
+                a.int_weighting = s->int_weighting;
+                SCHK(s, launch_bic_emit_multi(a, sh.ppw, t.d_taps, t.d_phases, t.d_tapk, t.nfg,
+                                              sh.pg_taps, s->cstride, st));
+            } else if (!s->multi) {
+                const BicTables &t = s->tabs[im][fi];
+                BicRgcArgs a = bic_args(t, s->shape[im][fi], s->d_U, s->d_px, s->w, s->h, M);
+                a.mode = 1;
+                a.emitC = s->d_C;
+                a.emitV = s->d_V;
+                a.out = s->d_dummy;
+                a.prev = s->d_dummy;
+                a.frame0 = 0;
+                a.nframes = s->nfmax;
+                a.first_init = 1;
+                a.rn = 1.0f;
+                a.rn1 = 1.0f;
+                a.sens = 1.0f;
+                a.sens_int = 1;
+                a.int_weighting = s->int_weighting;
+                SCHK(s, launch_bic_rgc(a, s->shape[im][fi].ppw, st));
+            }
             SweepAccArgs c{};
-            c.C = s->d_C;
+            c.C = s->d_C + s->cstride * slot;
             c.V = s->d_V;
             c.out = s->d_out[im];
             c.w = s->w;
@@ -170,6 +195,25 @@ int enqueue_run(esrrf_sweep *s, int dtype)
 
 int tune(esrrf_sweep *s)
 {
+    if (s->multi) {
+        // per (magnification, group of fwhm): the emit launch over up to 32 frames
+        for (size_t im = 0; im < s->Ms.size(); im++) {
+            const int M = s->Ms[im];
+            for (size_t g = 0; g < s->mtabs[im].size(); g++) {
+                const BicTablesMulti &t = s->mtabs[im][g];
+                BicRgcArgs a = bic_args_multi(t, s->mshape[im][g], s->d_U, s->d_px, s->w, s->h,
+                                              M);
+                a.emitC = s->d_C;
+                a.emitV = s->d_V;
+                a.nframes = std::min(s->nfmax, 32);
+                a.int_weighting = s->int_weighting;
+                EmitShape best;
+                SCHK(s, emit_autotune(t, a, M, s->cstride, 2, s->stream, &best));
+                s->mshape[im][g] = best;
+            }
+        }
+        return ESRRF_OK;
+    }
     // per (magnification, fwhm): the emit launch over up to 32 frames
     for (size_t im = 0; im < s->Ms.size(); im++) {
         const int M = s->Ms[im];
@@ -280,6 +324,7 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
     s->do_mp = p->do_mp_map_correction ? 1 : 0;
     s->autotune = p->autotune ? 1 : 0;
     s->use_graph = p->use_graph ? 1 : 0;
+    s->multi = !kernel_variants_from_env().sweep_per_fwhm;
     for (int M : s->Ms)
         if (M < 1 || M > 8) return bad(ESRRF_ERR_UNSUPPORTED);
     for (int nf : s->nfs) {
@@ -297,13 +342,15 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
     const long wh = (long)s->w * s->h;
     int MMmax = 1;
     for (int M : s->Ms) MMmax = std::max(MMmax, M * M);
+    s->cstride = wh * MMmax * s->nfmax;
     auto alloc = [&](void **ptr, size_t bytes) {
         return hipMalloc(ptr, bytes ? bytes : 4) == hipSuccess;
     };
     if (!alloc(&s->d_in, sizeof(float) * wh * s->nfmax) ||
         !alloc((void **)&s->d_px, sizeof(float) * wh * s->nfmax) ||
         !alloc((void **)&s->d_U, sizeof(float2) * 4 * wh * s->nfmax) ||
-        !alloc((void **)&s->d_C, sizeof(float) * wh * MMmax * s->nfmax) ||
+        !alloc((void **)&s->d_C, sizeof(float) * wh * MMmax * s->nfmax *
+                                     (s->multi ? std::min((int)s->fwhm.size(), SWEEP_NFG) : 1)) ||
         !alloc((void **)&s->d_V, sizeof(float) * wh * MMmax * s->nfmax) ||
         !alloc((void **)&s->d_dummy, 64))
         return bad(ESRRF_ERR_NOMEM);
@@ -323,6 +370,22 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
         s->shape[im].resize(s->fwhm.size());
         for (size_t fi = 0; fi < s->fwhm.size(); fi++)
             s->shape[im][fi] = bic_default_shape(s->tabs[im][fi], M);
+        if (s->multi) {
+            const int ng = ((int)s->fwhm.size() + SWEEP_NFG - 1) / SWEEP_NFG;
+            s->mtabs.resize(s->Ms.size());
+            s->mshape.resize(s->Ms.size());
+            s->mtabs[im].resize(ng);
+            s->mshape[im].resize(ng);
+            for (int g = 0; g < ng; g++) {
+                const int f0 = g * SWEEP_NFG;
+                const int nfg = std::min(SWEEP_NFG, (int)s->fwhm.size() - f0);
+                if (bic_multi_tables_create(s->w, s->h, M, s->fwhm.data() + f0, nfg,
+                                            s->mtabs[im][g]) != hipSuccess)
+                    return bad(ESRRF_ERR_HIP);
+                if (!s->mtabs[im][g].ok) return bad(ESRRF_ERR_UNSUPPORTED);
+                s->mshape[im][g] = emit_candidate_shapes(s->mtabs[im][g], M)[0];
+            }
+        }
         if (!alloc((void **)&s->d_out[im], sizeof(float) * nc * 4 * wh * M * M) ||
             !alloc((void **)&s->d_mp[im], sizeof(float) * nc * 3 * M * M))
             return bad(ESRRF_ERR_NOMEM);
@@ -409,6 +472,15 @@ int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int i_fwhm, int *nw, int *ppw, 
         i_fwhm >= (int)s->fwhm.size())
         return ESRRF_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->multi) {
+        const EmitShape &m = s->mshape[i_mag][i_fwhm / SWEEP_NFG];
+        if (nw) *nw = m.nw;
+        if (ppw) *ppw = m.ppw;
+        if (taps_in_lds) *taps_in_lds = 1;
+        if (frames_per_pass) *frames_per_pass = 2;
+        if (ms) *ms = m.ms;
+        return ESRRF_OK;
+    }
     const BicShape &b = s->shape[i_mag][i_fwhm];
     if (nw) *nw = b.nw;
     if (ppw) *ppw = b.ppw;

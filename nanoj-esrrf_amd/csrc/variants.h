@@ -1,7 +1,7 @@
 // variants.h -- alternative kernels kept beside the defaults so that tests can show they give
 // the same results bit for bit (per-point gradient stencils, scattered FHT stores, per-voxel
 // 3-D kernels, ...).  Selected by ESRRF_* environment variables that are read once, when an
-// engine is created; launches only see the copy the engine holds.
+// engine (or sweep) is created; launches only see the copy it holds.
 #pragma once
 
 namespace esrrf {
@@ -16,6 +16,7 @@ struct KernelVariants {
     int fht_pair = 1;             // ESRRF_FHT_PAIR=0: single-pixel FHT RGC kernel
     int k3d_kernel = 3;           // ESRRF_3D_KERNEL=1/2: per-voxel 3-D RGC kernels
     int k3d_split = 2;            // ESRRF_3D_SPLIT=1: one wave per phase in k3d_rgc3
+    int sweep_per_fwhm = 0;       // ESRRF_SWEEP_PER_FWHM=1: one sweep emit launch per fwhm
 };
 
 // The variants the environment selects (defaults when unset).

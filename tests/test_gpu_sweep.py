@@ -109,7 +109,8 @@ def test_sweep_autotune_picks_a_supported_shape(pkg):
     tuned = pkg.ParametersSweep(24, 24, [5], [1.5], [2.0], [8], autotune=True, use_graph=False)
     tuned.run(st)
     sh = tuned.shape(0, 0)
-    assert sh["ms"] > 0 and sh["nw"] in (4, 8, 12, 16) and sh["ppw"] * sh["nw"] >= 25
+    # emit blocks hold a group of nw x ppw of the 25 phases (grid.y covers the groups)
+    assert sh["ms"] > 0 and sh["nw"] in (4, 8, 12, 16) and sh["ppw"] in (1, 2)
     assert sh["frames_per_pass"] in (1, 2)
     base = pkg.ParametersSweep(24, 24, [5], [1.5], [2.0], [8], autotune=False, use_graph=False)
     base.run(st)
@@ -219,3 +220,25 @@ def test_every_runnable_launch_shape_is_bit_identical(pkg, M, fwhm):
     assert len(ran) >= 4, ran
     # the default shape is among the runnable ones
     assert (shape0["nw"], shape0["taps_in_lds"], shape0["frames_per_pass"]) in ran
+
+
+@pytest.mark.parametrize("fwhms", [[1.0, 1.5, 2.0, 2.5, 3.0], [2.2, 0.9, 1.4, 3.0, 1.1, 2.7, 1.8]])
+def test_multi_fwhm_emit_bit_identical_to_per_fwhm_emit(pkg, monkeypatch, fwhms):
+    """One emit launch per magnification and group of up to five fwhm values (one angular
+    term per sample, a weight per fwhm, 0 outside its gate) gives every configuration the
+    same bits as one emit launch per fwhm; seven unsorted fwhm values make a group of five
+    and one of two."""
+    st = stack(14, 36, 28, seed=515)
+    mags, sens, nfs = [2, 5], [1.0, 2.5], [6, 14]
+    res = {}
+    for per in ("1", "0"):
+        monkeypatch.setenv("ESRRF_SWEEP_PER_FWHM", per)
+        sw = pkg.ParametersSweep(36, 28, mags, fwhms, sens, nfs, autotune=(per == "0"),
+                                 use_graph=True)
+        sw.run(st)
+        res[per] = {(im, iq, i_s, i_f): np.asarray(sw.read(im, iq, i_s, i_f)).copy()
+                    for im in range(len(mags)) for iq in range(len(nfs))
+                    for i_s in range(len(sens)) for i_f in range(len(fwhms))}
+        sw.close()
+    for k, v in res["1"].items():
+        assert np.array_equal(v.view(np.uint32), res["0"][k].view(np.uint32)), k
