@@ -195,6 +195,9 @@ typedef struct esrrf_sweep_params {
     int autotune;                       /* 1: time the RGC launch shapes per magnification */
     int use_graph;                      /* 1: capture the device sequence in a hipGraph   */
     int device;                         /* -1: fastest                                    */
+    int do_fht_interpolation;           /* 1: FHT mode, the reference sweep's default
+                                         * (:117,452): frames upsampled once per
+                                         * magnification, no MP correction (:762)         */
 } esrrf_sweep_params;
 int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out);
 int esrrf_sweep_destroy(esrrf_sweep *s);

@@ -84,6 +84,7 @@ class SweepParams(ctypes.Structure):
         ("n_frames", ctypes.POINTER(ctypes.c_int)), ("n_n_frames", ctypes.c_int),
         ("int_weighting", ctypes.c_int), ("do_mp_map_correction", ctypes.c_int),
         ("autotune", ctypes.c_int), ("use_graph", ctypes.c_int), ("device", ctypes.c_int),
+        ("do_fht_interpolation", ctypes.c_int),
     ]
 
 

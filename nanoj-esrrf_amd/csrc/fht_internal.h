@@ -36,6 +36,9 @@ struct FhtRgcArgs {
     int ntaps;
     float wsum;
     int tiles_x, tiles_y;
+    // non-null: emit each pixel's normalised CGLH (before the sensitivity power) per frame,
+    // [frame][H][W], instead of accumulating (the parameter sweep)
+    float *emitC;
 };
 
 struct FhtGenericArgs {

@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(256) k_fht_rgc(const FhtRgcArgs a,
     const float invw = 1.0f / ws;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
     const long pix = (long)yM * W + xM;
-    if (!a.first_init && inimg) {
+    if (!a.first_init && inimg && !a.emitC) {
         s0 = a.out[pix];
         s1 = a.out[pix + wh];
         s2 = a.out[pix + 2 * wh];
@@ -489,11 +489,15 @@ __global__ void __launch_bounds__(256) k_fht_rgc(const FhtRgcArgs a,
             acc = tap_acc_f(u2, c2, acc);
             acc = tap_acc_f(u3, c3, acc);
         }
+        if (a.emitC) {
+            if (inimg) a.emitC[wh * f + pix] = (a.wsum + acc) * invw;
+            continue;
+        }
         // unshifted frames: the intensity is the magnified pixel itself (:356)
         const float v = inimg ? a.px[wh * f + pix] : 0.0f;
         cumulate_f(s0, s1, s2, s3, s4, (a.wsum + acc) * invw, v, fi == 0 && a.first_init, a);
     }
-    if (inimg) {
+    if (inimg && !a.emitC) {
         a.out[pix] = s0;
         a.out[pix + wh] = s1;
         a.out[pix + 2 * wh] = s2;
@@ -573,7 +577,7 @@ __global__ void __launch_bounds__(512) k_fht_rgc2(const FhtRgcArgs a,
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
     float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f, b4 = 0.0f;
     const long pixa = (long)yM * W + xa, pixb = pixa + 8;
-    if (!a.first_init) {
+    if (!a.first_init && !a.emitC) {
         if (ina) {
             a0 = a.out[pixa];
             a1 = a.out[pixa + wh];
@@ -637,6 +641,14 @@ __global__ void __launch_bounds__(512) k_fht_rgc2(const FhtRgcArgs a,
             acc2 = tap_acc_f(make_float2(v3.x, v3.y), c3, acc2);
             acc3 = tap_acc_f(make_float2(v3.z, v3.w), c3, acc3);
         }
+        if (a.emitC) {
+            float *Cf = a.emitC + wh * f;
+            if (ina) Cf[pixa] = (a.wsum + acc0) * invwa;
+            if (inb) Cf[pixb] = (a.wsum + acc2) * invwb;
+            if (two && ina) Cf[pixa + wh] = (a.wsum + acc1) * invwa;
+            if (two && inb) Cf[pixb + wh] = (a.wsum + acc3) * invwb;
+            continue;
+        }
         // unshifted frames: the intensity is the magnified pixel itself (:356)
         const float *Pf = a.px + wh * f;
         const float va0 = ina ? Pf[pixa] : 0.0f, vb0 = inb ? Pf[pixb] : 0.0f;
@@ -650,6 +662,7 @@ __global__ void __launch_bounds__(512) k_fht_rgc2(const FhtRgcArgs a,
             cumulate_f(b0, b1, b2, b3, b4, (a.wsum + acc3) * invwb, vb1, false, a);
         }
     }
+    if (a.emitC) return;
     if (ina) {
         a.out[pixa] = a0;
         a.out[pixa + wh] = a1;

@@ -117,6 +117,82 @@ __global__ void __launch_bounds__(256) k_sweep_acc(const SweepAccArgs a)
     }
 }
 
+// FHT mode (liveSRRF_noInterpolation.cl:358-390): one thread per magnified pixel o; the
+// emitted CGLH is [frame][hM][wM] and the intensity is the upsampled frame itself (a.V, the
+// same layout); TAC2 keeps Prev = t (single 1/(n-1)).  The operations are those of the
+// FHT engine's per-frame accumulation, spelled out, so every configuration equals the
+// per-configuration engine bit for bit.
+template <int NQ, int SG, bool POW>
+__global__ void __launch_bounds__(256) k_sweep_acc_fht(const SweepAccArgs a)
+{
+    const long n = (long)a.w * a.M * a.h * a.M;
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const int si0 = blockIdx.y * SG;
+    float sens[SG];
+    int sint[SG];
+#pragma unroll
+    for (int g = 0; g < SG; g++) {
+        const int si = min(si0 + g, a.nS - 1);
+        sens[g] = a.sens[si];
+        sint[g] = a.sens_int[si];
+    }
+    float s0[SG][NQ], s1[SG][NQ], s2[SG][NQ], s3[NQ], tp[SG];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        s3[q] = 0.0f;
+#pragma unroll
+        for (int g = 0; g < SG; g++) s0[g][q] = s1[g][q] = s2[g][q] = 0.0f;
+    }
+#pragma unroll
+    for (int g = 0; g < SG; g++) tp[g] = 0.0f;
+    const bool intw = a.int_weighting != 0;
+#pragma unroll 4
+    for (int f = 0; f < a.nframes; f++) {
+        const float C = a.C[(long)f * n + o];
+        const float v = a.V[(long)f * n + o];
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            if (q < a.nq && f < a.nf[q])
+                s3[q] = f == 0 ? __fmul_rn(v, a.rn[q]) : fmaf(v, a.rn[q], s3[q]);
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            const float Cs = sweep_sens<POW>(C, sens[g], sint[g]);
+            const float t = intw ? __fmul_rn(v, Cs) : Cs;
+            const float e2 = intw ? __fmul_rn(__fmul_rn(t, v), Cs) : __fmul_rn(Cs, Cs);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                if (q < a.nq && f < a.nf[q]) {
+                    const float rn = a.rn[q], rn1 = a.rn1[q];
+                    if (f == 0) {
+                        s0[g][q] = __fmul_rn(t, rn);
+                        s1[g][q] = __fmul_rn(e2, rn);
+                        s2[g][q] = 0.0f;
+                    } else {
+                        s0[g][q] = fmaf(t, rn, s0[g][q]);
+                        s1[g][q] = fmaf(e2, rn, s1[g][q]);
+                        s2[g][q] = fmaf(__fmul_rn(t, tp[g]), rn1, s2[g][q]);
+                    }
+                }
+            }
+            tp[g] = t;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        if (q >= a.nq) continue;
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            if (si0 + g >= a.nS) continue;
+            float *out = a.out + ((long)(q * a.nS + si0 + g) * a.n_fwhm + a.i_fwhm) * 4 * n;
+            out[o] = s0[g][q];
+            out[o + n] = s1[g][q];
+            out[o + 2 * n] = s2[g][q];
+            out[o + 3 * n] = s3[q];
+        }
+    }
+}
+
 template <int NQ, int SG>
 static hipError_t launch_acc(const SweepAccArgs &a, hipStream_t s)
 {
@@ -124,10 +200,16 @@ static hipError_t launch_acc(const SweepAccArgs &a, hipStream_t s)
     const dim3 grid((unsigned)((n + 255) / 256), (unsigned)((a.nS + SG - 1) / SG));
     bool pow = false;
     for (int i = 0; i < a.nS; i++) pow = pow || a.sens_int[i] < 1 || a.sens_int[i] > 4;
-    if (pow)
+    if (a.fht) {
+        if (pow)
+            hipLaunchKernelGGL((k_sweep_acc_fht<NQ, SG, true>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_sweep_acc_fht<NQ, SG, false>), grid, dim3(256), 0, s, a);
+    } else if (pow) {
         hipLaunchKernelGGL((k_sweep_acc<NQ, SG, true>), grid, dim3(256), 0, s, a);
-    else
+    } else {
         hipLaunchKernelGGL((k_sweep_acc<NQ, SG, false>), grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

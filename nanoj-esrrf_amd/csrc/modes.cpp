@@ -336,4 +336,33 @@ void free_mode_state(ModeState &ms)
     ms.tiled3 = false;
 }
 
+int fht_taps_create(float radius, float tSS, float tSO, FhtTaps &t)
+{
+    ModeState tmp;
+    build_fht_taps(tmp, radius, tSS, tSO);
+    t.R = tmp.fht_R;
+    t.Rhi = tmp.fht_Rhi;
+    t.TW = tmp.fht_TW;
+    t.pitch = tmp.fht_pitch;
+    t.ntaps = tmp.ntaps;
+    t.wsum = tmp.tap_wsum;
+    if (!dmalloc(&t.d_taps, sizeof(Tap) * tmp.h_taps.size()) ||
+        !dmalloc(&t.d_tapk, sizeof(int) * tmp.h_tapk.size()))
+        return ESRRF_ERR_NOMEM;
+    hipError_t st = hipMemcpy(t.d_taps, tmp.h_taps.data(), sizeof(Tap) * tmp.h_taps.size(),
+                              hipMemcpyHostToDevice);
+    if (st == hipSuccess)
+        st = hipMemcpy(t.d_tapk, tmp.h_tapk.data(), sizeof(int) * tmp.h_tapk.size(),
+                       hipMemcpyHostToDevice);
+    return st == hipSuccess ? ESRRF_OK : ESRRF_ERR_HIP;
+}
+
+void fht_taps_free(FhtTaps &t)
+{
+    if (t.d_taps) (void)hipFree(t.d_taps);
+    if (t.d_tapk) (void)hipFree(t.d_tapk);
+    t.d_taps = nullptr;
+    t.d_tapk = nullptr;
+}
+
 }  // namespace esrrf

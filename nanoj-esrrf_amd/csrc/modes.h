@@ -87,4 +87,15 @@ int mode_read_gradients(int mode, const ModeCtx &c, ModeState &ms, float *out);
 long long mode_active_samples(int mode, const ModeState &ms);
 void free_mode_state(ModeState &ms);
 
+// Device tap table of the FHT-mode RGC for one fwhm (the table mode_create builds for its
+// own fwhm), for callers that run one FHT state with several fwhm values (the sweep).
+struct FhtTaps {
+    Tap *d_taps = nullptr;
+    int *d_tapk = nullptr;
+    int R = 0, Rhi = 0, TW = 0, pitch = 0, ntaps = 0;
+    float wsum = 0.0f;
+};
+int fht_taps_create(float radius, float tSS, float tSO, FhtTaps &t);
+void fht_taps_free(FhtTaps &t);
+
 }  // namespace esrrf

@@ -28,6 +28,8 @@
 #include "../../include/esrrf.h"
 #include "bic_launch.h"
 #include "esrrf_internal.h"
+#include "fht_internal.h"
+#include "modes.h"
 #include "sweep_internal.h"
 #include "variants.h"
 
@@ -51,6 +53,13 @@ struct esrrf_sweep {
     float *d_C = nullptr, *d_V = nullptr;   // emitted CGLH [fwhm of group][..] / intensity
     long cstride = 0;              // floats between the CGLH planes of two fwhm
     bool multi = true;             // one emit launch per group of fwhm (else per fwhm)
+    // FHT mode: per magnification the upsampled frames and their gradients, per fwhm the
+    // magnified-space tap table
+    int fht = 0;
+    std::vector<ModeState> fms;                 // [iM]
+    std::vector<float *> f_px, f_aux;           // [iM]
+    std::vector<float2 *> f_U;                  // [iM]
+    std::vector<std::vector<FhtTaps>> ftaps;    // [iM][iF]
     float *d_dummy = nullptr;      // out/prev of the emit launches (never written)
     std::vector<std::vector<BicTables>> tabs;   // [iM][iF]
     std::vector<std::vector<BicShape>> shape;   // [iM][iFwhm]
@@ -99,6 +108,13 @@ void sweep_free(esrrf_sweep *s)
         for (auto &t : v) bic_tables_free(t);
     for (auto &v : s->mtabs)
         for (auto &t : v) bic_multi_tables_free(t);
+    for (auto &v : s->ftaps)
+        for (auto &t : v) fht_taps_free(t);
+    for (auto &m : s->fms) free_mode_state(m);
+    for (float *p : s->f_px)
+        if (p) (void)hipFree(p);
+    for (float2 *p : s->f_U)
+        if (p) (void)hipFree(p);
     for (float *p : s->d_out)
         if (p) (void)hipFree(p);
     for (float *p : s->d_mp)
@@ -109,8 +125,85 @@ void sweep_free(esrrf_sweep *s)
 size_t n_conf(const esrrf_sweep *s) { return s->nfs.size() * s->sens.size() * s->fwhm.size(); }
 
 // The device sequence of one run: frames in d_in -> every configuration's result.
+// FHT mode: per magnification the frames are upsampled and their gradients formed once
+// (mode_prepare), then per fwhm one RGC launch emits the normalised CGLH of every magnified
+// pixel and frame and one accumulation launch forms every (frame count, sensitivity)
+// configuration with the FHT cumulants (liveSRRF_noInterpolation.cl:358-390); VAR
+// finishes them (no MP correction in FHT mode, LiveSRRF_CL.java:762).
+int enqueue_run_fht(esrrf_sweep *s, int dtype)
+{
+    const hipStream_t st = s->stream;
+    const long wh = (long)s->w * s->h;
+    const int nS = (int)s->sens.size(), nF = (int)s->fwhm.size(), nQ = (int)s->nfs.size();
+    for (size_t im = 0; im < s->Ms.size(); im++) {
+        const int M = s->Ms[im];
+        const long whM = wh * M * M;
+        ModeCtx c = {st, s->w, s->h, s->w, s->h, 1, M, s->w * M, s->h * M,
+                     s->f_px[im], s->f_U[im], s->f_aux[im], false};
+        int rc = mode_prepare(1, c, s->fms[im], s->d_in, dtype, s->nfmax);
+        if (rc != ESRRF_OK)
+            return sfail(s, rc, "FHT upsampling: %s", s->fms[im].err.c_str());
+        for (int fi = 0; fi < nF; fi++) {
+            const FhtTaps &t = s->ftaps[im][fi];
+            FhtRgcArgs a{};
+            a.U = s->f_U[im];
+            a.px = s->f_px[im];
+            a.taps = t.d_taps;
+            a.tapk = t.d_tapk;
+            a.W = s->w * M;
+            a.H = s->h * M;
+            a.frame0 = 0;
+            a.nframes = s->nfmax;
+            a.first_init = 1;
+            a.rn = a.rn1 = a.sens = 1.0f;
+            a.sens_int = 1;
+            a.int_weighting = s->int_weighting;
+            a.R = t.R;
+            a.Rhi = t.Rhi;
+            a.pair = s->fms[im].var.fht_pair;
+            a.TW = t.TW;
+            a.TH = t.TW;
+            a.pitch = t.pitch;
+            a.ntaps = t.ntaps;
+            a.wsum = t.wsum;
+            a.tiles_x = (a.W + 15) / 16;
+            a.tiles_y = (a.H + 15) / 16;
+            a.emitC = s->d_C;
+            SCHK(s, launch_fht_rgc(a, st));
+            SweepAccArgs q{};
+            q.C = s->d_C;
+            q.V = s->f_px[im];
+            q.out = s->d_out[im];
+            q.w = s->w;
+            q.h = s->h;
+            q.M = M;
+            q.nframes = s->nfmax;
+            q.nS = nS;
+            q.n_fwhm = nF;
+            q.i_fwhm = fi;
+            q.nq = nQ;
+            q.int_weighting = s->int_weighting;
+            q.fht = 1;
+            for (int k = 0; k < nQ; k++) {
+                q.nf[k] = s->nfs[k];
+                q.rn[k] = 1.0f / (float)s->nfs[k];
+                q.rn1[k] = 1.0f / (float)(s->nfs[k] - 1);
+            }
+            for (int k = 0; k < nS; k++) {
+                q.sens[k] = s->sens[k];
+                q.sens_int[k] = sens_exact_int(s->sens[k]);
+            }
+            SCHK(s, launch_sweep_acc(q, st));
+            float *out = s->d_out[im] + (long)fi * 4 * whM;
+            SCHK(s, launch_var(out, whM, st, nQ * nS, (long)nF * 4 * whM));
+        }
+    }
+    return ESRRF_OK;
+}
+
 int enqueue_run(esrrf_sweep *s, int dtype)
 {
+    if (s->fht) return enqueue_run_fht(s, dtype);
     const hipStream_t st = s->stream;
     const long wh = (long)s->w * s->h;
     SCHK(s, launch_convert_frames(s->d_in, dtype, s->d_px, wh * s->nfmax, st));
@@ -248,6 +341,7 @@ int run_impl(esrrf_sweep *s, const void *frames, int dtype, int n, bool on_devic
     // first nfmax frames (gui/ParametersSweep_.java:808-816 takes frames 1..nf)
     SCHK(s, hipMemcpyAsync(s->d_in, frames, bytes,
                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
+    if (s->fht) s->tuned = true;   // FHT mode has one launch shape
     if (s->autotune && !s->tuned) {
         // tuning needs the gradients of these frames; it runs eagerly, before any capture
         const long wh = (long)s->w * s->h;
@@ -325,6 +419,8 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
     s->autotune = p->autotune ? 1 : 0;
     s->use_graph = p->use_graph ? 1 : 0;
     s->multi = !kernel_variants_from_env().sweep_per_fwhm;
+    s->fht = p->do_fht_interpolation ? 1 : 0;
+    if (s->fht) s->multi = false;
     for (int M : s->Ms)
         if (M < 1 || M > 8) return bad(ESRRF_ERR_UNSUPPORTED);
     for (int nf : s->nfs) {
@@ -359,8 +455,55 @@ int esrrf_sweep_create(const esrrf_sweep_params *p, esrrf_sweep **out)
     s->d_out.assign(s->Ms.size(), nullptr);
     s->d_mp.assign(s->Ms.size(), nullptr);
     const size_t nc = n_conf(s);
+    if (s->fht) {
+        const KernelVariants var = kernel_variants_from_env();
+        s->fms.resize(s->Ms.size());
+        s->f_px.assign(s->Ms.size(), nullptr);
+        s->f_U.assign(s->Ms.size(), nullptr);
+        s->f_aux.assign(s->Ms.size(), nullptr);
+        s->ftaps.resize(s->Ms.size());
+        for (size_t im = 0; im < s->Ms.size(); im++) {
+            const int M = s->Ms[im];
+            float sig = 0, rad = 0, tss = 0, tso = 0;
+            esrrf_fht_constants(s->fwhm[0], M, &sig, &rad, &tss, &tso);
+            ModeCreate mc{};
+            mc.mode = 1;
+            mc.w = mc.W = s->w;
+            mc.h = mc.H = s->h;
+            mc.nP = 1;
+            mc.M = M;
+            mc.nF = s->nfmax;
+            mc.radius = rad;
+            mc.tSS = tss;
+            mc.tSO = tso;
+            mc.var = var;
+            size_t auxb = 0;
+            int rc = mode_create(mc, s->fms[im], &s->f_px[im], &s->f_U[im], &s->f_aux[im], &auxb);
+            if (rc != ESRRF_OK) {
+                sfail(s, rc, "FHT state: %s", s->fms[im].err.c_str());
+                return bad(rc);
+            }
+            s->ftaps[im].resize(s->fwhm.size());
+            for (size_t fi = 0; fi < s->fwhm.size(); fi++) {
+                esrrf_fht_constants(s->fwhm[fi], M, &sig, &rad, &tss, &tso);
+                rc = fht_taps_create(rad, tss, tso, s->ftaps[im][fi]);
+                if (rc != ESRRF_OK) return bad(rc);
+                const FhtTaps &t = s->ftaps[im][fi];
+                // the emit launch keeps the window's tile and the taps in LDS
+                if (fht_rgc_lds_bytes(t.TW, t.pitch, t.ntaps) > 65536)
+                    return bad(ESRRF_ERR_UNSUPPORTED);
+            }
+        }
+    }
     for (size_t im = 0; im < s->Ms.size(); im++) {
         const int M = s->Ms[im];
+        if (s->fht) {
+            if (!alloc((void **)&s->d_out[im], sizeof(float) * nc * 4 * wh * M * M) ||
+                !alloc((void **)&s->d_mp[im], sizeof(float) * nc * 3 * M * M))
+                return bad(ESRRF_ERR_NOMEM);
+            s->tmp_floats = std::max(s->tmp_floats, (size_t)4 * wh * M * M);
+            continue;
+        }
         s->tabs[im].resize(s->fwhm.size());
         for (size_t fi = 0; fi < s->fwhm.size(); fi++) {
             if (bic_tables_create(s->w, s->h, M, s->fwhm[fi], s->tabs[im][fi]) != hipSuccess)
@@ -472,6 +615,7 @@ int esrrf_sweep_shape(esrrf_sweep *s, int i_mag, int i_fwhm, int *nw, int *ppw, 
         i_fwhm >= (int)s->fwhm.size())
         return ESRRF_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fht) return sfail(s, ESRRF_ERR_STATE, "FHT mode has no tiled 2-D launch shape");
     if (s->multi) {
         const EmitShape &m = s->mshape[i_mag][i_fwhm / SWEEP_NFG];
         if (nw) *nw = m.nw;

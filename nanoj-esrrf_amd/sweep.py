@@ -35,7 +35,9 @@ def sweep_axes(fwhm0, dfwhm, n_fwhm, S0, dS, n_S, nf0, dnf, n_nf, n_slices, frc=
 class ParametersSweep:
     def __init__(self, width, height, magnifications, fwhm, sensitivity, n_frames,
                  int_weighting=True, mp_correction=True, autotune=False, use_graph=True,
-                 device=-1):
+                 device=-1, fht=False):
+        """fht: the reference sweep's doFHTinterpolation (gui/ParametersSweep_.java:117,452):
+        Fourier-upsampled frames, no MP correction."""
         self.L = C.lib()
         self.width, self.height = int(width), int(height)
         self.mags = np.ascontiguousarray(magnifications, dtype=np.int32)
@@ -55,6 +57,8 @@ class ParametersSweep:
         p.autotune = 1 if autotune else 0
         p.use_graph = 1 if use_graph else 0
         p.device = int(device)
+        p.do_fht_interpolation = 1 if fht else 0
+        self.fht = bool(fht)
         h = ctypes.c_void_p()
         C.check(self.L.esrrf_sweep_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h

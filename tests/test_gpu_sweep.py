@@ -242,3 +242,40 @@ def test_multi_fwhm_emit_bit_identical_to_per_fwhm_emit(pkg, monkeypatch, fwhms)
         sw.close()
     for k, v in res["1"].items():
         assert np.array_equal(v.view(np.uint32), res["0"][k].view(np.uint32)), k
+
+
+def plain_fht(pkg, st, M, fwhm, S, nf):
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(st.shape[2], st.shape[1], M, fwhm, S, nf, nf, 20000, e.allCLdevices[0], True,
+                 True, True, None, 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(nf), np.zeros(nf))
+    e.calculateSRRF(st[:nf])
+    out = np.asarray(e.readSRRFbuffer()).copy()
+    e.release()
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_fht_sweep_bit_identical_to_per_configuration_engine(pkg, use_graph):
+    """The sweep in FHT mode (the reference sweep's default, gui/ParametersSweep_.java:117,
+    452): frames upsampled once per magnification, one CGLH emit launch per fwhm, one FHT
+    accumulation per fwhm for every (frame count, sensitivity); each configuration equals
+    initialise(doFHTinterpolation) + calculateSRRF + readSRRFbuffer bit for bit (no MP
+    correction in FHT mode, LiveSRRF_CL.java:762)."""
+    st = stack(10, 24, 20, seed=616)
+    mags, fwhms, sens, nfs = [2, 3], [1.0, 1.5, 2.0], [1.0, 2.5], [5, 10]
+    sw = pkg.ParametersSweep(24, 20, mags, fwhms, sens, nfs, use_graph=use_graph, fht=True)
+    sw.run(st)
+    for im, M in enumerate(mags):
+        for iq, nf in enumerate(nfs):
+            for i_s, S in enumerate(sens):
+                for i_f, fwhm in enumerate(fwhms):
+                    got = np.asarray(sw.read(im, iq, i_s, i_f))
+                    exp = plain_fht(pkg, st, M, fwhm, S, nf)
+                    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), \
+                        (M, nf, S, fwhm)
+    with pytest.raises(pkg._capi.EsrrfError):
+        sw.shape(0, 0)                # FHT mode has no tiled 2-D launch shape
+    sw.close()
