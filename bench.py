@@ -62,6 +62,9 @@ def parse():
                          "the Fourier-upsampling mode; 3d: 512 frames of a 9-plane 128x128 "
                          "multi-focus montage, M=4; rolling: the fused rolling engine "
                          "against the per-window engine loop on the production stack")
+    ap.add_argument("--axial", type=int, default=0,
+                    help="--mode 3d: axial magnification (output planes per focal plane); "
+                         "0 = the lateral magnification, as the reference has it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 plain 1 GiB device copies (HBM counter calibration)")
     return ap.parse_args()
@@ -330,7 +333,8 @@ def bench_rse(pkg, stack):
 
 def bench_3d(a, pkg, torch):
     """Multi-focus workload: 512 frames of 9 focal planes of 128x128 (a 384x384 3x3 montage,
-    uint16), lateral and axial magnification 4, fwhm 1.5, S=2, MP on -> 36 x 512 x 512."""
+    uint16), lateral magnification 4, axial magnification 4 (or --axial), fwhm 1.5, S=2, MP
+    on -> 9*Mz x 512 x 512."""
     import numpy as np
     C = pkg._capi
     from nanoj_esrrf_amd import synth
@@ -344,11 +348,13 @@ def bench_3d(a, pkg, torch):
     dp = ctypes.POINTER(ctypes.c_double)
     p = params(C, Wm, Hm, M3, FWHM, SENS, nf)
     p.n_planes = nP
+    Mz = a.axial if a.axial > 0 else M3
+    p.axial_magnification = Mz
     p.shift_x_3d, p.shift_y_3d, p.theta_3d, p.roi_3d, p.int_coeffs_3d = \
         [c.ctypes.data_as(dp) for c in cols]
     dev = torch.device("cuda:0")
     d_frames = torch.from_numpy(mont.view(np.int16).reshape(-1)).to(dev)
-    nvox = 128 * M3 * 128 * M3 * nP * M3
+    nvox = 128 * M3 * 128 * M3 * nP * Mz
     d_out = torch.empty(4 * nvox, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
     eng = pkg.Engine(p)
@@ -380,12 +386,14 @@ def bench_3d(a, pkg, torch):
     kname = eng.rgc_kernel()
     eng.close()
     flops = 24.0 * nact.value * nf
-    return {"metric": "eSRRF 3-D output Mvoxel/s (x4 lateral and axial, 512 frames)",
+    axial = "lateral and axial" if Mz == M3 else f"lateral, x{Mz} axial"
+    return {"metric": f"eSRRF 3-D output Mvoxel/s (x4 {axial}, 512 frames)",
             "value": round(nvox * nf / (ms_step / 1e3) / 1e6, 1), "unit": "Mvoxel/s",
             "ms_per_step": round(ms_step, 3), "steps": a.steps, "warmup": a.warmup,
             "dtype": "f32", "data": "synthetic (seeded 3-D blinking filaments, uint16)",
             "config": {"workload": "3-D eSRRF: 512 frames x 9 planes x 128x128 uint16 (3x3 "
-                                   "montage 384x384) -> 36 x 512 x 512, M=4, fwhm 1.5, S=2, MP on"},
+                                   f"montage 384x384) -> {nP * Mz} x 512 x 512, M=4, Mz={Mz}, "
+                                   "fwhm 1.5, S=2, MP on"},
             "roofline": {"bound": "valu", "kernel": kname, "unit": "TFLOP/s",
                          "achieved": round(flops / (rgc_avg / 1e3) / 1e12, 2),
                          "peak": PEAK_FP32_TFLOPS,

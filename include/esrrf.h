@@ -57,6 +57,9 @@ typedef struct esrrf_params {
     float background_3d;      /* "Background level (ADC)"[0]                     */
     float axial_offset;       /* nm (:236)                                       */
     float pixel_size;         /* um (:236)                                       */
+    int axial_magnification;  /* 3-D: output planes per input plane; 0 = magnification, the
+                               * reference's single magnification (live3DSRRF.cl:354-356).
+                               * No counterpart: the axial centre is (zM + 0.5) / this value */
 } esrrf_params;
 
 /* LiveSRRF_CL.checkDevices (LiveSRRF_CL.java:111-155): number of HIP devices. */

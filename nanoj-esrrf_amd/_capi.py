@@ -71,7 +71,7 @@ class Params(ctypes.Structure):
         ("roi_3d", ctypes.POINTER(ctypes.c_double)),
         ("int_coeffs_3d", ctypes.POINTER(ctypes.c_double)),
         ("background_3d", ctypes.c_float), ("axial_offset", ctypes.c_float),
-        ("pixel_size", ctypes.c_float),
+        ("pixel_size", ctypes.c_float), ("axial_magnification", ctypes.c_int),
     ]
 
 

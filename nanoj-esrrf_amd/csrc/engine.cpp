@@ -61,6 +61,7 @@ struct esrrf_engine {
     int W = 0, H = 0;            // raw frame (3-D: montage)
     int w = 0, h = 0, nP = 1;    // per plane (widthS, heightS, nPlanes)
     int M = 1, wM = 0, hM = 0, nPlanesM = 1;
+    int Mz = 1;                  // axial magnification (3-D), 1 in 2-D
     long nvox = 0;               // voxels of one output channel
     float sigma = 0, radius = 0, tSS = 0, tSO = 0;
     int sens_int = 0;
@@ -384,9 +385,8 @@ int finalize_impl(esrrf_engine *e, float *dst_dev)
 {
     HIPCHK(e, launch_var(e->d_out, e->nvox, e->stream));
     if (e->p.do_mp_map_correction && e->mode != MODE_FHT) {
-        const int dims = e->mode == MODE_3D ? 3 : 2;
-        HIPCHK(e, launch_mp_map(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, dims, e->stream));
-        HIPCHK(e, launch_mp_correct(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, dims, e->stream));
+        HIPCHK(e, launch_mp_map(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, e->Mz, e->stream));
+        HIPCHK(e, launch_mp_correct(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, e->Mz, e->stream));
         e->has_mp = true;
     }
     HIPCHK(e, launch_nan_to_zero_copy(e->d_out, dst_dev, 4 * e->nvox, e->stream));
@@ -506,7 +506,8 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         return code;
     };
     if (p->width <= 0 || p->height <= 0 || p->magnification < 1 || p->n_frames_on_gpu < 1 ||
-        p->n_frame_for_srrf < 1 || !(p->fwhm > 0.0f) || !(p->sensitivity == p->sensitivity)) {
+        p->n_frame_for_srrf < 1 || !(p->fwhm > 0.0f) || !(p->sensitivity == p->sensitivity) ||
+        p->axial_magnification < 0) {
         return bad(fail(e, ESRRF_ERR_INVALID, "invalid parameters"));
     }
     int dev = p->device;
@@ -533,7 +534,8 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         e->nP = p->n_planes;
         e->w = p->width / 3;    // nSplits = 3 (LiveSRRF_CL.java:39,229-230)
         e->h = p->height / 3;
-        e->nPlanesM = e->nP * e->M;
+        e->Mz = p->axial_magnification > 0 ? p->axial_magnification : e->M;
+        e->nPlanesM = e->nP * e->Mz;
     } else {
         e->nP = 1;
         e->w = p->width;
@@ -544,7 +546,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         return bad(fail(e, ESRRF_ERR_INVALID, "frames must be at least 4x4 pixels (per plane)"));
     e->wM = e->w * e->M;
     e->hM = e->h * e->M;
-    e->nvox = (long)e->wM * e->hM * (do3d ? (long)e->nP * e->M : 1);
+    e->nvox = (long)e->wM * e->hM * e->nPlanesM;
     e->sens_int = sens_exact_int(p->sensitivity);
     e->var = kernel_variants_from_env();
     if (e->mode == MODE_FHT)
@@ -561,7 +563,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
     if ((rc = dalloc(e, &e->d_tmp, sizeof(float) * 4 * e->nvox))) return bad(rc);
     if ((rc = dalloc(e, &e->d_prev, sizeof(float) * e->nvox))) return bad(rc);
     if ((rc = dalloc(e, &e->d_drift, sizeof(float) * 2 * p->n_frame_for_srrf))) return bad(rc);
-    const int Md = do3d ? e->M * e->M * e->M : e->M * e->M;
+    const int Md = e->M * e->M * e->Mz;
     if ((rc = dalloc(e, &e->d_mp, sizeof(float) * 3 * Md))) return bad(rc);
     // defined initial state (the reference leaves fresh OpenCL buffers uninitialised)
     if (hipMemsetAsync(e->d_out, 0, sizeof(float) * 4 * e->nvox, e->stream) != hipSuccess ||
@@ -592,6 +594,7 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
         mc.H = e->H;
         mc.nP = e->nP;
         mc.M = e->M;
+        mc.Mz = e->Mz;
         mc.nF = nF;
         mc.radius = e->radius;
         mc.tSS = e->tSS;
@@ -736,7 +739,7 @@ int esrrf_read_mp_maps(esrrf_engine *e, float *out)
     if (e->mode == MODE_FHT)
         return fail(e, ESRRF_ERR_STATE, "no MP map in FHT mode (LiveSRRF_CL.java:762)");
     (void)hipSetDevice(e->dev);
-    const int Md = e->mode == MODE_3D ? e->M * e->M * e->M : e->M * e->M;
+    const int Md = e->M * e->M * e->Mz;
     HIPCHK(e, hipMemcpyAsync(out, e->d_mp, sizeof(float) * 3 * Md, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     for (int i = 0; i < 3 * Md; i++)

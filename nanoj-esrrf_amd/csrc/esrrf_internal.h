@@ -155,11 +155,12 @@ hipError_t launch_bic_emit_multi(const BicRgcArgs &a, int ppw, const TapM *taps,
                                  long cstride, hipStream_t s);
 size_t bic_emit_multi_lds_bytes(const BicRgcArgs &a, int pg_taps);   // LDS of the frame-pair kernel
 hipError_t launch_generic_rgc(const GenericRgcArgs &a, hipStream_t s);
-// nbatch images `ostride` floats apart (MP maps `mstride` floats apart) in one launch
+// nbatch images `ostride` floats apart (MP maps `mstride` floats apart) in one launch.
+// MP maps: nP input planes magnified M laterally and Mz axially (2-D: nP = Mz = 1).
 hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch = 1, long ostride = 0);
-hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,
+hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int Mz,
                          hipStream_t s, int nbatch = 1, long ostride = 0, long mstride = 0);
-hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int dims,
+hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int Mz,
                              hipStream_t s, int nbatch = 1, long ostride = 0, long mstride = 0);
 hipError_t launch_nan_to_zero_copy(const float *src, float *dst, long n, hipStream_t s);
 

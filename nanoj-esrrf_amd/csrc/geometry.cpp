@@ -351,8 +351,8 @@ namespace esrrf {
 // grid rectangle its 8x8 macro-pixels reach, each row split into its even and odd columns
 // so that the stride-2 sample pattern of a wave reads consecutive float4s.
 // ---------------------------------------------------------------------------------------
-bool build_3d_geometry(int w, int h, int nP, int M, float radius, float tSS, float tSO,
-                       Geometry3 &g)
+bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float tSS,
+                       float tSO, Geometry3 &g)
 {
     g.R2lo = -(int)(2.0f * radius);
     g.R2hi = (int)(2.0f * radius + 1.0f);
@@ -374,14 +374,14 @@ bool build_3d_geometry(int w, int h, int nP, int M, float radius, float tSS, flo
     const bool pow2 = (M & (M - 1)) == 0;
     g.ncls_x = pow2 ? 1 : class_of(w > 0 ? w - 1 : 0, lg) + 1;
     g.ncls_y = pow2 ? 1 : class_of(h > 0 ? h - 1 : 0, lg) + 1;
-    g.nZM = nP * M;
+    g.nZM = nP * Mz;
     const int MM = M * M;
 
     // planes each z-slice can reach (|dz| <= tSO, 0 < vz < nP)
     std::vector<int> zlo(g.nZM), nz(g.nZM);
     g.NZ = 0;
     for (int zM = 0; zM < g.nZM; zM++) {
-        const float zc = (float)((zM + 0.5) / M);
+        const float zc = (float)((zM + 0.5) / Mz);
         int lo = 1 << 30, hi = -1;
         for (int k = klo; k <= khi; k++) {
             const float vz = (float)((int)zc) + (float)k;
@@ -406,7 +406,7 @@ bool build_3d_geometry(int w, int h, int nP, int M, float radius, float tSS, flo
         for (int cx = 0; cx < g.ncls_x; cx++) {
             const int Xr = pow2 ? 8 : rep_of(cx, lg), Yr = pow2 ? 8 : rep_of(cy, lg);
             for (int zM = 0; zM < g.nZM; zM++) {
-                const float zc = (float)((zM + 0.5) / M);
+                const float zc = (float)((zM + 0.5) / Mz);
                 const int zi = (int)zc;
                 Phase3 *PB = &g.phases[(((size_t)cy * g.ncls_x + cx) * g.nZM + zM) * MM];
                 for (int sy = 0; sy < M; sy++) {

@@ -57,7 +57,7 @@ struct Geometry3 {
 
 // Tables for frames without drift; false when the window or tile exceeds the kernel's
 // limits (the per-voxel kernel is used then).
-bool build_3d_geometry(int w, int h, int nP, int M, float radius, float tSS, float tSO,
-                       Geometry3 &g);
+bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float tSS,
+                       float tSO, Geometry3 &g);
 
 }  // namespace esrrf

@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256) k3d_rgc(const Rgc3dArgs a)
     if (xM >= a.wM || yM >= a.hM) return;
     const int w = a.w, h = a.h, nP = a.nP, M = a.M;
     const long whM = (long)a.wM * a.hM;
-    const long whdM = whM * nP * M;
+    const long whdM = whM * nP * a.Mz;
     const long vox = whM * zM + (long)yM * a.wM + xM;
     const long whI = 4L * w * h;
     const int wI = 2 * w;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) k3d_rgc(const Rgc3dArgs a)
         s3 = a.out[vox + 3 * whdM];
         s4 = a.prev[vox];
     }
-    const float zc = (float)((zM + 0.5) / M);
+    const float zc = (float)((zM + 0.5) / a.Mz);
     int z0;
     const float zcof = zc - 0.5f;
     if (zcof < 0.0f) z0 = 0;
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256) k3d_rgc2(const Rgc3dArgs a)
     if (xM >= a.wM || yM >= a.hM) return;
     const int w = a.w, h = a.h, nP = a.nP, M = a.M;
     const long whM = (long)a.wM * a.hM;
-    const long whdM = whM * nP * M;
+    const long whdM = whM * nP * a.Mz;
     const long vox = whM * zM + (long)yM * a.wM + xM;
     const long whI = 4L * w * h;
     const int wI = 2 * w;
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) k3d_rgc2(const Rgc3dArgs a)
         s3 = a.out[vox + 3 * whdM];
         s4 = a.prev[vox];
     }
-    const float zc = (float)((zM + 0.5) / M);
+    const float zc = (float)((zM + 0.5) / a.Mz);
     int z0;
     const float zcof = zc - 0.5f;
     if (zcof < 0.0f) z0 = 0;
@@ -595,7 +595,7 @@ __global__ void __launch_bounds__(256 * SPLIT) k3d_rgc3(const Rgc3dArgs a, const
     const bool inimg = active && X < w && Y < h;
     const int xM = X * M + sx, yM = Y * M + sy;
     const long whM = (long)a.wM * a.hM;
-    const long whdM = whM * nP * M;
+    const long whdM = whM * nP * a.Mz;
     const long vox = whM * zM + (long)yM * a.wM + xM;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
     if (a.count0 > 0 && inimg) {
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(256 * SPLIT) k3d_rgc3(const Rgc3dArgs a, const
         s4 = a.prev[vox];
     }
     // axial cubic planes (live3DSRRF.cl:418-452)
-    const float zc = (float)((zM + 0.5) / M);
+    const float zc = (float)((zM + 0.5) / a.Mz);
     int z0;
     const float zcof = zc - 0.5f;
     if (zcof < 0.0f) z0 = 0;
@@ -727,11 +727,12 @@ size_t rgc3_lds_bytes(const Rgc3Tables &t)
 }
 
 // Gated samples of one frame without drift, summed over all voxels.
-__global__ void __launch_bounds__(256) k3d_count(int w, int h, int nP, int M, float radius,
-                                                 float tSO, unsigned long long *total)
+__global__ void __launch_bounds__(256) k3d_count(int w, int h, int nP, int M, int Mz,
+                                                 float radius, float tSO,
+                                                 unsigned long long *total)
 {
     const int wM = w * M, hM = h * M;
-    const long n = (long)wM * hM * nP * M;
+    const long n = (long)wM * hM * nP * Mz;
     unsigned long long cnt = 0;
     const int jlo = -(int)(2.0f * radius), jhi = (int)(2.0f * radius + 1.0f);
     const int klo = -(int)radius, khi = (int)radius + 1;
@@ -741,7 +742,7 @@ __global__ void __launch_bounds__(256) k3d_count(int w, int h, int nP, int M, fl
         const long r = o - (long)zM * wM * hM;
         const int yM = (int)(r / wM), xM = (int)(r - (long)yM * wM);
         const float xc = (float)((xM + 0.5) / M), yc = (float)((yM + 0.5) / M);
-        const float zc = (float)((zM + 0.5) / M);
+        const float zc = (float)((zM + 0.5) / Mz);
         const float bx = (float)((int)(2.0f * xc)), by = (float)((int)(2.0f * yc));
         for (int j = jlo; j <= jhi; j++) {
             const float vy = (by + (float)j) / 2.0f;
@@ -807,14 +808,14 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t
     const long nl = (long)a.nframes * a.nP * a.wM * a.hM;
     hipLaunchKernelGGL(k3d_lateral, dim3(grid1(nl)), dim3(256), 0, s, a.px, a.L, a.w, a.h, a.nP,
                        a.M, a.frame0, a.nframes, a.drift, a.count0, a.nFor);
-    const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.M);
+    const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.Mz);
     // (the fast kernel addresses one frame of U with 32-bit byte offsets)
     const bool off32 = (double)a.nP * 4.0 * a.w * a.h * sizeof(float4) < 4294967296.0;
     // the tiled kernel addresses one frame of U with 32-bit element offsets
     const bool off31 = (double)a.nP * 4.0 * a.w * a.h < 2147483648.0;
     if (drift_zero && t && kern == 3 && off31 && rgc3_lds_bytes(*t) <= lds_limit) {
         const int tiles = ((a.w + 7) >> 3) * ((a.h + 7) >> 3);
-        const dim3 g3(tiles, (a.M * a.M + 3) / 4, a.nP * a.M);
+        const dim3 g3(tiles, (a.M * a.M + 3) / 4, a.nP * a.Mz);
         if (split == 1)
             hipLaunchKernelGGL(k3d_rgc3<1>, g3, dim3(256), rgc3_lds_bytes(*t), s, a, *t);
         else
@@ -833,12 +834,12 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t
     return hipGetLastError();
 }
 
-hipError_t launch3d_count(int w, int h, int nP, int M, float radius, float tSO,
+hipError_t launch3d_count(int w, int h, int nP, int M, int Mz, float radius, float tSO,
                           unsigned long long *d_total, hipStream_t s)
 {
-    const long n = (long)w * M * h * M * nP * M;
-    hipLaunchKernelGGL(k3d_count, dim3(grid1(n)), dim3(256), 0, s, w, h, nP, M, radius, tSO,
-                       d_total);
+    const long n = (long)w * M * h * M * nP * Mz;
+    hipLaunchKernelGGL(k3d_count, dim3(grid1(n)), dim3(256), 0, s, w, h, nP, M, Mz, radius,
+                       tSO, d_total);
     return hipGetLastError();
 }
 

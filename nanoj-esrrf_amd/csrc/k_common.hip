@@ -66,36 +66,36 @@ hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch, long ost
     return hipGetLastError();
 }
 
-// One block per MP-map entry (channel, [z,] y, x sub-position): mean of that sub-position
-// over every macro-pixel (macro-voxel).  The reference sums sequentially in float; this
-// tree-reduces in double, which is at least as accurate.  blockIdx.y: image of a batch
-// (images `ostride` floats apart, maps `mstride` floats apart).
+// One block per MP-map entry (channel, z, y, x sub-position): mean of that sub-position over
+// every macro-pixel (M x M) or macro-voxel (M x M x Mz).  An image is nP planes of
+// (w*M) x (h*M) pixels magnified to nP*Mz planes; a 2-D image is nP = Mz = 1.  The reference
+// sums sequentially in float; this tree-reduces in double, which is at least as accurate.
+// blockIdx.y: image of a batch (images `ostride` floats apart, maps `mstride` floats apart).
 __global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
                                                 float *__restrict__ mp, int w, int h, int nP,
-                                                int M, int dims, long ostride, long mstride)
+                                                int M, int Mz, long ostride, long mstride)
 {
     out += ostride * blockIdx.y;
     mp += mstride * blockIdx.y;
     const int o = blockIdx.x;
     const int MM = M * M;
-    const int Md = dims == 3 ? MM * M : MM;
+    const int Md = MM * Mz;
     const int frame = o / Md;
     const int r = o - frame * Md;
-    const int zMP = dims == 3 ? r / MM : 0;
+    const int zMP = r / MM;
     const int yMP = (r - zMP * MM) / M;
     const int xMP = r - zMP * MM - yMP * M;
     const long wM = (long)w * M;
     const long whM = wM * h * M;
-    const long nz = dims == 3 ? nP : 1;
-    const long whdM = whM * (dims == 3 ? nP * M : 1);
-    const long count = (long)w * h * nz;
+    const long whdM = whM * nP * Mz;
+    const long count = (long)w * h * nP;
     double acc = 0.0;
     for (long i = threadIdx.x; i < count; i += blockDim.x) {
         long z = i / ((long)w * h);
         long rem = i - z * (long)w * h;
         long y = rem / w;
         long x = rem - y * w;
-        long off = xMP + x * M + wM * (yMP + y * M) + whM * (zMP + z * M) + frame * whdM;
+        long off = xMP + x * M + wM * (yMP + y * M) + whM * (zMP + z * Mz) + frame * whdM;
         acc += (double)out[off];
     }
     __shared__ double red[256];
@@ -109,14 +109,13 @@ __global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
 }
 
 __global__ void k_mp_correct(float *__restrict__ out, const float *__restrict__ mp, int w,
-                             int h, int nP, int M, int dims, long ostride, long mstride)
+                             int h, int nP, int M, int Mz, long ostride, long mstride)
 {
     out += ostride * blockIdx.y;
     mp += mstride * blockIdx.y;
     const long wM = (long)w * M, hM = (long)h * M;
     const long whM = wM * hM;
-    const long zMn = dims == 3 ? (long)nP * M : 1;
-    const long whdM = whM * zMn;
+    const long whdM = whM * nP * Mz;
     const long n = 3 * whdM;
     long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
@@ -128,28 +127,27 @@ __global__ void k_mp_correct(float *__restrict__ out, const float *__restrict__ 
         rem -= zM * whM;
         long yM = rem / wM;
         long xM = rem - yM * wM;
-        int sx = (int)(xM % M), sy = (int)(yM % M), sz = (int)(zM % M);
-        long om = dims == 3 ? (long)MM * M * frame + MM * sz + M * sy + sx
-                            : (long)MM * frame + M * sy + sx;
+        int sx = (int)(xM % M), sy = (int)(yM % M), sz = (int)(zM % Mz);
+        long om = (long)MM * Mz * frame + MM * sz + M * sy + sx;
         out[o] = out[o] / mp[om];
     }
 }
 
-hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int dims,
+hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int Mz,
                          hipStream_t s, int nbatch, long ostride, long mstride)
 {
-    int nentries = 3 * M * M * (dims == 3 ? M : 1);
+    int nentries = 3 * M * M * Mz;
     hipLaunchKernelGGL(k_mp_map, dim3(nentries, nbatch), dim3(256), 0, s, out, mp, w, h, nP, M,
-                       dims, ostride, mstride);
+                       Mz, ostride, mstride);
     return hipGetLastError();
 }
 
-hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int dims,
+hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int Mz,
                              hipStream_t s, int nbatch, long ostride, long mstride)
 {
-    long n = 3L * w * M * h * M * (dims == 3 ? (long)nP * M : 1);
+    long n = 3L * w * M * h * M * nP * Mz;
     hipLaunchKernelGGL(k_mp_correct, dim3(grid_for(n, 256), nbatch), dim3(256), 0, s, out, mp,
-                       w, h, nP, M, dims, ostride, mstride);
+                       w, h, nP, M, Mz, ostride, mstride);
     return hipGetLastError();
 }
 

@@ -62,7 +62,7 @@ int build_tiled3(const ModeCreate &c, ModeState &ms)
         return err3(ms, ESRRF_ERR_HIP, "3-D: device LDS limit query failed");
     ms.lds_limit = (size_t)lds;
     Geometry3 g;
-    if (!build_3d_geometry(c.w, c.h, c.nP, c.M, c.radius, c.tSS, c.tSO, g)) return ESRRF_OK;
+    if (!build_3d_geometry(c.w, c.h, c.nP, c.M, c.Mz, c.radius, c.tSS, c.tSO, g)) return ESRRF_OK;
     const size_t ntap = g.tapv.size();
     if (ntap > ((size_t)1 << 28) || g.NZ * g.TH * g.TW > 256 * 16) return ESRRF_OK;
     if (!alloc(&ms.d_tapv3, sizeof(float4) * ntap) || !alloc(&ms.d_tapo3, sizeof(int) * ntap) ||
@@ -157,11 +157,12 @@ int mode3d_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U
         return err3(ms, ESRRF_ERR_NOMEM, "3-D sample count: hipMalloc failed");
     unsigned long long cnt = 0;
     e = hipMemcpy(d_cnt, &cnt, sizeof(cnt), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = launch3d_count(w, h, nP, M, c.radius, c.tSO, d_cnt, 0);
+    if (e == hipSuccess) e = launch3d_count(w, h, nP, M, c.Mz, c.radius, c.tSO, d_cnt, 0);
     if (e == hipSuccess) e = hipMemcpy(&cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost);
     (void)hipFree(d_cnt);
     if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D sample count", e);
     ms.active = (long long)cnt;
+    ms.Mz = c.Mz;
     ms.wM = w * M;
     ms.hM = h * M;
     return build_tiled3(c, ms);
@@ -189,6 +190,7 @@ int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
     a.h = c.h;
     a.nP = c.nP;
     a.M = c.M;
+    a.Mz = ms.Mz;
     a.wM = c.wM;
     a.hM = c.hM;
     a.frame0 = r.frame0;

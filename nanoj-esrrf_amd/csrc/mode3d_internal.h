@@ -25,6 +25,7 @@ struct Rgc3dArgs {
     const float *drift;    // [2][nFor]
     float *out, *prev;
     int w, h, nP, M, wM, hM;
+    int Mz;                // axial magnification: nP * Mz output planes, zc = (zM + 0.5) / Mz
     int frame0, nframes, count0, nFor;
     float sens;
     int sens_int, int_weighting;
@@ -43,7 +44,7 @@ hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t
                         size_t lds_limit, hipStream_t s, const char **kname, int kern = 3,
                         int split = 2);
 size_t rgc3_lds_bytes(const Rgc3Tables &t);
-hipError_t launch3d_count(int w, int h, int nP, int M, float radius, float tSO,
+hipError_t launch3d_count(int w, int h, int nP, int M, int Mz, float radius, float tSO,
                           unsigned long long *d_total, hipStream_t s);
 
 int mode3d_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U);

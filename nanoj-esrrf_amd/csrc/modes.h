@@ -40,6 +40,7 @@ struct ModeState {
     float4 *d_u3 = nullptr;       // unit tap gradients [nF*nP][2h][2w]
     float *d_lat = nullptr;       // lateral intensities [nF][nP][hM][wM]
     int n_loaded = 0;             // frames of the current load (gradient plane stride)
+    int Mz = 1;                   // axial magnification
     // tiled 3-D RGC tables (frames without drift)
     float4 *d_tapv3 = nullptr;
     int *d_tapo3 = nullptr;
@@ -64,6 +65,7 @@ struct ModeCtx {
 struct ModeCreate {
     int mode;
     int w, h, W, H, nP, M, nF;
+    int Mz;                       // 3-D: axial magnification (planes per input plane)
     float radius, tSS, tSO;
     const esrrf_params *p;
     KernelVariants var;

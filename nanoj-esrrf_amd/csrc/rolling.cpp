@@ -240,8 +240,8 @@ int step_chunk(esrrf_rolling *r)
         float *out = slot_of(r, win);
         LCHK(r, launch_var(out, whM, st));
         if (r->mp) {
-            LCHK(r, launch_mp_map(out, r->d_mp, r->w, r->h, 1, r->M, 2, st));
-            LCHK(r, launch_mp_correct(out, r->d_mp, r->w, r->h, 1, r->M, 2, st));
+            LCHK(r, launch_mp_map(out, r->d_mp, r->w, r->h, 1, r->M, 1, st));
+            LCHK(r, launch_mp_correct(out, r->d_mp, r->w, r->h, 1, r->M, 1, st));
         }
         r->n_complete = std::max(r->n_complete, win + 1);
     }

@@ -276,9 +276,9 @@ int enqueue_run(esrrf_sweep *s, int dtype)
             SCHK(s, launch_var(out, whM, st, nb, (long)nF * 4 * whM));
             if (s->do_mp) {
                 float *mp = s->d_mp[im] + (long)fi * 3 * M * M;
-                SCHK(s, launch_mp_map(out, mp, s->w, s->h, 1, M, 2, st, nb, (long)nF * 4 * whM,
+                SCHK(s, launch_mp_map(out, mp, s->w, s->h, 1, M, 1, st, nb, (long)nF * 4 * whM,
                                       (long)nF * 3 * M * M));
-                SCHK(s, launch_mp_correct(out, mp, s->w, s->h, 1, M, 2, st, nb,
+                SCHK(s, launch_mp_correct(out, mp, s->w, s->h, 1, M, 1, st, nb,
                                           (long)nF * 4 * whM, (long)nF * 3 * M * M));
             }
         }

@@ -108,7 +108,10 @@ class LiveSRRF_CL:
     def initialise(self, width, height, magnification, fwhm, sensitivity, nFramesOnGPU,
                    nFrameForSRRF, blockLength, chosenDevice, intWeighting, doMPmapCorrection,
                    doFHTinterpolation, calibTablePath3DSRRF=None, axialOffset=0.0,
-                   pixelSize=1.0):
+                   pixelSize=1.0, axialMagnification=0):
+        """The reference's initialise; ``axialMagnification`` (3-D only, no counterpart in
+        the reference) sets the output planes per input plane, 0 meaning ``magnification``
+        as live3DSRRF.cl has it."""
         self.release()
         p = C.Params()
         p.width, p.height, p.magnification = int(width), int(height), int(magnification)
@@ -139,12 +142,15 @@ class LiveSRRF_CL:
             p.background_3d = self._bg
             p.axial_offset = float(axialOffset)
             p.pixel_size = float(pixelSize)
+            p.axial_magnification = int(axialMagnification)
         self._eng = Engine(p)
         e = self._eng
         self.widthS, self.heightS, self.nPlanes, self.nPlanesM = (e.widthS, e.heightS,
                                                                   e.nPlanes, e.nPlanesM)
         self.widthM, self.heightM = e.wM, e.hM
         self.magnification = int(magnification)
+        self.axialMagnification = (self.nPlanesM // self.nPlanes
+                                   if calibTablePath3DSRRF is not None else 1)
         self.nFrameForSRRF = int(nFrameForSRRF)
         self.width, self.height = int(width), int(height)
 
@@ -211,7 +217,7 @@ class LiveSRRF_CL:
     def readMPmaps(self):
         e = self._e()
         M = self.magnification
-        n = 3 * (M if self._calib is not None else 1)
+        n = 3 * self.axialMagnification
         out = np.empty((n, M, M), np.float32)
         e.check(e.L.esrrf_read_mp_maps(e.h, C.f32ptr(out)))
         return out

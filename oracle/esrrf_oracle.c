@@ -642,14 +642,17 @@ static inline float cross_mag(float a1, float a2, float a3, float b1, float b2, 
     return sqrtf(s);
 }
 
-/* calculateRadialGradientConvergence 3D (live3DSRRF.cl:325-498) */
+/* calculateRadialGradientConvergence 3D (live3DSRRF.cl:325-498).  The reference has one
+ * magnification for x, y and z; with axialMagnification Mz the voxel centre is
+ * zc = (zM + 0.5) / Mz over nP * Mz planes (Mz = M is the reference). */
 void orc3d_rgc_frame(const orc_3d_cfg *c, const float *px, const float *gxi,
                      const float *gyi, const float *gzi, const float *shiftXY, int count0,
                      int count1, float *out, float *prev, int nthreads)
 {
     const int w = c->width, h = c->height, M = c->magnification, nP = c->nPlanes;
+    const int Mz = c->axialMagnification > 0 ? c->axialMagnification : M;
     const int wM = w * M, hM = h * M;
-    const long whM = (long)wM * hM, whdM = whM * nP * M;
+    const long whM = (long)wM * hM, whdM = whM * nP * Mz;
     const int wI = 2 * w, hI = 2 * h;
     const long whI = (long)wI * hI;
     const long wh = (long)w * h, whd = wh * nP;
@@ -674,7 +677,7 @@ void orc3d_rgc_frame(const orc_3d_cfg *c, const float *px, const float *gxi,
         const int xM = (int)(offset - zM * whM - (long)yM * wM);
         const float xc = (float)((xM + 0.5) / M + (double)driftX);
         const float yc = (float)((yM + 0.5) / M + (double)driftY);
-        const float zc = (float)((zM + 0.5) / M);
+        const float zc = (float)((zM + 0.5) / Mz);
         float CGLH = 0.0f, sumW = 0.0f;
         for (int j = jlo; j <= jhi; j++) {
             float vy = ((float)((int)(2.0f * yc)) + (float)j) / 2.0f;
@@ -756,12 +759,12 @@ void orc3d_rgc_frame(const orc_3d_cfg *c, const float *px, const float *gxi,
     }
 }
 
-/* kernelCalculateMPmap 3D (live3DSRRF.cl:561-588) */
-void orc3d_mp_map(const float *out, int w, int h, int nP, int M, float *mp)
+/* kernelCalculateMPmap 3D (live3DSRRF.cl:561-588); Mz sub-positions along z */
+void orc3d_mp_map(const float *out, int w, int h, int nP, int M, int Mz, float *mp)
 {
-    const int MM = M * M, MMM = MM * M;
+    const int MM = M * M, MMM = MM * Mz;
     const int wM = w * M;
-    const long whM = (long)wM * h * M, whdM = whM * nP * M;
+    const long whM = (long)wM * h * M, whdM = whM * nP * Mz;
     const long wh = (long)w * h, whd = wh * nP;
     for (int o = 0; o < 3 * MMM; o++) {
         int frame = o / MMM;
@@ -774,7 +777,7 @@ void orc3d_mp_map(const float *out, int w, int h, int nP, int M, float *mp)
             int y = (int)((i - z * wh) / w);
             int x = (int)(i - (long)y * w - z * wh);
             long off = xMP + (long)x * M + (long)wM * (yMP + (long)y * M) +
-                       whM * (zMP + (long)z * M) + frame * whdM;
+                       whM * (zMP + (long)z * Mz) + frame * whdM;
             s += out[off];
         }
         mp[o] = s / (float)whd;
@@ -782,18 +785,18 @@ void orc3d_mp_map(const float *out, int w, int h, int nP, int M, float *mp)
 }
 
 /* kernelCorrectMPmap 3D (live3DSRRF.cl:591-611) */
-void orc3d_mp_correct(float *out, int w, int h, int nP, int M, const float *mp)
+void orc3d_mp_correct(float *out, int w, int h, int nP, int M, int Mz, const float *mp)
 {
     const int wM = w * M;
-    const long whM = (long)wM * h * M, whdM = whM * nP * M;
+    const long whM = (long)wM * h * M, whdM = whM * nP * Mz;
     const int MM = M * M;
     for (long o = 0; o < 3 * whdM; o++) {
         int frame = (int)(o / whdM);
         int zM = (int)((o - frame * whdM) / whM);
         int yM = (int)((o - frame * whdM - zM * whM) / wM);
         int xM = (int)(o - frame * whdM - zM * whM - (long)yM * wM);
-        int x = xM / M, y = yM / M, z = zM / M;
-        int om = MM * M * frame + MM * (zM - z * M) + M * (yM - y * M) + xM - x * M;
+        int x = xM / M, y = yM / M, z = zM / Mz;
+        int om = MM * Mz * frame + MM * (zM - z * Mz) + M * (yM - y * M) + xM - x * M;
         out[o] = out[o] / mp[om];
     }
 }

@@ -31,6 +31,7 @@ typedef struct {
     float radius, tSS, tSO;
     int nFrameForSRRF;
     int intWeighting;
+    int axialMagnification;   /* output planes per input plane; 0 = magnification */
 } orc_3d_cfg;
 
 void orc_java_float_to_string(float x, char *out, int len);
@@ -64,8 +65,8 @@ void orc3d_interp(const float *g, int w, int h, int nP, int nF, float *gi);
 void orc3d_rgc_frame(const orc_3d_cfg *c, const float *px, const float *gxi,
                      const float *gyi, const float *gzi, const float *shiftXY, int count0,
                      int count1, float *out, float *prev, int nthreads);
-void orc3d_mp_map(const float *out, int w, int h, int nP, int M, float *mp);
-void orc3d_mp_correct(float *out, int w, int h, int nP, int M, const float *mp);
+void orc3d_mp_map(const float *out, int w, int h, int nP, int M, int Mz, float *mp);
+void orc3d_mp_correct(float *out, int w, int h, int nP, int M, int Mz, const float *mp);
 void orc3d_reshape(const float *frame, int W, int H, int nP, const double *roi,
                    const double *intCoeff, float bg, float *planes);
 

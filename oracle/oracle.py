@@ -50,7 +50,8 @@ class _Cfg3D(ctypes.Structure):
                 ("nPlanes", ctypes.c_int), ("magnification", ctypes.c_int),
                 ("sensitivity", ctypes.c_float),
                 ("radius", ctypes.c_float), ("tSS", ctypes.c_float), ("tSO", ctypes.c_float),
-                ("nFrameForSRRF", ctypes.c_int), ("intWeighting", ctypes.c_int)]
+                ("nFrameForSRRF", ctypes.c_int), ("intWeighting", ctypes.c_int),
+                ("axialMagnification", ctypes.c_int)]
 
 
 def lib():
@@ -90,9 +91,10 @@ def lib():
     L.orc3d_interp.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
     L.orc3d_rgc_frame.argtypes = [ctypes.POINTER(_Cfg3D), _f32p, _f32p, _f32p, _f32p, _f32p,
                                   ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int]
-    L.orc3d_mp_map.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+    L.orc3d_mp_map.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _f32p]
     L.orc3d_mp_correct.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   _f32p]
+                                   ctypes.c_int, _f32p]
     L.orc3d_reshape.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p,
                                 ctypes.c_float, _f32p]
     _LIB = L
@@ -191,8 +193,12 @@ class OracleLiveSRRF:
 
     def __init__(self, width, height, magnification, fwhm, sensitivity, nFramesOnGPU,
                  nFrameForSRRF, intWeighting=True, doMPmapCorrection=True,
-                 doFHTinterpolation=False, calib3d: Calib3D | None = None, nthreads=0):
+                 doFHTinterpolation=False, calib3d: Calib3D | None = None, nthreads=0,
+                 axialMagnification=None):
+        """axialMagnification: output planes per input plane in 3-D mode (the reference
+        has one magnification; None keeps it, M)."""
         self.M = int(magnification)
+        self.Mz = int(axialMagnification) if axialMagnification else self.M
         self.nFrameForSRRF = int(nFrameForSRRF)
         self.nFramesOnGPU = int(nFramesOnGPU)
         self.doMP = bool(doMPmapCorrection)
@@ -206,7 +212,7 @@ class OracleLiveSRRF:
             self.calib = calib3d
             self.nPlanes = len(calib3d.shift_x)
             self.widthS, self.heightS = width // 3, height // 3
-            self.nPlanesM = self.nPlanes * self.M
+            self.nPlanesM = self.nPlanes * self.Mz
         else:
             self.nPlanes, self.widthS, self.heightS, self.nPlanesM = 1, width, height, 1
         self.wM, self.hM = self.widthS * self.M, self.heightS * self.M
@@ -214,7 +220,7 @@ class OracleLiveSRRF:
             self.k = fht_constants(fwhm, self.M)
         else:
             self.k = bic_constants(fwhm, self.gradientMag)
-        nvox = self.wM * self.hM * (self.nPlanes * self.M if self.do3D else 1)
+        nvox = self.wM * self.hM * (self.nPlanes * self.Mz if self.do3D else 1)
         self.nvox = nvox
         self.out = np.zeros(4 * nvox, np.float32)
         self.prev = np.zeros(nvox, np.float32)
@@ -254,7 +260,8 @@ class OracleLiveSRRF:
                            self.nFrameForSRRF, self.intWeighting)
         if self.do3D:
             return _Cfg3D(self.widthS, self.heightS, self.nPlanes, self.M, self.sensitivity,
-                          k["radius"], k["tSS"], k["tSO"], self.nFrameForSRRF, self.intWeighting)
+                          k["radius"], k["tSS"], k["tSO"], self.nFrameForSRRF, self.intWeighting,
+                          self.Mz)
         return _BicCfg(self.widthS, self.heightS, self.M, self.sensitivity, k["radius"],
                        k["tSS"], k["tSO"], self.nFrameForSRRF, self.intWeighting)
 
@@ -335,10 +342,11 @@ class OracleLiveSRRF:
         L.orc_var(self.out, self.nvox)
         if self.doMP and not self.doFHT:
             if self.do3D:
-                self.mp = np.empty(3 * self.M ** 3, np.float32)
-                L.orc3d_mp_map(self.out, self.widthS, self.heightS, self.nPlanes, self.M, self.mp)
+                self.mp = np.empty(3 * self.M ** 2 * self.Mz, np.float32)
+                L.orc3d_mp_map(self.out, self.widthS, self.heightS, self.nPlanes, self.M,
+                               self.Mz, self.mp)
                 L.orc3d_mp_correct(self.out, self.widthS, self.heightS, self.nPlanes, self.M,
-                                   self.mp)
+                                   self.Mz, self.mp)
             else:
                 self.mp = np.empty(3 * self.M ** 2, np.float32)
                 L.orc_mp_map(self.out, self.widthS, self.heightS, self.M, self.mp)

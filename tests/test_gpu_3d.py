@@ -39,20 +39,20 @@ def calib_dict(sx, sy, th, roi, ic, bg):
 
 
 def run_both(pkg, O, mont, nP, M, fwhm, S, calib, intW=True, mp=True, drift=None, loads=None,
-             nOnGPU=None, extra=None):
+             nOnGPU=None, extra=None, Mz=0):
     n, H, W = mont.shape
     nOnGPU = nOnGPU or n
     cd = calib_dict(*calib)
     e = pkg.LiveSRRF_CL()
     e.checkDevices()
     e.initialise(W, H, M, fwhm, S, nOnGPU, n, 20000, e.allCLdevices[0], intW, mp, False, cd,
-                 0, 1)
+                 0, 1, axialMagnification=Mz)
     e.resetFramePosition()
     dx, dy = (np.zeros(n), np.zeros(n)) if drift is None else drift
     e.loadDriftXYGPUbuffer(dx, dy)
     o = O.OracleLiveSRRF(W, H, M, fwhm, S, nOnGPU, n, intW, mp, False,
                          calib3d=O.Calib3D(*[np.asarray(c, np.float64) for c in calib[:5]],
-                                           float(calib[5])))
+                                           float(calib[5])), axialMagnification=Mz or None)
     if drift is not None:
         o.loadDriftXYGPUbuffer(*drift)
     got_extra = {}
@@ -196,3 +196,48 @@ def test_3d_single_plane_is_rejected(pkg):
     with pytest.raises(_capi.EsrrfError):
         e.initialise(48, 48, 2, 1.5, 1, 2, 2, 20000, e.allCLdevices[0], True, True, False,
                      calib_dict(*zero_calib(1)), 0, 1)
+
+
+@pytest.mark.parametrize("w,h,nP,M,Mz,loads", [
+    (16, 16, 4, 2, 1, [(0, 4)]), (12, 14, 3, 3, 2, [(0, 2), (2, 5)]),
+    (20, 16, 9, 4, 2, [(0, 4)]), (16, 12, 5, 2, 6, [(0, 3)]),
+])
+def test_3d_axial_magnification_parity(pkg, O, w, h, nP, M, Mz, loads):
+    """Axial magnification Mz != M (no counterpart in the reference, whose output has nP * M
+    planes): nP * Mz planes with centres (zM + 0.5) / Mz, M x M x Mz macro-voxels in the MP
+    map.  The engine against the oracle's per-voxel restatement, MP maps included; one test
+    each for a reduced (Mz < M), enlarged (Mz > M) and unit axial factor, with ragged
+    loads."""
+    n = loads[-1][1]
+    planes = planes_stack(n, w, h, nP, seed=90 + w + nP + Mz)
+    mont = montage(planes, np.arange(nP))
+    got, exp, cnt, ocnt, ex = run_both(pkg, O, mont, nP, M, 1.5, 2, zero_calib(nP),
+                                       loads=loads, nOnGPU=max(b - a for a, b in loads), Mz=Mz)
+    assert np.asarray(got).shape == (4 * nP * Mz, h * M, w * M)
+    assert cnt == ocnt == [n, loads[-1][1] - loads[-1][0]]
+    assert_parity(got, exp, f"3D {w}x{h}x{nP} M{M} Mz{Mz}")
+    gmp, omp = ex["mp"]
+    assert gmp.shape == omp.shape == (3 * Mz, M, M)
+    np.testing.assert_allclose(gmp, omp, rtol=1e-4, atol=1e-6)
+
+
+def test_3d_axial_magnification_plane_invariant(pkg):
+    """Plane 3z+1 at axial factor 3*Mz has the centre (3z + 1.5) / (3 Mz) = (z + 0.5) / Mz of
+    plane z at Mz, so without MP correction the engine's AVG, VAR, TAC2 and INT planes agree
+    exactly: every sample, weight and sum is the same float computation."""
+    w, h, nP, M, n = 16, 12, 4, 2, 5
+    planes = planes_stack(n, w, h, nP, seed=123)
+    mont = montage(planes, np.arange(nP))
+    cd = calib_dict(*zero_calib(nP))
+    out = {}
+    for Mz in (1, 3):
+        e = pkg.LiveSRRF_CL()
+        e.checkDevices()
+        e.initialise(3 * w, 3 * h, M, 1.5, 2, n, n, 20000, e.allCLdevices[0], True, False,
+                     False, cd, 0, 1, axialMagnification=Mz)
+        e.resetFramePosition()
+        e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
+        e.calculateSRRF(mont)
+        out[Mz] = np.asarray(e.readSRRFbuffer()).reshape(4, nP * Mz, h * M, w * M)
+        e.release()
+    np.testing.assert_array_equal(out[3][:, 1::3], out[1])

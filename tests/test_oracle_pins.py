@@ -428,3 +428,52 @@ def test_sweep_axes_match_oracle_plan(O, pkg):
         assert s == plan["sensitivity"] and n == plan["nframes"]
     assert pkg.sweep_axes(1.0, 0.5, 5, 1, 1, 4, 100, 150, 3, 500)[2] == [100, 250, 400]
     assert pkg.sweep_axes(1.0, 0.5, 5, 1, 1, 4, 300, 100, 3, 200)[2] == [300][:0]
+
+
+# ---------------------------------------------------------------------------------------
+# 3-D axial magnification (an extension: live3DSRRF.cl has one magnification)
+# ---------------------------------------------------------------------------------------
+def _oracle_3d(O, planes, M, Mz, mp):
+    """planes [n][nP][h][w] as a 3x3 montage through the oracle's 3-D state machine."""
+    n, nP, h, w = planes.shape
+    mont = np.zeros((n, 3 * h, 3 * w), np.float32)
+    for z in range(nP):
+        i, j = z % 3, z // 3
+        mont[:, j * h:(j + 1) * h, i * w:(i + 1) * w] = planes[:, z]
+    zc = np.zeros(nP)
+    calib = O.Calib3D(zc, zc, zc, np.arange(nP, dtype=np.float64), np.ones(nP), 0.0)
+    o = O.OracleLiveSRRF(3 * w, 3 * h, M, 1.5, 2.0, n, n, True, mp, False, calib3d=calib,
+                         axialMagnification=Mz)
+    o.calculateSRRF(mont)
+    return o.readSRRFbuffer().reshape(4, nP * (Mz or M), h * M, w * M)
+
+
+@pytest.mark.parametrize("M,Mz", [(2, 1), (3, 2), (2, 4)])
+def test_oracle_axial_magnification_plane_centres(O, M, Mz):
+    """Output plane 3z+1 at axial magnification 3 Mz has the centre
+    (3z + 1.5) / (3 Mz) = (z + 0.5) / Mz of plane z at Mz, bit for bit the same float, and
+    every other quantity of a voxel depends on (xM, yM) alone: without MP correction the two
+    planes agree exactly.  A z index or centre taken from the lateral magnification fails
+    this; Mz = M is the reference itself (the golden 3-D vectors)."""
+    rng = np.random.default_rng(M * 10 + Mz)
+    planes = rng.integers(100, 900, size=(3, 4, 9, 10)).astype(np.float32)
+    a = _oracle_3d(O, planes, M, Mz, False)
+    b = _oracle_3d(O, planes, M, 3 * Mz, False)
+    nP = planes.shape[1]
+    for z in range(nP * Mz):
+        assert np.array_equal(b[:, 3 * z + 1], a[:, z]), z
+    assert np.array_equal(_oracle_3d(O, planes, M, None, False), _oracle_3d(O, planes, M, M, False))
+
+
+def test_oracle_axial_mp_map_means_are_one(O):
+    """With axial magnification the MP correction divides each (z, y, x) sub-position of the
+    M x M x Mz macro-voxel by its mean: afterwards every sub-position averages 1."""
+    rng = np.random.default_rng(4)
+    planes = rng.integers(100, 900, size=(3, 4, 9, 10)).astype(np.float32)
+    M, Mz = 2, 3
+    out = _oracle_3d(O, planes, M, Mz, True)
+    nP, h, w = 4, 9, 10
+    for c in range(3):
+        v = out[c].astype(np.float64).reshape(nP, Mz, h, M, w, M)
+        means = v.mean(axis=(0, 2, 4))
+        assert np.allclose(means, 1.0, atol=1e-4), (c, means)
