@@ -471,6 +471,9 @@ def main():
     # pick the RGC launch shape on this load (esrrf_autotune), as a deployment would once
     tuned = None
     if not fht and not a.no_autotune:
+        if a.warmup == 0:          # esrrf_autotune times the RGC on the loaded stack
+            step()
+            torch.cuda.synchronize()
         nw_, ppw_, tl_, ms_ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         eng.check(L.esrrf_autotune(h, 2, ctypes.byref(nw_), ctypes.byref(ppw_),
                                    ctypes.byref(tl_), ctypes.byref(ms_)))
