@@ -363,8 +363,12 @@ int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtyp
             r.drift_zero = drift_zero(e, e->counters[0], k);
             r.out = e->d_out;
             r.prev = e->d_prev;
-            ProfScope ps(e, 0);
-            rc = mode_rgc(e->mode == MODE_FHT ? 1 : 2, c, e->ms, r);
+            const int md = e->mode == MODE_FHT ? 1 : 2;
+            rc = mode_pre_rgc(md, c, e->ms, r);
+            if (rc == ESRRF_OK) {
+                ProfScope ps(e, 0);
+                rc = mode_rgc(md, c, e->ms, r);
+            }
             if (rc != ESRRF_OK) rc = fail(e, rc, "%s", e->ms.err.c_str());
         }
         if (rc != ESRRF_OK) return rc;

@@ -135,6 +135,7 @@ struct Rgc3Tables {
     int PW, P, PZ, NZ;      // half-row width, row pitch, plane pitch (float4), planes
     int xs_log2, ncls_x, ncls_y, nZM;
     int gtaps;              // LDS tap slots per block (largest group of 4 phases + spares)
+    int NZ2;                // planes of a two-slice block's tile (union of the slices' planes)
 };
 
 // Host helpers implemented in the .hip files (all launches on `s`).

@@ -394,6 +394,18 @@ bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float 
         nz[zM] = hi >= lo ? hi - lo + 1 : 0;
         g.NZ = std::max(g.NZ, nz[zM]);
     }
+    // planes a block of two consecutive z-slices stages: the union of their ranges
+    g.NZ2 = 0;
+    for (int z2 = 0; z2 < g.nZM; z2 += 2) {
+        int lo = 1 << 30, hi = -1;
+        for (int zM = z2; zM < std::min(g.nZM, z2 + 2); zM++) {
+            if (nz[zM] == 0) continue;
+            lo = std::min(lo, zlo[zM]);
+            hi = std::max(hi, zlo[zM] + nz[zM] - 1);
+        }
+        if (hi >= lo) g.NZ2 = std::max(g.NZ2, hi - lo + 1);
+    }
+    int max_inplane = 0;
 
     g.tapv.clear();
     g.tapo.clear();
@@ -449,6 +461,7 @@ bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float 
                                     const int gx = kxr + g.G0, gy = kyr + g.G0;
                                     const int off = kz * g.PZ + gy * g.P + (gx & 1) * g.PW + (gx >> 1);
                                     max_off = std::max(max_off, off);
+                                    max_inplane = std::max(max_inplane, off - kz * g.PZ);
                                     g.tapv.push_back(make_float4((float)dx / d, (float)dy / d,
                                                                  (float)dz / d, (float)W));
                                     g.tapo.push_back(off);
@@ -484,6 +497,8 @@ bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float 
     }
     // every lane's reads (base 2 ly P + lx, lx, ly < 8) stay inside the NZ-plane tile
     if (max_off + 14 * g.P + 7 >= g.NZ * g.PZ) return false;
+    // ... and inside the tap's own plane, so a slice may sit at any plane of a larger tile
+    g.pair_ok = max_inplane + 14 * g.P + 7 < g.PZ;
     if ((int)(2 * g.PW) > g.P) return false;
     return true;
 }

@@ -50,6 +50,8 @@ struct Geometry3 {
     int TW = 0, TH = 0, PW = 0, P = 0, PZ = 0, NZ = 0;
     int xs_log2 = 3, ncls_x = 1, ncls_y = 1, nZM = 0;
     int max_group_taps = 0;   // taps of the largest group of 4 consecutive phases
+    int NZ2 = 0;              // planes of the union of z-slices 2k and 2k+1 (largest k)
+    bool pair_ok = false;     // every tap's in-plane offset + lane base stays in its plane
     std::vector<float4> tapv;
     std::vector<int> tapo, tapk;
     std::vector<Phase3> phases;

@@ -720,6 +720,252 @@ __global__ void __launch_bounds__(256 * SPLIT) k3d_rgc3(const Rgc3dArgs a, const
     }
 }
 
+// One float4 of a frame's unit gradients through a bounds-checked buffer load: `off` is a
+// byte offset into the frame's `nbytes`; an offset at or past the end (kOffNone) reads 0
+// without touching memory, which is how the tile's gated-out elements are filled.
+constexpr unsigned kOffNone = 0x80000000u;
+
+__device__ __forceinline__ float4 load_u4_or_zero(const float4 *frame, unsigned nbytes,
+                                                  unsigned off)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float4 *>(frame), (short)0, (int)nbytes, 0x00020000);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+#else
+    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#endif
+}
+
+// Tiled 3-D RGC, two z-slices per block.  Consecutive slices zM = 2k, 2k+1 reach nearly the
+// same planes, so one staged tile (the union of their planes, t.NZ2) serves both: the block's
+// 8 waves are (phase 4 blockIdx.y + w, slice zs), one wave per phase and slice, and each
+// frame's tile is staged once for twice the samples of a one-slice block.  The staging loads
+// are bounds-checked buffer loads with 32-bit offsets (a frame of gradients is below 2 GiB):
+// tile elements outside the image gates read 0 through an out-of-range offset, with no
+// branch or register clearing per element.  The per-voxel divisions by the weight sum and by nFrameForSRRF
+// (live3DSRRF.cl:427,454-465) are multiplications by reciprocals taken once (within an ulp of
+// the quotient).  The taps, their order and the sample arithmetic are those of k3d_rgc3.
+template <int ZP>
+__global__ void __launch_bounds__(256 * ZP) k3d_rgc3z(const Rgc3dArgs a, const Rgc3Tables t)
+{
+    constexpr int NT = 256 * ZP;
+    constexpr int FE = K3_FE * 256 / NT;       // 4096 tile elements per block
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4 *lU = reinterpret_cast<float4 *>(smem);
+    float4 *lTv = lU + t.NZ2 * t.PZ;
+    int *lTo = reinterpret_cast<int *>(lTv + ZP * t.gtaps);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wave = wv & 3, zs = wv >> 2;     // phase slot, slice of the pair
+    const int lx = lane & 7, ly = lane >> 3;
+    const int w = a.w, h = a.h, nP = a.nP, M = a.M, MM = M * M;
+    const int tiles_x = (w + 7) >> 3;
+    const int tX = blockIdx.x % tiles_x, tY = blockIdx.x / tiles_x;
+    const int X0 = tX << 3, Y0 = tY << 3, X = X0 + lx, Y = Y0 + ly;
+    const int zM0 = blockIdx.z * ZP;
+    const int nzs = min(ZP, t.nZM - zM0);      // slices of this block
+    const int zM = zM0 + zs;
+    const int ph0 = blockIdx.y * 4;
+    const int phN = min(MM, ph0 + 4);
+    const int ph = ph0 + wave;
+    const bool active = ph < phN && zs < nzs;
+    int cls = 0;
+    if (t.ncls_x > 1 || t.ncls_y > 1)
+        cls = axis_class3(Y0, t.xs_log2) * t.ncls_x + axis_class3(X0, t.xs_log2);
+    const Phase3 *PC = t.phases + (long)cls * t.nZM * MM;
+
+    // the union of the slices' planes, and each slice's tap tables (phases ph0 .. phN-1)
+    int zlo = 1 << 30, zhi = -1;
+    for (int s = 0; s < nzs; s++) {
+        const Phase3 *PB = PC + (long)(zM0 + s) * MM;
+        const int tg0 = PB[ph0].first_tap;
+        const int tg1 = PB[phN - 1].first_tap + PB[phN - 1].ntaps;
+        for (int e = tid; e < tg1 - tg0 + TAP_SPARE; e += NT) {
+            lTv[s * t.gtaps + e] = t.tapv[tg0 + e];
+            lTo[s * t.gtaps + e] = t.tapo[tg0 + e];
+        }
+        if (PB[ph0].nz > 0) {
+            zlo = min(zlo, PB[ph0].zlo);
+            zhi = max(zhi, PB[ph0].zlo + PB[ph0].nz - 1);
+        }
+    }
+    const int nzU = zhi >= zlo ? zhi - zlo + 1 : 0;
+
+    // fill plan over the union's planes; gated-out elements are cleared here, once
+    const int wI = 2 * w, hI = 2 * h, whI = wI * hI;
+    const int per_plane = t.TH * t.TW;
+    const int nel = nzU * per_plane;
+    // goff: byte offset of the element's gradient in one frame (kOffNone outside the gates);
+    // lidx: its LDS index, -1 for the thread's unused element slots
+    unsigned goff[FE];
+    int lidx[FE];
+#pragma unroll
+    for (int q = 0; q < FE; q++) {
+        const int e = tid + q * NT;
+        goff[q] = kOffNone;
+        lidx[q] = -1;
+        if (e < nel) {
+            const int kz = e / per_plane;
+            const int r = e - kz * per_plane;
+            const int gy = r / t.TW, gx = r - gy * t.TW;
+            const int bxi = 2 * X0 - t.G0 + gx, byj = 2 * Y0 - t.G0 + gy;
+            lidx[q] = kz * t.PZ + gy * t.P + (gx & 1) * t.PW + (gx >> 1);
+            if (bxi >= 1 && bxi <= wI - 1 && byj >= 1 && byj <= hI - 1)
+                goff[q] = (unsigned)((zlo + kz) * whI + (byj - 1) * wI + (bxi - 1)) *
+                          (unsigned)sizeof(float4);
+        }
+    }
+
+    int ft = 0, nt = 0, sx = 0, sy = 0, zoff = 0;
+    float wsum = 0.0f, wsg = 0.0f;
+    if (active) {
+        const Phase3 *PB = PC + (long)zM * MM;
+        const Phase3 P = PB[ph];
+        ft = zs * t.gtaps + P.first_tap - PB[ph0].first_tap;
+        nt = P.ntaps;
+        wsum = wsg = P.wsum;
+        sx = P.sx;
+        sy = P.sy;
+        zoff = P.nz > 0 ? (P.zlo - zlo) * t.PZ : 0;
+        const bool interior = (2 * X0 + t.R2lo >= 1) && (2 * (X0 + 7) + 1 + t.R2hi <= wI - 1) &&
+                              (2 * Y0 + t.R2lo >= 1) && (2 * (Y0 + 7) + 1 + t.R2hi <= hI - 1);
+        if (!interior) {
+            wsg = 0.0f;
+            for (int k = 0; k < nt; k++) {
+                const int kk = t.tapk[P.first_tap + k];
+                const int kx = 2 * X + (kk & 255) - 128, ky = 2 * Y + ((kk >> 8) & 255) - 128;
+                if (kx >= 1 && kx <= wI - 1 && ky >= 1 && ky <= hI - 1)
+                    wsg += t.tapv[P.first_tap + k].w;
+            }
+        }
+    }
+    const float rwsg = 1.0f / wsg;
+    const bool inimg = active && X < w && Y < h;
+    const int xM = X * M + sx, yM = Y * M + sy;
+    const long whM = (long)a.wM * a.hM;
+    const long whdM = whM * nP * a.Mz;
+    const long vox = whM * zM + (long)yM * a.wM + xM;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+    if (a.count0 > 0 && inimg) {
+        s0 = a.out[vox];
+        s1 = a.out[vox + whdM];
+        s2 = a.out[vox + 2 * whdM];
+        s3 = a.out[vox + 3 * whdM];
+        s4 = a.prev[vox];
+    }
+    const float zc = (float)((zM + 0.5) / a.Mz);
+    int z0;
+    const float zcof = zc - 0.5f;
+    if (zcof < 0.0f) z0 = 0;
+    else if (zcof >= (float)(nP - 1)) z0 = nP - 2;
+    else z0 = (int)floorf(zcof);
+    const int zm1 = max(z0 - 1, 0), zp2 = min(z0 + 1, nP - 1);
+    const float z01 = zc - ((float)z0 + 0.5f);
+    const float rnF = 1.0f / (float)a.nFor, rnF1 = 1.0f / (float)(a.nFor - 1);
+
+    const long fstride = (long)nP * whI;
+    const unsigned fbytes = (unsigned)(fstride * (long)sizeof(float4));
+    {
+        const float4 *Uf = a.U + (long)a.frame0 * fstride;
+#pragma unroll
+        for (int q = 0; q < FE; q++) {
+            const float4 v = load_u4_or_zero(Uf, fbytes, goff[q]);
+            if (lidx[q] >= 0) lU[lidx[q]] = v;
+        }
+    }
+    __syncthreads();
+
+    const int base = 2 * ly * t.P + lx + zoff;
+    const float4 *Tv = lTv + ft;
+    const int *To = lTo + ft;
+    for (int fi = 0; fi < a.nframes; fi++) {
+        const bool more = fi + 1 < a.nframes;
+        float v_1 = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+        if (inimg) {
+            const float *Lf = a.L + (long)fi * nP * whM + (long)yM * a.wM + xM;
+            v_1 = Lf[whM * zm1];
+            v0 = Lf[whM * z0];
+            v1 = Lf[whM * (z0 + 1)];
+            v2 = Lf[whM * zp2];
+        }
+        float4 st[FE];
+        if (more) {
+            const float4 *Un = a.U + (long)(a.frame0 + fi + 1) * fstride;
+#pragma unroll
+            for (int q = 0; q < FE; q++) st[q] = load_u4_or_zero(Un, fbytes, goff[q]);
+        }
+        float acc = 0.0f;
+        if (active) {
+#pragma unroll 1
+            for (int k = 0; k < nt; k += 4) {
+                const int4 o = *reinterpret_cast<const int4 *>(To + k);
+                const float4 q0 = Tv[k], q1 = Tv[k + 1], q2 = Tv[k + 2], q3 = Tv[k + 3];
+                const float4 u0 = lU[base + o.x], u1 = lU[base + o.y];
+                const float4 u2 = lU[base + o.z], u3 = lU[base + o.w];
+                acc = tap3_acc(u0, q0, acc);
+                acc = tap3_acc(u1, q1, acc);
+                acc = tap3_acc(u2, q2, acc);
+                acc = tap3_acc(u3, q3, acc);
+            }
+        }
+        if (inimg) {
+            float C = (wsum + acc) * rwsg;
+            C = apply_sens3(C, a.sens, a.sens_int);
+            // axial cubic (live3DSRRF.cl:448-452), left-to-right as written
+            float aa = __fmul_rn(-0.5f, v_1);
+            aa = __fadd_rn(aa, __fmul_rn(1.5f, v0));
+            aa = __fsub_rn(aa, __fmul_rn(1.5f, v1));
+            aa = __fadd_rn(aa, __fmul_rn(0.5f, v2));
+            float bb = __fsub_rn(v_1, __fmul_rn(2.5f, v0));
+            bb = __fadd_rn(bb, __fmul_rn(2.0f, v1));
+            bb = __fsub_rn(bb, __fmul_rn(0.5f, v2));
+            const float cc = __fadd_rn(__fmul_rn(-0.5f, v_1), __fmul_rn(0.5f, v1));
+            float v = __fmul_rn(__fmul_rn(__fmul_rn(aa, z01), z01), z01);
+            v = __fadd_rn(v, __fmul_rn(__fmul_rn(bb, z01), z01));
+            v = __fadd_rn(v, __fmul_rn(cc, z01));
+            v = __fadd_rn(v, v0);
+            const float tt = a.int_weighting ? v * C : C;
+            const float e2 = a.int_weighting ? ((tt * v) * C) : C * C;
+            if (a.count0 + fi == 0) {
+                s0 = tt * rnF;
+                s1 = e2 * rnF;
+                s2 = 0.0f;
+                s3 = v * rnF;
+            } else {
+                s0 = s0 + tt * rnF;
+                s1 = s1 + e2 * rnF;
+                s2 = s2 + (tt * s4) * rnF1;
+                s3 = s3 + v * rnF;
+            }
+            s4 = tt;   // single 1/(n-1) in TAC2 (live3DSRRF.cl:472)
+        }
+        if (more) {
+            __syncthreads();   // every wave is done with this frame's tile
+#pragma unroll
+            for (int q = 0; q < FE; q++)
+                if (lidx[q] >= 0) lU[lidx[q]] = st[q];
+            __syncthreads();
+        }
+    }
+    if (inimg) {
+        a.out[vox] = s0;
+        a.out[vox + whdM] = s1;
+        a.out[vox + 2 * whdM] = s2;
+        a.out[vox + 3 * whdM] = s3;
+        a.prev[vox] = s4;
+    }
+}
+
+size_t rgc3z_lds_bytes(const Rgc3Tables &t)
+{
+    return sizeof(float4) * ((size_t)t.NZ2 * t.PZ + 2 * (size_t)t.gtaps) +
+           sizeof(int) * 2 * (size_t)t.gtaps;
+}
+
 size_t rgc3_lds_bytes(const Rgc3Tables &t)
 {
     return sizeof(float4) * ((size_t)t.NZ * t.PZ + t.gtaps) + sizeof(int) * (size_t)t.gtaps +
@@ -800,21 +1046,35 @@ hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long 
     return hipGetLastError();
 }
 
-hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
-                        size_t lds_limit, hipStream_t s, const char **kname, int kern,
-                        int split)
+hipError_t launch3d_lateral(const Rgc3dArgs &a, hipStream_t s)
 {
     if (a.nframes <= 0) return hipSuccess;
     const long nl = (long)a.nframes * a.nP * a.wM * a.hM;
     hipLaunchKernelGGL(k3d_lateral, dim3(grid1(nl)), dim3(256), 0, s, a.px, a.L, a.w, a.h, a.nP,
                        a.M, a.frame0, a.nframes, a.drift, a.count0, a.nFor);
+    return hipGetLastError();
+}
+
+hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
+                        size_t lds_limit, hipStream_t s, const char **kname, int kern,
+                        int split, int zpair)
+{
+    if (a.nframes <= 0) return hipSuccess;
     const dim3 grid((a.wM + 63) / 64, (a.hM + 3) / 4, a.nP * a.Mz);
     // (the fast kernel addresses one frame of U with 32-bit byte offsets)
     const bool off32 = (double)a.nP * 4.0 * a.w * a.h * sizeof(float4) < 4294967296.0;
     // the tiled kernel addresses one frame of U with 32-bit element offsets
     const bool off31 = (double)a.nP * 4.0 * a.w * a.h < 2147483648.0;
-    if (drift_zero && t && kern == 3 && off31 && rgc3_lds_bytes(*t) <= lds_limit) {
-        const int tiles = ((a.w + 7) >> 3) * ((a.h + 7) >> 3);
+    const int tiles = ((a.w + 7) >> 3) * ((a.h + 7) >> 3);
+    // the two-slice kernel's buffer loads address one frame of U with 31-bit byte offsets
+    const bool buf31 = (double)a.nP * 4.0 * a.w * a.h * sizeof(float4) < 2147483648.0;
+    if (drift_zero && t && kern == 3 && buf31 && zpair && t->NZ2 > 0 &&
+        rgc3z_lds_bytes(*t) <= lds_limit) {
+        const int nz2 = (a.nP * a.Mz + 1) / 2;
+        const dim3 g3(tiles, (a.M * a.M + 3) / 4, nz2);
+        hipLaunchKernelGGL(k3d_rgc3z<2>, g3, dim3(512), rgc3z_lds_bytes(*t), s, a, *t);
+        if (kname) *kname = "k3d_rgc3z";
+    } else if (drift_zero && t && kern == 3 && off31 && rgc3_lds_bytes(*t) <= lds_limit) {
         const dim3 g3(tiles, (a.M * a.M + 3) / 4, a.nP * a.Mz);
         if (split == 1)
             hipLaunchKernelGGL(k3d_rgc3<1>, g3, dim3(256), rgc3_lds_bytes(*t), s, a, *t);

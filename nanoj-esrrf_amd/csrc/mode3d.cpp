@@ -98,6 +98,8 @@ int build_tiled3(const ModeCreate &c, ModeState &ms)
     t.ncls_y = g.ncls_y;
     t.nZM = g.nZM;
     t.gtaps = g.max_group_taps + TAP_SPARE;
+    // two z-slices per block when their union tile fits the staging plan (4096 elements)
+    t.NZ2 = g.pair_ok && g.NZ2 * g.TH * g.TW <= 256 * 16 ? g.NZ2 : 0;
     ms.tiled3 = true;
     return ESRRF_OK;
 }
@@ -177,7 +179,7 @@ int mode3d_prepare(const ModeCtx &c, ModeState &ms, const void *src, int dtype, 
     return ESRRF_OK;
 }
 
-int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
+static Rgc3dArgs rgc3_args(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
 {
     Rgc3dArgs a{};
     a.U = ms.d_u3;
@@ -203,8 +205,22 @@ int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
     a.radius = r.radius;
     a.tSS = r.tSS;
     a.tSO = r.tSO;
+    return a;
+}
+
+int mode3d_lateral(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
+{
+    hipError_t e = launch3d_lateral(rgc3_args(c, ms, r), c.s);
+    if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D lateral interpolation", e);
+    return ESRRF_OK;
+}
+
+int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
+{
+    const Rgc3dArgs a = rgc3_args(c, ms, r);
     hipError_t e = launch3d_rgc(a, r.drift_zero, ms.tiled3 ? &ms.t3 : nullptr, ms.lds_limit,
-                                c.s, &ms.rgc_kernel, ms.var.k3d_kernel, ms.var.k3d_split);
+                                c.s, &ms.rgc_kernel, ms.var.k3d_kernel, ms.var.k3d_split,
+                                ms.var.k3d_zpair);
     if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D RGC launch", e);
     return ESRRF_OK;
 }

@@ -40,15 +40,19 @@ hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long 
 // fits `lds_limit`; *kname receives the name of the kernel launched.
 // kern: 3 tiled (default), 2 / 1 the per-voxel kernels; split: waves per phase in the tiled
 // kernel (2 default)
+// lateral bicubic of the frames' planes at the magnified grid (k3d_lateral), before the RGC
+hipError_t launch3d_lateral(const Rgc3dArgs &a, hipStream_t s);
 hipError_t launch3d_rgc(const Rgc3dArgs &a, bool drift_zero, const Rgc3Tables *t,
                         size_t lds_limit, hipStream_t s, const char **kname, int kern = 3,
-                        int split = 2);
+                        int split = 2, int zpair = 1);
 size_t rgc3_lds_bytes(const Rgc3Tables &t);
+size_t rgc3z_lds_bytes(const Rgc3Tables &t);   // two-slice tiled kernel
 hipError_t launch3d_count(int w, int h, int nP, int M, int Mz, float radius, float tSO,
                           unsigned long long *d_total, hipStream_t s);
 
 int mode3d_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U);
 int mode3d_prepare(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n);
+int mode3d_lateral(const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode3d_read_gradients(const ModeCtx &c, ModeState &ms, float *out);
 

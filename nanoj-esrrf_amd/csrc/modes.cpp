@@ -230,6 +230,11 @@ int mode_prepare(int mode, const ModeCtx &c, ModeState &ms, const void *src, int
     return ESRRF_OK;
 }
 
+int mode_pre_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
+{
+    return mode == 2 ? mode3d_lateral(c, ms, r) : ESRRF_OK;
+}
+
 int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r)
 {
     if (mode == 2) return mode3d_rgc(c, ms, r);

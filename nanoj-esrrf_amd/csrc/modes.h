@@ -84,6 +84,9 @@ struct ModeRgc {
 int mode_create(const ModeCreate &c, ModeState &ms, float **d_px, float2 **d_U, float **d_aux,
                 size_t *aux_bytes);
 int mode_prepare(int mode, const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n);
+// work a mode does per launch before its RGC kernel (3-D: the lateral interpolation), so
+// that the RGC can be timed alone
+int mode_pre_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode_read_gradients(int mode, const ModeCtx &c, ModeState &ms, float *out);
 long long mode_active_samples(int mode, const ModeState &ms);

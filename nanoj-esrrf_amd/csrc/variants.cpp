@@ -27,6 +27,7 @@ KernelVariants kernel_variants_from_env()
     k.fht_pair = env_int("ESRRF_FHT_PAIR", 1) != 0;
     k.k3d_kernel = env_int("ESRRF_3D_KERNEL", 3);
     k.k3d_split = env_int("ESRRF_3D_SPLIT", 2) == 1 ? 1 : 2;
+    k.k3d_zpair = env_int("ESRRF_3D_ZPAIR", 1) != 0;
     k.sweep_per_fwhm = env_int("ESRRF_SWEEP_PER_FWHM", 0) == 1;
     return k;
 }

@@ -16,6 +16,7 @@ struct KernelVariants {
     int fht_pair = 1;             // ESRRF_FHT_PAIR=0: single-pixel FHT RGC kernel
     int k3d_kernel = 3;           // ESRRF_3D_KERNEL=1/2: per-voxel 3-D RGC kernels
     int k3d_split = 2;            // ESRRF_3D_SPLIT=1: one wave per phase in k3d_rgc3
+    int k3d_zpair = 1;            // ESRRF_3D_ZPAIR=0: one z-slice per tiled 3-D block
     int sweep_per_fwhm = 0;       // ESRRF_SWEEP_PER_FWHM=1: one sweep emit launch per fwhm
 };
 

@@ -103,7 +103,8 @@ def test_3d_zero_calibration_parity(pkg, O, w, h, nP, M, fwhm, S):
 ])
 def test_3d_fast_kernels_match_window_kernel(pkg, monkeypatch, w, h, nP, M, fwhm, loads):
     """The drift-free 3-D kernels against the window-order kernel (k3d_rgc) on the same data:
-    k3d_rgc2 (per-voxel geometry once per chunk of 8 frames) and the tiled k3d_rgc3
+    k3d_rgc2 (per-voxel geometry once per chunk of 8 frames), the tiled k3d_rgc3 and its
+    two-z-slice form k3d_rgc3z (one staged tile for two slices, reciprocal multiplies)
     (wave-uniform tap tables per x/y class, z-slice and phase; LDS-staged gradient tiles;
     partial tiles, border classes, M = 2..5, one-frame and ragged loads continued across
     launches).  They sum the samples in different orders, so they agree to float rounding,
@@ -114,8 +115,10 @@ def test_3d_fast_kernels_match_window_kernel(pkg, monkeypatch, w, h, nP, M, fwhm
     mont = montage(planes, np.arange(nP))
     cd = calib_dict(*zero_calib(nP))
     res = {}
-    for kern, name in (("1", "k3d_rgc"), ("2", "k3d_rgc2"), ("3", "k3d_rgc3")):
-        monkeypatch.setenv("ESRRF_3D_KERNEL", kern)
+    for kern, name in (("1", "k3d_rgc"), ("2", "k3d_rgc2"), ("3", "k3d_rgc3"),
+                       ("3z", "k3d_rgc3z")):
+        monkeypatch.setenv("ESRRF_3D_KERNEL", kern[0])
+        monkeypatch.setenv("ESRRF_3D_ZPAIR", "1" if kern == "3z" else "0")
         e = pkg.LiveSRRF_CL()
         e.checkDevices()
         e.initialise(3 * w, 3 * h, M, fwhm, 2, max(b - a for a, b in loads), n, 20000,
@@ -127,7 +130,7 @@ def test_3d_fast_kernels_match_window_kernel(pkg, monkeypatch, w, h, nP, M, fwhm
         assert e._e().rgc_kernel() == name
         res[kern] = np.asarray(e.readSRRFbuffer()).reshape(4, -1)
         e.release()
-    for kern in ("2", "3"):
+    for kern in ("2", "3", "3z"):
         for c in range(4):
             scale = max(float(np.abs(res["1"][c]).max()), 1e-30)
             b = res["1"][c] / scale

@@ -115,7 +115,7 @@ def test_full_size_3d_locality(pkg):
         e.resetFramePosition()
         e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
         e.calculateSRRF(mont)
-        assert e._e().rgc_kernel() == "k3d_rgc3"
+        assert e._e().rgc_kernel() == "k3d_rgc3z"
         out = np.asarray(e.readSRRFbuffer()).reshape(4, nP * 4, pl.shape[2] * 4, pl.shape[3] * 4)
         e.release()
         return out
