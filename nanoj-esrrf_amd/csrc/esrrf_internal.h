@@ -50,7 +50,8 @@ struct Phase {
     float wy[4];
     int sx, sy;     // sub-pixel phase (xM mod M, yM mod M)
     float wsumd;    // sum of dW/dtSS over all taps (same order as wsum)
-    int pad1;
+    int ntaps2;     // gated taps rounded up to an even count (<= ntaps): the taps beyond it
+                    // are zero-weight padding, which adds W * h = -0 to a sum
 };
 
 // Launch parameters of the BIC 2-D radial-gradient-convergence kernel.

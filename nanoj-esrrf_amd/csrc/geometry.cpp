@@ -143,6 +143,7 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
                             wsumd = wsumd + Wd;
                         }
                     }
+                    P.ntaps2 = ((int)g.taps.size() - P.first_tap + 1) & ~1;
                     // pad to a multiple of 4 with zero-weight taps (the kernel walks the
                     // list in chunks of 4); they add W * h = 0 and are not part of wsum
                     while (((int)g.taps.size() - P.first_tap) % 4 != 0) {

@@ -304,6 +304,8 @@ hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h
 constexpr int RAW_T = 12;   // raw intensity tile: image columns X0-2 .. X0+9
 constexpr int RAW_P = 72;   // pitch = 8 (mod 64): the 8x8 lanes of a read hit distinct banks
 constexpr int RAW_SZ = RAW_T * RAW_P;
+constexpr int XS_PER_SX = RAW_T * 8;   // x row sums of one column phase: tile rows x lx
+constexpr int XS_MAX_M = 8;           // frame-pair kernel: M * M <= 16 waves x 4 phases
 
 __device__ __forceinline__ int axis_class(int X0, int xs_log2)
 {
@@ -753,7 +755,7 @@ __device__ __forceinline__ void cumulate(float &avg, float &e2s, float &tac, flo
 // evaluated and only stored to LDS between the barriers.
 template <int PPW, int MODE, bool INTW, bool TSM, int STG>
 __global__ void __launch_bounds__(1024)
-__attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
+__attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
                                                    const Tap *__restrict__ taps,
                                                    const Phase *__restrict__ phases,
                                                    const int *__restrict__ tapk)
@@ -762,7 +764,12 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
     float4 *lU = reinterpret_cast<float4 *>(smem);
     const int usz = 4 * a.PP * a.PH;
     float2 *lR = reinterpret_cast<float2 *>(lU + usz);   // raw tiles of f and f+1, interleaved
-    Tap *lT = reinterpret_cast<Tap *>(lR + RAW_SZ);
+    float4 *lXW = reinterpret_cast<float4 *>(lR + RAW_SZ);   // [M] Catmull-Rom x weights
+    float2 *lXS = reinterpret_cast<float2 *>(lXW + a.M);    // [M][RAW_T][8] x row sums
+    int *lXD = reinterpret_cast<int *>(lXS + a.M * XS_PER_SX);   // [M] first tile column - lx
+    // (every region above is a multiple of 16 bytes long, so the tap table is 16-byte
+    // aligned and read with one 16-byte LDS load per tap)
+    Tap *lT = static_cast<Tap *>(__builtin_assume_aligned(lXD + XS_MAX_M, 16));
     int *lPlan = TSM ? reinterpret_cast<int *>(lT)
                      : reinterpret_cast<int *>(lT + a.max_class_taps + TAP_SPARE);
     constexpr int PH_UNROLL = PPW <= 4 ? PPW : 1;
@@ -785,12 +792,12 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
 
     float s_avg[PPW], s_e2[PPW], s_tac[PPW], s_int[PPW], s_prv[PPW];
     float invw[PPW];
-    int ibase[PPW];
+    int xbase[PPW];
 #pragma unroll
     for (int p = 0; p < PPW; p++) {
         const int ph = wave + p * a.nw;
         invw[p] = 0.0f;
-        ibase[p] = -1;
+        xbase[p] = -1;
         s_avg[p] = s_e2[p] = s_tac[p] = s_int[p] = s_prv[p] = 0.0f;
         if (ph < MM) {
             const Phase P = PHS[ph];
@@ -807,9 +814,11 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
                 }
             }
             invw[p] = 1.0f / ws;
+            // bicubic regime: the four x row sums of the phase's column phase sx, tile rows
+            // v0 - 1 .. v0 + 2 (tile row 0 = image row Y0 - 2)
             const int u0 = X + P.du, v0 = Y + P.dv;
             if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2)
-                ibase[p] = (v0 - 1 - (Y0 - 2)) * RAW_P + (u0 - 1 - (X0 - 2));
+                xbase[p] = (P.sx * RAW_T + (v0 - 1 - (Y0 - 2))) * 8 + lx;
             if (MODE == 0 && !a.first_init && inimg) {
                 const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
                 s_avg[p] = a.out[pix];
@@ -842,7 +851,40 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
         const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
         for (int e = tid; e < tcls1 - tcls0 + TAP_SPARE; e += nthr) lT[e] = taps[tcls0 + e];
     }
+    // x weights and column offset of each column phase sx (phase (0, sx) of the class: both
+    // depend on sx only)
+    for (int e = tid; e < M; e += nthr) {
+        const Phase P = PHS[e];
+        lXW[e] = make_float4(P.wx[0], P.wx[1], P.wx[2], P.wx[3]);
+        lXD[e] = P.du + 1;
+    }
     __syncthreads();
+
+    // The bicubic intensity (liveSRRF.cl:361) sums each of the 4 rows along x first, then
+    // the 4 row sums along y.  A row sum depends on the tile row, the macro-pixel column lx
+    // and the column phase sx only, so the block forms each (sx, tile row, lx) sum of frames
+    // f and f+1 once per frame pair, in the same order, and every phase of that column
+    // phase reads four of them.
+    auto xpass = [&]() {
+        const int nxs = M * XS_PER_SX;
+        for (int e = tid; e < nxs; e += nthr) {
+            const int sx = e / XS_PER_SX;
+            const int rr = e - sx * XS_PER_SX;
+            const float4 wv = lXW[sx];
+            const float2 *src = lR + (rr >> 3) * RAW_P + (rr & 7) + lXD[sx];
+            const float2 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
+            float pa = 0.0f, pb = 0.0f;
+            pa = fmaf(v0.x, wv.x, pa);
+            pb = fmaf(v0.y, wv.x, pb);
+            pa = fmaf(v1.x, wv.y, pa);
+            pb = fmaf(v1.y, wv.y, pb);
+            pa = fmaf(v2.x, wv.z, pa);
+            pb = fmaf(v2.y, wv.z, pb);
+            pa = fmaf(v3.x, wv.w, pa);
+            pb = fmaf(v3.y, wv.w, pb);
+            lXS[e] = make_float2(pa, pb);
+        }
+    };
 
     // frames f and f+1 (f+1 only when `two`) into the LDS tiles
     auto fill = [&](int f, bool two) {
@@ -869,48 +911,61 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
     };
 
     // register staging (STG > 0; the launch guarantees nU <= STG * nthr and
-    // RAW_T^2 <= nthr)
+    // RAW_T^2 <= nthr).  A thread stages the same tile elements for every frame pair, so
+    // their offsets within a frame (-1: outside the image, the element stays 0) and their
+    // LDS slots are formed once.
     float4 su[STG > 0 ? STG : 1];
     float2 sr = make_float2(0.0f, 0.0f);
+    int sgo[STG > 0 ? STG : 1], sdst[STG > 0 ? STG : 1];
+    int rgo = -1;
+#pragma unroll
+    for (int k = 0; k < STG; k++) {
+        const int e = tid + k * nthr;
+        sgo[k] = -1;
+        sdst[k] = -1;
+        if (e < nU) {
+            const int pk = lPlan[e];
+            const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
+            const int kx = kxo + gx, ky = kyo + gy;
+            sdst[k] = pk & 0x7fff;
+            if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI)
+                sgo[k] = ky * wI + kx;
+        }
+    }
+    if (STG && tid < RAW_T * RAW_T) {
+        const int r = tid / RAW_T, c = tid - r * RAW_T;
+        rgo = min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+    }
     auto prefetch = [&](int f, bool two) {
         const float2 *Ua = a.U + whI * f;
 #pragma unroll
         for (int k = 0; k < STG; k++) {
-            const int e = tid + k * nthr;
             float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (e < nU) {
-                const int pk = lPlan[e];
-                const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
-                const int kx = kxo + gx, ky = kyo + gy;
-                if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI) {
-                    const long g = (long)ky * wI + kx;
-                    const float2 ua = Ua[g];
-                    const float2 ub = two ? Ua[g + whI] : make_float2(0.0f, 0.0f);
-                    v = make_float4(ua.x, ua.y, ub.x, ub.y);
-                }
+            if (sgo[k] >= 0) {
+                const float2 ua = Ua[sgo[k]];
+                const float2 ub = two ? Ua[sgo[k] + whI] : make_float2(0.0f, 0.0f);
+                v = make_float4(ua.x, ua.y, ub.x, ub.y);
             }
             su[k] = v;
         }
-        if (tid < RAW_T * RAW_T) {
+        if (rgo >= 0) {
             const float *Pa = a.px + wh * f;
-            const int r = tid / RAW_T, c = tid - r * RAW_T;
-            const long g = (long)min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
-            sr = make_float2(Pa[g], two ? Pa[g + wh] : 0.0f);
+            sr = make_float2(Pa[rgo], two ? Pa[rgo + wh] : 0.0f);
         }
     };
     auto commit = [&]() {
 #pragma unroll
-        for (int k = 0; k < STG; k++) {
-            const int e = tid + k * nthr;
-            if (e < nU) lU[lPlan[e] & 0x7fff] = su[k];
-        }
-        if (tid < RAW_T * RAW_T) {
+        for (int k = 0; k < STG; k++)
+            if (sdst[k] >= 0) lU[sdst[k]] = su[k];
+        if (rgo >= 0) {
             const int r = tid / RAW_T, c = tid - r * RAW_T;
             lR[r * RAW_P + c] = sr;
         }
     };
 
     fill(a.frame0, a.nframes > 1);
+    __syncthreads();
+    xpass();
     __syncthreads();
 
     for (int fi = 0; fi < a.nframes; fi += 2) {
@@ -939,8 +994,12 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
                     n2 = Tl[2];
                     n3 = Tl[3];
                 }
+                // (from the LDS tables the loop stops at the even tap count: the zero-weight
+                // padding beyond it adds -0 to each sum)
+                const int n4 = TSM ? P.ntaps : (P.ntaps2 & ~3);
+                int t = 0;
 #pragma unroll 1
-                for (int t = 0; t < P.ntaps; t += 4) {
+                for (; t < n4; t += 4) {
                     Tap c0, c1, c2, c3;
                     if (TSM) {
                         c0 = n0;
@@ -970,25 +1029,28 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
                     acc0 = tap_acc(make_float2(u3.x, u3.y), c3, acc0);
                     acc1 = tap_acc(make_float2(u3.z, u3.w), c3, acc1);
                 }
+                if (!TSM && (P.ntaps2 & 2)) {
+                    const Tap c0 = Tl[t], c1 = Tl[t + 1];
+                    const float4 u0 = lU[base + c0.off];
+                    const float4 u1 = lU[base + c1.off];
+                    acc0 = tap_acc(make_float2(u0.x, u0.y), c0, acc0);
+                    acc1 = tap_acc(make_float2(u0.z, u0.w), c0, acc1);
+                    acc0 = tap_acc(make_float2(u1.x, u1.y), c1, acc0);
+                    acc1 = tap_acc(make_float2(u1.z, u1.w), c1, acc1);
+                }
                 const float C0a = (P.wsum + acc0) * invw[p];
                 const float C0b = (P.wsum + acc1) * invw[p];
 
                 // interpolated intensity at (xM/M - 0.5, yM/M - 0.5) (liveSRRF.cl:361)
                 float va, vb;
-                if (ibase[p] >= 0) {
-                    const float2 *row = lR + ibase[p];
+                if (xbase[p] >= 0) {
+                    const float2 *xs = lXS + xbase[p];
                     float qa = 0.0f, qb = 0.0f;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        float pa = 0.0f, pb = 0.0f;
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const float2 r2 = row[j * RAW_P + i];
-                            pa = fmaf(r2.x, P.wx[i], pa);
-                            pb = fmaf(r2.y, P.wx[i], pb);
-                        }
-                        qa = fmaf(pa, P.wy[j], qa);
-                        qb = fmaf(pb, P.wy[j], qb);
+                        const float2 r2 = xs[j * 8];
+                        qa = fmaf(r2.x, P.wy[j], qa);
+                        qb = fmaf(r2.y, P.wy[j], qb);
                     }
                     va = qa;
                     vb = qb;
@@ -1027,6 +1089,8 @@ __attribute__((amdgpu_waves_per_eu(STG ? 4 : 5))) k_bic_rgc2(const BicRgcArgs a,
                 commit();
             else
                 fill(a.frame0 + fi + 2, fi + 3 < a.nframes);
+            __syncthreads();
+            xpass();
             __syncthreads();
         }
     }
@@ -1298,6 +1362,8 @@ hipError_t launch_bic_emit_multi(const BicRgcArgs &a, int ppw, const TapM *taps,
 size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm)
 {
     return sizeof(float4) * 4 * (size_t)a.PP * a.PH + sizeof(float2) * RAW_SZ +
+           (sizeof(float4) + sizeof(float2) * XS_PER_SX) * (size_t)a.M +
+           sizeof(int) * XS_MAX_M +
            (tsm ? 0 : sizeof(Tap) * ((size_t)a.max_class_taps + TAP_SPARE)) +
            sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
 }
