@@ -68,8 +68,16 @@ __device__ __forceinline__ void load_grad_tiles(const float *__restrict__ P, int
     __syncthreads();
 }
 
+// T / |T|, or 0 for a zero or non-finite T.  Where |T|^2 lies well inside the float range
+// (all gradients of 16-bit frames) one reciprocal square root normalises it; elsewhere T is
+// first scaled by its largest component so that the square neither overflows nor underflows.
 __device__ __forceinline__ float2 unit_or_zero(float tx, float ty)
 {
+    const float t = fmaf(tx, tx, ty * ty);
+    if (t >= 1e-30f && t <= 1e30f) {
+        const float r = rsqrtf(t);
+        return make_float2(tx * r, ty * r);
+    }
     const float m = fmaxf(fabsf(tx), fabsf(ty));
     if (!(m > 0.0f) || !(m <= 3.4e38f)) return make_float2(0.0f, 0.0f);
     const float sx = tx / m, sy = ty / m;
@@ -140,7 +148,6 @@ __global__ void __launch_bounds__(256) k_bic_prep(const float *__restrict__ px,
 // ---------------------------------------------------------------------------------------
 constexpr int P2_T = 32;   // sample points per side of a block
 constexpr int P2_G = 20;   // gradient tile: image columns kx0/2 - 2 .. kx0/2 + 17
-constexpr int P2_R = 21;   // raw tile: one more column and row on the low side
 
 __device__ __forceinline__ void cubic_half_weights(int odd, float &w0, float &w1, float &w2,
                                                    float &w3)
@@ -155,7 +162,6 @@ __device__ __forceinline__ void cubic_half_weights(int odd, float &w0, float &w1
 __global__ void __launch_bounds__(256) k_bic_prep2(const float *__restrict__ px,
                                                    float2 *__restrict__ U, int w, int h)
 {
-    __shared__ float lraw[P2_R * P2_R];
     __shared__ float lgx[P2_G * P2_G], lgy[P2_G * P2_G];
     __shared__ float rsx[P2_G * P2_T], rsy[P2_G * P2_T];   // row sums [tile row][sample x]
     const int f = blockIdx.z;
@@ -163,25 +169,33 @@ __global__ void __launch_bounds__(256) k_bic_prep2(const float *__restrict__ px,
     const int gx0 = (kx0 >> 1) - 2, gy0 = (ky0 >> 1) - 2;
     const int tid = threadIdx.x;
     const float *P = px + (long)w * h * f;
-    for (int e = tid; e < P2_R * P2_R; e += 256) {
-        const int r = e / P2_R, c = e - r * P2_R;
-        const int ix = min(max(gx0 - 1 + c, 0), w - 1);
-        const int iy = min(max(gy0 - 1 + r, 0), h - 1);
-        lraw[e] = P[(long)iy * w + ix];
-    }
-    __syncthreads();
-    // 2-point gradient (liveSRRF.cl:199-204) at image (gx0 + c, gy0 + r)
-    for (int e = tid; e < P2_G * P2_G; e += 256) {
+    // 2-point gradient (liveSRRF.cl:199-204) at image (gx0 + c, gy0 + r): p minus its left
+    // and upper neighbours, edge-clamped, 0 outside the image.  Each thread reads its points'
+    // three pixels straight from the frame and issues every load before the first use.
+    constexpr int GK = (P2_G * P2_G + 255) / 256;
+    float vp[GK], vl[GK], vu[GK];
+    bool ok[GK];
+#pragma unroll
+    for (int k = 0; k < GK; k++) {
+        const int e = tid + k * 256;
         const int r = e / P2_G, c = e - r * P2_G;
         const int x1 = gx0 + c, y1 = gy0 + r;
-        float gx = 0.0f, gy = 0.0f;
-        if (x1 >= 0 && x1 < w && y1 >= 0 && y1 < h) {
-            const float p = lraw[(r + 1) * P2_R + (c + 1)];
-            gx = p - lraw[(r + 1) * P2_R + c];
-            gy = p - lraw[r * P2_R + (c + 1)];
+        ok[k] = e < P2_G * P2_G && x1 >= 0 && x1 < w && y1 >= 0 && y1 < h;
+        vp[k] = vl[k] = vu[k] = 0.0f;
+        if (ok[k]) {
+            const float *row = P + (long)y1 * w;
+            vp[k] = row[x1];
+            vl[k] = row[max(x1 - 1, 0)];
+            vu[k] = P[(long)max(y1 - 1, 0) * w + x1];
         }
-        lgx[e] = gx;
-        lgy[e] = gy;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; k++) {
+        const int e = tid + k * 256;
+        if (e < P2_G * P2_G) {
+            lgx[e] = ok[k] ? vp[k] - vl[k] : 0.0f;
+            lgy[e] = ok[k] ? vp[k] - vu[k] : 0.0f;
+        }
     }
     __syncthreads();
     // row sums along x: GxInt samples x = kx, GyInt samples x = kx - 1
@@ -984,7 +998,7 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
                 float acc0 = 0.0f, acc1 = 0.0f;
                 const Tap *Tl = TSM ? taps + P.first_tap : lT + (P.first_tap - tcls0);
                 // (one chunk of 4 taps per iteration: two in flight exceed the register
-                // budget of 5 waves per SIMD; with TSM the next chunk's taps are loaded
+                // budget of 4 waves per SIMD; with TSM the next chunk's taps are loaded
                 // into SGPRs during this one -- every phase's list is a multiple of 4 long
                 // and the table ends in TAP_SPARE zero taps, so t + 7 stays inside it)
                 Tap n0, n1, n2, n3;
