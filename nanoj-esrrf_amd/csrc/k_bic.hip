@@ -24,6 +24,7 @@
 
 #include "device_cumulants.h"
 #include "device_interp.h"
+#include "device_tiles.h"
 #include "esrrf_internal.h"
 
 namespace esrrf {
@@ -398,7 +399,8 @@ __attribute__((amdgpu_waves_per_eu(PPW <= 5 ? 5 : 4))) k_bic_rgc(const BicRgcArg
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lx = lane & 7, ly = lane >> 3;
-    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int lin = xcd_tile_1d(a.tiles_x * a.tiles_y);
+    const int tX = lin % a.tiles_x, tY = lin / a.tiles_x;
     const int X0 = tX << 3, Y0 = tY << 3;
     const int X = X0 + lx, Y = Y0 + ly;
     const int M = a.M, MM = M * M;
@@ -792,7 +794,8 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lx = lane & 7, ly = lane >> 3;
-    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int lin = xcd_tile_1d(a.tiles_x * a.tiles_y);
+    const int tX = lin % a.tiles_x, tY = lin / a.tiles_x;
     const int X0 = tX << 3, Y0 = tY << 3;
     const int X = X0 + lx, Y = Y0 + ly;
     const int M = a.M, MM = M * M;
@@ -1173,7 +1176,8 @@ __global__ void __launch_bounds__(1024) k_bic_emit_multi(const BicRgcArgs a,
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lx = lane & 7, ly = lane >> 3;
-    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int lin = xcd_tile_1d(a.tiles_x * a.tiles_y);
+    const int tX = lin % a.tiles_x, tY = lin / a.tiles_x;
     const int X0 = tX << 3, Y0 = tY << 3;
     const int X = X0 + lx, Y = Y0 + ly;
     const int M = a.M, MM = M * M;

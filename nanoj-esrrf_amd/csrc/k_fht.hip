@@ -15,6 +15,7 @@
 // load and the (Re - Im) crop-scatter into the last store.
 #include "device_fft.h"
 #include "device_interp.h"
+#include "device_tiles.h"
 #include "esrrf_internal.h"
 #include "fht_internal.h"
 
@@ -435,7 +436,8 @@ __global__ void __launch_bounds__(256) k_fht_rgc(const FhtRgcArgs a,
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lx = (lane & 7) + 8 * (wave & 1), ly = (lane >> 3) + 8 * (wave >> 1);
-    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int lin = xcd_tile_1d(a.tiles_x * a.tiles_y);
+    const int tX = lin % a.tiles_x, tY = lin / a.tiles_x;
     const int X0 = tX * FT, Y0 = tY * FT;
     const int xM = X0 + lx, yM = Y0 + ly;
     const int W = a.W, H = a.H;
@@ -546,7 +548,8 @@ __global__ void __launch_bounds__(512) k_fht_rgc2(const FhtRgcArgs a,
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lx = (lane & 7) + 16 * (wave & 1), ly = (lane >> 3) + 8 * (wave >> 1);
-    const int tX = blockIdx.x % a.tiles_x, tY = blockIdx.x / a.tiles_x;
+    const int lin = xcd_tile_1d(a.tiles_x * a.tiles_y);
+    const int tX = lin % a.tiles_x, tY = lin / a.tiles_x;
     const int X0 = tX * FT2, Y0 = tY * FT2;
     const int xa = X0 + lx, xb = xa + 8, yM = Y0 + ly;
     const int W = a.W, H = a.H;
