@@ -91,6 +91,17 @@ int esrrf_calculate(esrrf_engine *e, const void *frames, int dtype, int n_frames
 int esrrf_calculate_device(esrrf_engine *e, const void *d_frames, int dtype, int n_frames,
                            const volatile int *cancel, int *cancelled);
 
+/* LiveSRRF_CL.prepareDataFHT_SRRF (:597-633), FHT mode only: upload n_frames host frames,
+ * Fourier-upsample them by the magnification (a plain copy at M = 1) and form their
+ * gradients, without the RGC; resets the load counter like kernelCalculateGradient.
+ * esrrf_read_gradients then returns frame 0's Gx, Gy.  ESRRF_ERR_STATE in other modes. */
+int esrrf_prepare_fht(esrrf_engine *e, const void *frames, int dtype, int n_frames);
+/* LiveSRRF_CL.reshapeImageStack3D (:963-994), 3-D mode only: n_frames host montage frames
+ * (width x height) -> out [n_frames][n_planes][height_s][width_s] float, each plane the
+ * calibration's montage cell, (p - background) / intensity scaling, computed on the
+ * device.  Leaves the SR state unchanged.  ESRRF_ERR_STATE in other modes. */
+int esrrf_reshape_3d(esrrf_engine *e, const void *frames, int dtype, int n_frames, float *out);
+
 /* LiveSRRF_CL.readSRRFbuffer (:749-794): VAR/TAC2 finalise, MP correction (BIC and 3-D
  * only, :762), NaN -> 0, copy out [4][n_planes_m][hM][wM] (AVG, VAR, TAC2, INT). */
 int esrrf_read_result(esrrf_engine *e, float *out);

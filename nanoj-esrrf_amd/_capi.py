@@ -23,7 +23,8 @@ _ERRNAMES = {1: "ESRRF_ERR_INVALID", 2: "ESRRF_ERR_HIP", 3: "ESRRF_ERR_STATE",
 EXPORTS = [
     "esrrf_device_count", "esrrf_device_name", "esrrf_fastest_device", "esrrf_create",
     "esrrf_destroy", "esrrf_load_drift", "esrrf_reset_frame_position", "esrrf_calculate",
-    "esrrf_calculate_device", "esrrf_read_result", "esrrf_read_result_device",
+    "esrrf_calculate_device", "esrrf_prepare_fht", "esrrf_reshape_3d",
+    "esrrf_read_result", "esrrf_read_result_device",
     "esrrf_read_gradients", "esrrf_read_aligned", "esrrf_read_mp_maps",
     "esrrf_read_frame_counters", "esrrf_get_dims", "esrrf_finish", "esrrf_set_stream",
     "esrrf_set_upload_chunks", "esrrf_host_alloc", "esrrf_host_free",
@@ -151,6 +152,8 @@ def lib():
         "esrrf_read_result_device": [vp, vp],
         "esrrf_read_gradients": [vp, fp],
         "esrrf_read_aligned": [vp, fp],
+        "esrrf_prepare_fht": [vp, vp, c_int, c_int],
+        "esrrf_reshape_3d": [vp, vp, c_int, c_int, fp],
         "esrrf_read_mp_maps": [vp, fp],
         "esrrf_read_frame_counters": [vp, ip],
         "esrrf_get_dims": [vp, ip, ip, ip, ip, ip, ip],

@@ -664,6 +664,53 @@ int esrrf_read_gradients(esrrf_engine *e, float *out)
     return ESRRF_OK;
 }
 
+int esrrf_prepare_fht(esrrf_engine *e, const void *frames, int dtype, int n_frames)
+{
+    if (!e || !frames) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    if (e->mode != MODE_FHT)
+        return fail(e, ESRRF_ERR_STATE, "prepareDataFHT_SRRF exists only in FHT mode");
+    if (dtype != ESRRF_U16 && dtype != ESRRF_F32)
+        return fail(e, ESRRF_ERR_INVALID, "dtype must be ESRRF_U16 or ESRRF_F32");
+    if (n_frames <= 0 || n_frames > e->p.n_frames_on_gpu)
+        return fail(e, ESRRF_ERR_INVALID, "n_frames=%d outside [1, n_frames_on_gpu=%d]",
+                    n_frames, e->p.n_frames_on_gpu);
+    const size_t esz = dtype == ESRRF_U16 ? 2 : 4;
+    HIPCHK(e, hipMemcpyAsync(e->d_in, frames, esz * (size_t)e->W * e->H * n_frames,
+                             hipMemcpyHostToDevice, e->stream));
+    ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,
+                 e->d_px, e->d_U, e->d_aux, e->profiling};
+    const int rc = mode_prepare(1, c, e->ms, e->d_in, dtype, n_frames);
+    if (rc != ESRRF_OK) return fail(e, rc, "%s", e->ms.err.c_str());
+    e->counters[1] = 0;   // the gradient kernel resets the load counter (liveSRRF.cl:207)
+    e->loaded = n_frames;
+    HIPCHK(e, hipStreamSynchronize(e->stream));   // the host frames are free on return
+    return ESRRF_OK;
+}
+
+int esrrf_reshape_3d(esrrf_engine *e, const void *frames, int dtype, int n_frames, float *out)
+{
+    if (!e || !frames || !out) return ESRRF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    (void)hipSetDevice(e->dev);
+    if (e->mode != MODE_3D)
+        return fail(e, ESRRF_ERR_STATE, "reshapeImageStack3D exists only in 3-D mode");
+    if (dtype != ESRRF_U16 && dtype != ESRRF_F32)
+        return fail(e, ESRRF_ERR_INVALID, "dtype must be ESRRF_U16 or ESRRF_F32");
+    if (n_frames <= 0 || n_frames > e->p.n_frames_on_gpu)
+        return fail(e, ESRRF_ERR_INVALID, "n_frames=%d outside [1, n_frames_on_gpu=%d]",
+                    n_frames, e->p.n_frames_on_gpu);
+    const size_t esz = dtype == ESRRF_U16 ? 2 : 4;
+    HIPCHK(e, hipMemcpyAsync(e->d_in, frames, esz * (size_t)e->W * e->H * n_frames,
+                             hipMemcpyHostToDevice, e->stream));
+    ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,
+                 e->d_px, e->d_U, e->d_aux, e->profiling};
+    const int rc = mode_reshape_3d(c, e->ms, e->d_in, dtype, n_frames, out);
+    if (rc != ESRRF_OK) return fail(e, rc, "%s", e->ms.err.c_str());
+    return ESRRF_OK;
+}
+
 int esrrf_read_aligned(esrrf_engine *e, float *out)
 {
     if (!e || !out) return ESRRF_ERR_INVALID;

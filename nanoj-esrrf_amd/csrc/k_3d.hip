@@ -1038,6 +1038,16 @@ hipError_t launch3d_prepare(const void *src, int dtype, const Prep3dArgs &p, int
     return hipGetLastError();
 }
 
+hipError_t launch3d_reshape(const void *src, int dtype, const Prep3dArgs &p, int n,
+                            hipStream_t s)
+{
+    const long npx = (long)n * p.nP * p.w * p.h;
+    if (npx == 0) return hipSuccess;
+    hipLaunchKernelGGL(k3d_reshape, dim3(grid1(npx)), dim3(256), 0, s, src, dtype, p.planes,
+                       p.W, p.H, p.w, p.h, p.nP, p.cell, p.ic, p.bg, npx);
+    return hipGetLastError();
+}
+
 hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long gn,
                          hipStream_t s)
 {

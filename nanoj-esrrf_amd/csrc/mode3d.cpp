@@ -179,6 +179,22 @@ int mode3d_prepare(const ModeCtx &c, ModeState &ms, const void *src, int dtype, 
     return ESRRF_OK;
 }
 
+// The planes buffer is the reshape stage of every load; the SR state (aligned planes,
+// gradients, cumulants) does not depend on its contents after the load, so it serves as
+// the device side of this read-back.
+int mode3d_reshape(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n,
+                   float *out)
+{
+    const Prep3dArgs p = prep_args(c, ms);
+    hipError_t e = launch3d_reshape(src, dtype, p, n, c.s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, ms.d_planes, sizeof(float) * (size_t)n * c.nP * c.w * c.h,
+                           hipMemcpyDeviceToHost, c.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.s);
+    if (e != hipSuccess) return err3(ms, ESRRF_ERR_HIP, "3-D reshape", e);
+    return ESRRF_OK;
+}
+
 static Rgc3dArgs rgc3_args(const ModeCtx &c, ModeState &ms, const ModeRgc &r)
 {
     Rgc3dArgs a{};

@@ -34,6 +34,9 @@ struct Rgc3dArgs {
 
 hipError_t launch3d_prepare(const void *src, int dtype, const Prep3dArgs &p, int n,
                             hipStream_t s);
+// k3d_reshape alone: montage cells of n frames -> p.planes
+hipError_t launch3d_reshape(const void *src, int dtype, const Prep3dArgs &p, int n,
+                            hipStream_t s);
 hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long gn,
                          hipStream_t s);
 // The tiled kernel (k3d_rgc3) runs for frames without drift when `t` is given and its LDS
@@ -55,5 +58,8 @@ int mode3d_prepare(const ModeCtx &c, ModeState &ms, const void *src, int dtype, 
 int mode3d_lateral(const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode3d_rgc(const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode3d_read_gradients(const ModeCtx &c, ModeState &ms, float *out);
+// reshapeImageStack3D: n montage frames at src (device) -> out (host) [n][nP][h][w]
+int mode3d_reshape(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n,
+                   float *out);
 
 }  // namespace esrrf

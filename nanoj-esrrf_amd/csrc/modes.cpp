@@ -315,6 +315,12 @@ int mode_read_gradients(int mode, const ModeCtx &c, ModeState &ms, float *out)
     return ESRRF_OK;
 }
 
+int mode_reshape_3d(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n,
+                    float *out)
+{
+    return mode3d_reshape(c, ms, src, dtype, n, out);
+}
+
 long long mode_active_samples(int, const ModeState &ms) { return ms.active; }
 
 void free_mode_state(ModeState &ms)

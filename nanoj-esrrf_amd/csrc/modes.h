@@ -89,6 +89,9 @@ int mode_prepare(int mode, const ModeCtx &c, ModeState &ms, const void *src, int
 int mode_pre_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode_rgc(int mode, const ModeCtx &c, ModeState &ms, const ModeRgc &r);
 int mode_read_gradients(int mode, const ModeCtx &c, ModeState &ms, float *out);
+// 3-D mode: reshapeImageStack3D of n montage frames at src (device) into out (host)
+int mode_reshape_3d(const ModeCtx &c, ModeState &ms, const void *src, int dtype, int n,
+                    float *out);
 long long mode_active_samples(int mode, const ModeState &ms);
 void free_mode_state(ModeState &ms);
 

@@ -169,15 +169,7 @@ class LiveSRRF_CL:
     # -- :636-746
     def calculateSRRF(self, imsRawData, cancel=None) -> bool:
         e = self._e()
-        a = np.asarray(imsRawData)
-        if a.ndim == 2:
-            a = a[None]
-        if a.dtype == np.uint16:
-            dt = C.ESRRF_U16
-        else:
-            a = a.astype(np.float32, copy=False)
-            dt = C.ESRRF_F32
-        a = np.ascontiguousarray(a)
+        a, dt = self._frames(imsRawData)
         flag = ctypes.c_int(0)
         cancelled = ctypes.c_int(0)
         if cancel is not None and cancel():
@@ -238,21 +230,33 @@ class LiveSRRF_CL:
         if self._eng is not None:
             self._eng.check(self._eng.L.esrrf_finish(self._eng.h))
 
-    # -- :963-994 (host-side montage split; the engine does the same on the device)
+    # -- :963-994
     def reshapeImageStack3D(self, ims):
-        c = self._calib
-        ims = np.asarray(ims, dtype=np.float32)
-        n, H, W = ims.shape
-        wS, hS = W // 3, H // 3
-        bg = np.float32(self._bg)
-        out = np.empty((n, len(c["ROI #"]), hS, wS), np.float32)
-        for z, roi in enumerate(c["ROI #"]):
-            i = int(roi) % 3
-            j = (int(roi) - i) // 3
-            x = int(np.floor(np.float32(W) / np.float32(3) * np.float32(i) + np.float32(0.5)))
-            y = int(np.floor(np.float32(H) / np.float32(3) * np.float32(j) + np.float32(0.5)))
-            out[:, z] = (ims[:, y:y + hS, x:x + wS] - bg) / np.float32(c["Intensity scaling"][z])
+        """Montage frames -> [n][nPlanes][heightS][widthS] float planes, (p - background) /
+        intensity scaling per calibration cell, computed on the device (esrrf_reshape_3d)."""
+        e = self._e()
+        a, dt = self._frames(ims)
+        out = np.empty((a.shape[0], e.nPlanes, e.heightS, e.widthS), np.float32)
+        e.check(e.L.esrrf_reshape_3d(e.h, a.ctypes.data_as(ctypes.c_void_p), dt, a.shape[0],
+                                     C.f32ptr(out)))
         return out
+
+    # -- :597-633
+    def prepareDataFHT_SRRF(self, imsRawData):
+        """FHT mode: upload, Fourier-upsample and form the gradients of a load without the
+        RGC (esrrf_prepare_fht); readGradientBuffers then returns frame 0's gradients."""
+        e = self._e()
+        a, dt = self._frames(imsRawData)
+        e.check(e.L.esrrf_prepare_fht(e.h, a.ctypes.data_as(ctypes.c_void_p), dt, a.shape[0]))
+
+    @staticmethod
+    def _frames(ims):
+        a = np.asarray(ims)
+        if a.ndim == 2:
+            a = a[None]
+        if a.dtype == np.uint16:
+            return np.ascontiguousarray(a), C.ESRRF_U16
+        return np.ascontiguousarray(a, dtype=np.float32), C.ESRRF_F32
 
     def autotune(self, reps=2):
         """Time the tiled kernel's launch shapes on the current load and keep the fastest

@@ -244,3 +244,39 @@ def test_3d_axial_magnification_plane_invariant(pkg):
         out[Mz] = np.asarray(e.readSRRFbuffer()).reshape(4, nP * Mz, h * M, w * M)
         e.release()
     np.testing.assert_array_equal(out[3][:, 1::3], out[1])
+
+
+def test_3d_reshape_matches_oracle_and_keeps_state(pkg, O):
+    """reshapeImageStack3D (LiveSRRF_CL.java:963-994) on the device: permuted ROI cells, a
+    montage whose sides are not multiples of 3, background and intensity scaling; uint16 and
+    float frames.  Calling it between loads leaves the SR result unchanged."""
+    w, h, nP, M = 14, 13, 5, 2
+    planes = planes_stack(6, w, h, nP, seed=83)
+    rng = np.random.default_rng(17)
+    rois = rng.permutation(9)[:nP].astype(np.float64)
+    mont = montage(planes, rois, W=3 * w + 2, H=3 * h + 1)
+    n, H, W = mont.shape
+    calib = (rng.uniform(-0.5, 0.5, nP), rng.uniform(-0.5, 0.5, nP), rng.uniform(-3, 3, nP),
+             rois, rng.uniform(0.8, 1.25, nP), 75.0)
+    o = O.OracleLiveSRRF(W, H, M, 1.5, 2, n, n, True, True, False,
+                         calib3d=O.Calib3D(*[np.asarray(c, np.float64) for c in calib[:5]],
+                                           float(calib[5])))
+    exp = o.reshape3D(mont.astype(np.float32))
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(W, H, M, 1.5, 2, n, n, 20000, e.allCLdevices[0], True, True, False,
+                 calib_dict(*calib), 0, 1)
+    e.resetFramePosition()
+    e.loadDriftXYGPUbuffer(np.zeros(n), np.zeros(n))
+    got = e.reshapeImageStack3D(mont)
+    assert got.shape == (n, nP, h, w)
+    np.testing.assert_allclose(got, exp, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(e.reshapeImageStack3D(mont.astype(np.float32)), exp, rtol=1e-6,
+                               atol=0)
+    e.calculateSRRF(mont[:3])
+    e.reshapeImageStack3D(mont[3:])
+    e.calculateSRRF(mont[3:])
+    res = e.readSRRFbuffer()
+    e.release()
+    ref, _, _, _, _ = run_both(pkg, O, mont, nP, M, 1.5, 2, calib, loads=[(0, 3), (3, n)])
+    np.testing.assert_array_equal(res, ref)

@@ -217,3 +217,36 @@ def test_fht_tile_order_and_prefetch_do_not_change_results(pkg, monkeypatch, var
         e.release()
     assert np.array_equal(grads["0"], grads["1"])
     assert np.array_equal(res["0"], res["1"])
+
+
+@pytest.mark.parametrize("M,w,h", [(3, 20, 17), (1, 16, 12)])
+def test_fht_prepare_data_gives_the_load_gradients(pkg, O, M, w, h):
+    """prepareDataFHT_SRRF (LiveSRRF_CL.java:597-633): upload, upsampling and gradients of a
+    load without the RGC.  The gradients equal those of calculateSRRF on the same frames,
+    the oracle's within the parity tolerance, the load counter is reset and the SR frame's
+    counter and result are untouched."""
+    from nanoj_esrrf_amd import _capi
+    st = stack(4, w, h, seed=29)
+    _, _, g_calc = run_engine(pkg, st, M, 1.5, 2)
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(w, h, M, 1.5, 2, 4, 4, 20000, e.allCLdevices[0], True, True, True, None, 0, 1)
+    e.resetFramePosition()
+    e.prepareDataFHT_SRRF(st)
+    g = e.readGradientBuffers()
+    assert e.frameCounters() == [0, 0]
+    np.testing.assert_array_equal(g, g_calc)
+    _, _, g_or = run_oracle(O, st, M, 1.5, 2)
+    scale = float(np.abs(g_or).max())
+    assert float(np.abs(g.reshape(-1) - np.asarray(g_or).reshape(-1)).max()) <= 1e-5 * scale
+    # float frames take the same path
+    e.prepareDataFHT_SRRF(st.astype(np.float32))
+    np.testing.assert_array_equal(e.readGradientBuffers(), g_calc)
+    e.release()
+    # BIC mode has no FHT data path
+    b = pkg.LiveSRRF_CL()
+    b.checkDevices()
+    b.initialise(w, h, 2, 1.5, 2, 4, 4, 20000, b.allCLdevices[0], True, True, False, None, 0, 1)
+    with pytest.raises(_capi.EsrrfError):
+        b.prepareDataFHT_SRRF(st)
+    b.release()
