@@ -73,18 +73,60 @@ static int rep_of(int cls, int xs_log2)
     return cls == 0 ? (1 << xs_log2) / 2 : (1 << (xs_log2 + cls - 1));
 }
 
+// Range [kmin, kmax] of the 2x-grid offsets kx - 2X (and ky - 2Y) of the taps that pass
+// the distance gate d <= tSO, over every class representative and sub-pixel phase, with the
+// reference's float arithmetic (the same tests as the table builders below).  The LDS tile
+// only has to reach these offsets, which at the Java-text gate lie well inside the window.
+static void gated_extent(int M, int R2lo, int R2hi, float tSO, int ncls_x, int ncls_y, int lg,
+                         int &kmin, int &kmax)
+{
+    kmin = 1 << 20;
+    kmax = -(1 << 20);
+    for (int cy = 0; cy < ncls_y; cy++)
+    for (int cx = 0; cx < ncls_x; cx++) {
+        const int Xr = rep_of(cx, lg), Yr = rep_of(cy, lg);
+        for (int sy = 0; sy < M; sy++) {
+            for (int sx = 0; sx < M; sx++) {
+                const int xM = Xr * M + sx, yM = Yr * M + sy;
+                const float xc = (float)((xM + 0.5) / M);
+                const float yc = (float)((yM + 0.5) / M);
+                const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc);
+                for (int j = R2lo; j <= R2hi; j++) {
+                    const float vy = ((float)by + (float)j) / 2.0f;
+                    for (int i = R2lo; i <= R2hi; i++) {
+                        const float vx = ((float)bx + (float)i) / 2.0f;
+                        volatile float dx = vx - xc;
+                        volatile float dy = vy - yc;
+                        volatile float d2 = dx * dx;
+                        volatile float dy2 = dy * dy;
+                        d2 = d2 + dy2;
+                        const float d = std::sqrt((float)d2);
+                        if (!(d != 0.0f && d <= tSO)) continue;
+                        const int kx = bx - 2 * Xr + i, ky = by - 2 * Yr + j;
+                        kmin = std::min(kmin, std::min(kx, ky));
+                        kmax = std::max(kmax, std::max(kx, ky));
+                    }
+                }
+            }
+        }
+    }
+    if (kmin > kmax) kmin = kmax = 0;   // no gated tap at all
+}
+
+// LDS tile of an 8x8 macro-pixel block: lanes reach 2x-grid columns 2 lx + k, lx in [0, 8),
+// k in [kmin, kmax]; G0 (even) shifts the lowest to >= 0.
+static void tile_extent(int kmin, int kmax, int &G0, int &TW)
+{
+    G0 = std::max(0, 2 * ((-kmin + 1) / 2));
+    TW = 15 + kmax + G0;
+    if (TW & 1) TW++;
+}
+
 bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO, BicGeometry &g)
 {
     g.R2lo = -(int)(2.0f * radius);
     g.R2hi = (int)(2.0f * radius + 1.0f);
     if (g.R2hi - g.R2lo > 120) return false;
-    g.G0 = 2 * ((-g.R2lo + 1) / 2);
-    int TW = 16 + g.R2hi + g.G0;
-    if (TW & 1) TW++;
-    g.TW = g.TH = TW;
-    g.PW = g.PH = TW / 2;
-    g.PP = 24;
-    while (g.PP < g.PW) g.PP += 32;
     int xs = 8;
     while (xs < 1 - g.R2lo) xs *= 2;
     int lg = 0;
@@ -92,6 +134,13 @@ bool build_bic_geometry(int w, int h, int M, float radius, float tSS, float tSO,
     g.xs_log2 = lg;
     g.ncls_x = class_of(w > 0 ? w - 1 : 0, lg) + 1;
     g.ncls_y = class_of(h > 0 ? h - 1 : 0, lg) + 1;
+    int kmin, kmax, TW;
+    gated_extent(M, g.R2lo, g.R2hi, tSO, g.ncls_x, g.ncls_y, lg, kmin, kmax);
+    tile_extent(kmin, kmax, g.G0, TW);
+    g.TW = g.TH = TW;
+    g.PW = g.PH = TW / 2;
+    g.PP = 24;
+    while (g.PP < g.PW) g.PP += 32;
     const int MM = M * M;
     const int planesz = g.PP * g.PH;
     g.taps.clear();
@@ -206,13 +255,6 @@ bool build_bic_geometry_multi(int w, int h, int M, const float *radius, const fl
     g.R2lo = -(int)(2.0f * rmax);
     g.R2hi = (int)(2.0f * rmax + 1.0f);
     if (g.R2hi - g.R2lo > 120) return false;
-    g.G0 = 2 * ((-g.R2lo + 1) / 2);
-    int TW = 16 + g.R2hi + g.G0;
-    if (TW & 1) TW++;
-    g.TW = g.TH = TW;
-    g.PW = g.PH = TW / 2;
-    g.PP = 24;
-    while (g.PP < g.PW) g.PP += 32;
     int xs = 8;
     while (xs < 1 - g.R2lo) xs *= 2;
     int lg = 0;
@@ -220,6 +262,13 @@ bool build_bic_geometry_multi(int w, int h, int M, const float *radius, const fl
     g.xs_log2 = lg;
     g.ncls_x = class_of(w > 0 ? w - 1 : 0, lg) + 1;
     g.ncls_y = class_of(h > 0 ? h - 1 : 0, lg) + 1;
+    int kmin, kmax, TW;
+    gated_extent(M, g.R2lo, g.R2hi, tmax, g.ncls_x, g.ncls_y, lg, kmin, kmax);
+    tile_extent(kmin, kmax, g.G0, TW);
+    g.TW = g.TH = TW;
+    g.PW = g.PH = TW / 2;
+    g.PP = 24;
+    while (g.PP < g.PW) g.PP += 32;
     const int MM = M * M;
     const int planesz = g.PP * g.PH;
     g.taps.clear();
