@@ -408,14 +408,6 @@ bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float 
     g.R2hi = (int)(2.0f * radius + 1.0f);
     const int klo = -(int)radius, khi = (int)radius + 1;
     if (M < 1 || nP < 2 || g.R2hi - g.R2lo > 60 || khi - klo > 40) return false;
-    g.G0 = 2 * ((-g.R2lo + 1) / 2);
-    // lane columns 2 lx + kx_rel + G0 with kx_rel in [R2lo, R2hi + 1], lx in [0, 8)
-    g.TW = g.TH = 16 + g.R2hi + g.G0;
-    g.PW = (g.TW + 1) / 2;
-    // row pitch = 4 (mod 8) float4s: rows 2 ly apart land in opposite 128-byte bank halves
-    g.P = 2 * g.PW;
-    while (g.P % 8 != 4) g.P++;
-    g.PZ = g.TH * g.P;
     int xs = 8;
     while (xs < 1 - g.R2lo) xs *= 2;
     int lg = 0;
@@ -426,6 +418,57 @@ bool build_3d_geometry(int w, int h, int nP, int M, int Mz, float radius, float 
     g.ncls_y = pow2 ? 1 : class_of(h > 0 ? h - 1 : 0, lg) + 1;
     g.nZM = nP * Mz;
     const int MM = M * M;
+    // lane columns 2 lx + kx_rel + G0, lx in [0, 8), kx_rel over the lateral offsets the gated
+    // 3-D taps reach (every class, z-slice and phase, the reference's float arithmetic)
+    {
+        int kmin = 1 << 20, kmax = -(1 << 20);
+        for (int cy = 0; cy < g.ncls_y; cy++)
+        for (int cx = 0; cx < g.ncls_x; cx++) {
+            const int Xr = pow2 ? 8 : rep_of(cx, lg), Yr = pow2 ? 8 : rep_of(cy, lg);
+            for (int zM = 0; zM < g.nZM; zM++) {
+                const float zc = (float)((zM + 0.5) / Mz);
+                const int zi = (int)zc;
+                for (int sy = 0; sy < M; sy++)
+                for (int sx = 0; sx < M; sx++) {
+                    const float xc = (float)((Xr * M + sx + 0.5) / M);
+                    const float yc = (float)((Yr * M + sy + 0.5) / M);
+                    const int bx = (int)(2.0f * xc), by = (int)(2.0f * yc);
+                    for (int j = g.R2lo; j <= g.R2hi; j++) {
+                        const float vy = ((float)by + (float)j) / 2.0f;
+                        for (int i = g.R2lo; i <= g.R2hi; i++) {
+                            const float vx = ((float)bx + (float)i) / 2.0f;
+                            for (int k = klo; k <= khi; k++) {
+                                const float vz = (float)zi + (float)k;
+                                if (!(vz > 0.0f && vz < (float)nP)) continue;
+                                volatile float dx = vx - xc;
+                                volatile float dy = vy - yc;
+                                volatile float dz = vz - zc;
+                                volatile float d2 = dx * dx;
+                                volatile float t2 = dy * dy;
+                                d2 = d2 + t2;
+                                t2 = dz * dz;
+                                d2 = d2 + t2;
+                                const float d = std::sqrt((float)d2);
+                                if (!(d != 0.0f && d <= tSO)) continue;
+                                const int kx = bx - 2 * Xr + i, ky = by - 2 * Yr + j;
+                                kmin = std::min(kmin, std::min(kx, ky));
+                                kmax = std::max(kmax, std::max(kx, ky));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (kmin > kmax) kmin = kmax = 0;
+        int TW;
+        tile_extent(kmin, kmax, g.G0, TW);
+        g.TW = g.TH = TW;
+    }
+    g.PW = (g.TW + 1) / 2;
+    // row pitch = 4 (mod 8) float4s: rows 2 ly apart land in opposite 128-byte bank halves
+    g.P = 2 * g.PW;
+    while (g.P % 8 != 4) g.P++;
+    g.PZ = g.TH * g.P;
 
     // planes each z-slice can reach (|dz| <= tSO, 0 < vz < nP)
     std::vector<int> zlo(g.nZM), nz(g.nZM);
