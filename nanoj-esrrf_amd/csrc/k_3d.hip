@@ -15,6 +15,7 @@
 //   k3d_rgc       3-D radial gradient convergence + axial cubic + temporal cumulants
 //                 (live3DSRRF.cl:325-498), cumulants in registers across frames.  Frames
 //                 without drift share one geometry, evaluated once per chunk of FC frames.
+#include <algorithm>
 #include <cstdlib>
 
 #include "device_interp.h"
@@ -103,28 +104,120 @@ __device__ __forceinline__ float4 unit3_or_zero(float tx, float ty, float tz)
     return make_float4(sx * r, sy * r, sz * r, 0.0f);
 }
 
-// U: [nF*nP][2h][2w]; the last column and row of the 2x grid are never sampled (gates).
-__global__ void k3d_unit(const float *__restrict__ g, float4 *__restrict__ U, int w, int h,
-                         long npl)
+// GxInt-style value of plane `a` at the 2x-grid point (X/2, Y/2) (getInterpolatedValue,
+// live3DSRRF.cl:30-140).  At integer and half-integer arguments the Catmull-Rom weights are
+// exactly (0, 1, 0, 0) and (-1, 9, 9, -1)/16 (cubic(0) = 1, cubic(1) = cubic(2) = 0,
+// cubic(1/2) = 9/16, cubic(3/2) = -1/16), so they are taken by parity instead of evaluated;
+// the products are summed in the same order, rows along x first (interp_global).
+__device__ __forceinline__ float half_grid_value(const float *__restrict__ a, int w, int h,
+                                                 int X, int Y)
 {
-    const int wI = 2 * w, hI = 2 * h;
-    const long whI = (long)wI * hI, wh = (long)w * h;
-    const long n = npl * whI;
-    const long gn = npl * wh;
-    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
-         o += (long)gridDim.x * blockDim.x) {
-        const long pl = o / whI;
-        const long r = o - pl * whI;
-        const int m = (int)(r / wI), nn = (int)(r - (long)m * wI);
-        float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (nn <= wI - 2 && m <= hI - 2) {
-            const float *Gx = g + wh * pl, *Gy = g + gn + wh * pl, *Gz = g + 2 * gn + wh * pl;
-            const float tx = interp_global(Gx, w, h, (float)(nn + 1) / 2.0f, (float)m / 2.0f);
-            const float ty = interp_global(Gy, w, h, (float)nn / 2.0f, (float)(m + 1) / 2.0f);
-            const float tz = interp_global(Gz, w, h, (float)nn / 2.0f, (float)m / 2.0f);
-            u = unit3_or_zero(tx, ty, tz);
+    const int u0 = X >> 1, v0 = Y >> 1;
+    if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+        const float c0 = -0.0625f, c1 = 0.5625f;
+        const float wx0 = (X & 1) ? c0 : 0.0f, wx1 = (X & 1) ? c1 : 1.0f;
+        const float wx2 = (X & 1) ? c1 : 0.0f, wx3 = (X & 1) ? c0 : 0.0f;
+        const float wy[4] = {(Y & 1) ? c0 : 0.0f, (Y & 1) ? c1 : 1.0f, (Y & 1) ? c1 : 0.0f,
+                             (Y & 1) ? c0 : 0.0f};
+        const float *r = a + (v0 - 1) * w + (u0 - 1);
+        float q = 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            float p = 0.0f;
+            p = fmaf(r[0], wx0, p);
+            p = fmaf(r[1], wx1, p);
+            p = fmaf(r[2], wx2, p);
+            p = fmaf(r[3], wx3, p);
+            q = fmaf(p, wy[jj], q);
+            r += w;
         }
-        U[o] = u;
+        return q;
+    }
+    return interp_global(a, w, h, (float)X / 2.0f, (float)Y / 2.0f);
+}
+
+// U: [nF*nP][2h][2w]; the last column and row of the 2x grid are never sampled (gates).
+// Block: 32 x 32 points of the 2x grid of one plane (planes over grid.z).  The three
+// gradient tiles the block reaches (image columns and rows kx0/2 - 1 .. kx0/2 + 18) are
+// staged in LDS.  A 4-tap sum along x of a gradient row depends only on the row and the
+// point's x, so the block forms each (row, x) sum once and every point combines four of them
+// along y, in half_grid_value's order; border points take its extrapolation branch.
+constexpr int U3_T = 32;   // points per block side
+constexpr int U3_G = 20;   // gradient tile side
+
+__global__ void __launch_bounds__(256) k3d_unit(const float *__restrict__ g,
+                                                float4 *__restrict__ U, int w, int h, long npl)
+{
+    __shared__ float lg[3][U3_G * U3_G];
+    __shared__ float rs[3][U3_G * U3_T];   // row sums [plane c][tile row][point x]
+    const int wI = 2 * w, hI = 2 * h;
+    const long wh = (long)w * h, gn = npl * wh, whI = (long)wI * hI;
+    const int kx0 = blockIdx.x * U3_T, ky0 = blockIdx.y * U3_T;
+    const int gx0 = (kx0 >> 1) - 1, gy0 = (ky0 >> 1) - 1;
+    const int tid = threadIdx.x;
+    const float c0 = -0.0625f, c1 = 0.5625f;
+    for (long pl = blockIdx.z; pl < npl; pl += gridDim.z) {
+        const float *G[3] = {g + wh * pl, g + gn + wh * pl, g + 2 * gn + wh * pl};
+        __syncthreads();   // the previous plane's tiles are no longer read
+        for (int e = tid; e < 3 * U3_G * U3_G; e += 256) {
+            const int c = e / (U3_G * U3_G), rr = e - c * U3_G * U3_G;
+            const int r = rr / U3_G, q = rr - r * U3_G;
+            const int x = gx0 + q, y = gy0 + r;
+            lg[c][rr] = (x >= 0 && x < w && y >= 0 && y < h)
+                            ? g[c * gn + wh * pl + (long)y * w + x] : 0.0f;
+        }
+        __syncthreads();
+        // row sums along x: Gx at X = kx + 1, Gy and Gz at X = kx
+        for (int e = tid; e < 3 * U3_G * U3_T; e += 256) {
+            const int c = e / (U3_G * U3_T), rr = e - c * U3_G * U3_T;
+            const int r = rr / U3_T, q = rr - r * U3_T;
+            const int X = kx0 + q + (c == 0 ? 1 : 0), u0 = X >> 1;
+            const float wx0 = (X & 1) ? c0 : 0.0f, wx1 = (X & 1) ? c1 : 1.0f;
+            const float wx2 = (X & 1) ? c1 : 0.0f, wx3 = (X & 1) ? c0 : 0.0f;
+            const int col = u0 - 1 - gx0;
+            float pp = 0.0f;
+            if (col >= 0 && col + 3 < U3_G) {
+                const float *t = &lg[c][r * U3_G + col];
+                pp = fmaf(t[0], wx0, pp);
+                pp = fmaf(t[1], wx1, pp);
+                pp = fmaf(t[2], wx2, pp);
+                pp = fmaf(t[3], wx3, pp);
+            }
+            rs[c][rr] = pp;
+        }
+        __syncthreads();
+        float4 *Up = U + pl * whI;
+#pragma unroll 1
+        for (int o = 0; o < (U3_T * U3_T) / 256; o++) {
+            const int e = tid + o * 256;
+            const int q = e & (U3_T - 1), yy = e / U3_T;
+            const int nn = kx0 + q, m = ky0 + yy;
+            if (nn >= wI || m >= hI) continue;
+            float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (nn <= wI - 2 && m <= hI - 2) {
+                float t3[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int X = nn + (c == 0 ? 1 : 0), Y = m + (c == 1 ? 1 : 0);
+                    const int u0 = X >> 1, v0 = Y >> 1;
+                    if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+                        const float wy0 = (Y & 1) ? c0 : 0.0f, wy1 = (Y & 1) ? c1 : 1.0f;
+                        const float wy2 = (Y & 1) ? c1 : 0.0f, wy3 = (Y & 1) ? c0 : 0.0f;
+                        const float *cs = &rs[c][(v0 - 1 - gy0) * U3_T + q];
+                        float qq = 0.0f;
+                        qq = fmaf(cs[0], wy0, qq);
+                        qq = fmaf(cs[U3_T], wy1, qq);
+                        qq = fmaf(cs[2 * U3_T], wy2, qq);
+                        qq = fmaf(cs[3 * U3_T], wy3, qq);
+                        t3[c] = qq;
+                    } else {
+                        t3[c] = interp_global(G[c], w, h, (float)X / 2.0f, (float)Y / 2.0f);
+                    }
+                }
+                u = unit3_or_zero(t3[0], t3[1], t3[2]);
+            }
+            Up[(long)m * wI + nn] = u;
+        }
     }
 }
 
@@ -145,24 +238,72 @@ __global__ void k3d_gint(const float *__restrict__ g, float *__restrict__ out, i
     }
 }
 
-// L: [nframes][nP][hM][wM]
-__global__ void k3d_lateral(const float *__restrict__ px, float *__restrict__ L, int w, int h,
-                            int nP, int M, int frame0, int nframes, const float *__restrict__ drift,
-                            int count0, int nFor)
+// L: [nframes][nP][hM][wM].  Block: 64 x 16 magnified pixels of one (frame, plane)
+// (frame-planes over grid.z).  The raw pixels the block reaches are staged in LDS: sample
+// positions x = xM/M + dX - 0.5 increase with xM, so the columns floor(x(x0)) - 1 ..
+// floor(x(x0 + 63)) + 2 (at most 68 for M >= 1) and the rows likewise (at most 20) cover
+// every bicubic stencil of the block.  A thread keeps its column's Catmull-Rom weights for
+// its four rows; weights, products and sums are interp_global's (same arguments, same order).
+constexpr int LAT_X = 64, LAT_Y = 16, LAT_TW = LAT_X + 4, LAT_TH = LAT_Y + 4;
+
+__global__ void __launch_bounds__(256)
+k3d_lateral(const float *__restrict__ px, float *__restrict__ L, int w, int h, int nP, int M,
+            int frame0, int nframes, const float *__restrict__ drift, int count0, int nFor)
 {
+    __shared__ float tile[LAT_TH * LAT_TW];
     const int wM = w * M, hM = h * M;
     const long whM = (long)wM * hM, wh = (long)w * h;
-    const long n = (long)nframes * nP * whM;
-    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n;
-         o += (long)gridDim.x * blockDim.x) {
-        const int fi = (int)(o / (nP * whM));
-        const int z = (int)((o - (long)fi * nP * whM) / whM);
-        const long r = o - (long)fi * nP * whM - (long)z * whM;
-        const int yM = (int)(r / wM), xM = (int)(r - (long)yM * wM);
+    const int x0 = blockIdx.x * LAT_X, y0 = blockIdx.y * LAT_Y;
+    const int tid = threadIdx.x, tx = tid & (LAT_X - 1), rg = tid >> 6;
+    const int xM = x0 + tx;
+    const int nfz = nframes * nP;
+#pragma unroll 1
+    for (int fz = blockIdx.z; fz < nfz; fz += gridDim.z) {
+        const int fi = fz / nP, z = fz - fi * nP;
         const float dX = drift[count0 + fi], dY = drift[count0 + fi + nFor];
+        const float *P = px + wh * ((long)(frame0 + fi) * nP + z);
+        const float xs0 = __fsub_rn(__fadd_rn((float)x0 / (float)M, dX), 0.5f);
+        const float ys0 = __fsub_rn(__fadd_rn((float)y0 / (float)M, dY), 0.5f);
+        const int ux0 = (int)floorf(xs0) - 1, vy0 = (int)floorf(ys0) - 1;
+        __syncthreads();   // the previous frame-plane's tile is no longer read
+        for (int e = tid; e < LAT_TH * LAT_TW; e += 256) {
+            const int r = e / LAT_TW, c = e - r * LAT_TW;
+            const int x = ux0 + c, y = vy0 + r;
+            tile[e] = (x >= 0 && x < w && y >= 0 && y < h) ? P[(long)y * w + x] : 0.0f;
+        }
+        __syncthreads();
+        if (xM >= wM) continue;
         const float xi = __fsub_rn(__fadd_rn((float)xM / (float)M, dX), 0.5f);
-        const float yi = __fsub_rn(__fadd_rn((float)yM / (float)M, dY), 0.5f);
-        L[o] = interp_global(px + wh * ((long)(frame0 + fi) * nP + z), w, h, xi, yi);
+        const int u0 = (int)floorf(xi);
+        const bool inx = u0 > 0 && u0 < w - 2;
+        float wx[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) wx[i] = cubic_w(xi - (float)(u0 - 1 + i));
+        float *Lo = L + (long)fz * whM + xM;
+#pragma unroll 1
+        for (int k = 0; k < LAT_Y / 4; k++) {
+            const int yM = y0 + rg + 4 * k;
+            if (yM >= hM) break;
+            const float yi = __fsub_rn(__fadd_rn((float)yM / (float)M, dY), 0.5f);
+            const int v0 = (int)floorf(yi);
+            float q = 0.0f;
+            if (inx && v0 > 0 && v0 < h - 2) {
+                const float *t = &tile[(v0 - 1 - vy0) * LAT_TW + (u0 - 1 - ux0)];
+#pragma unroll
+                for (int j = 0; j <= 3; j++) {
+                    float p = 0.0f;
+                    p = fmaf(t[0], wx[0], p);
+                    p = fmaf(t[1], wx[1], p);
+                    p = fmaf(t[2], wx[2], p);
+                    p = fmaf(t[3], wx[3], p);
+                    q = fmaf(p, cubic_w(yi - (float)(v0 - 1 + j)), q);
+                    t += LAT_TW;
+                }
+            } else {
+                q = interp_global(P, w, h, xi, yi);
+            }
+            Lo[(long)yM * wM] = q;
+        }
     }
 }
 
@@ -1033,8 +1174,10 @@ hipError_t launch3d_prepare(const void *src, int dtype, const Prep3dArgs &p, int
                        p.h, p.nP, p.st, npx);
     hipLaunchKernelGGL(k3d_grad, dim3(grid1(npx)), dim3(256), 0, s, p.aligned, p.g, p.w, p.h,
                        p.nP, npx);
-    hipLaunchKernelGGL(k3d_unit, dim3(grid1(4 * npx)), dim3(256), 0, s, p.g, p.U, p.w, p.h,
-                       (long)n * p.nP);
+    const long npl = (long)n * p.nP;
+    hipLaunchKernelGGL(k3d_unit, dim3((2 * p.w + U3_T - 1) / U3_T, (2 * p.h + U3_T - 1) / U3_T,
+                                      (unsigned)std::min(npl, 65535L)),
+                       dim3(256), 0, s, p.g, p.U, p.w, p.h, npl);
     return hipGetLastError();
 }
 
@@ -1059,9 +1202,11 @@ hipError_t launch3d_gint(const float *g, float *out, int w, int h, int nP, long 
 hipError_t launch3d_lateral(const Rgc3dArgs &a, hipStream_t s)
 {
     if (a.nframes <= 0) return hipSuccess;
-    const long nl = (long)a.nframes * a.nP * a.wM * a.hM;
-    hipLaunchKernelGGL(k3d_lateral, dim3(grid1(nl)), dim3(256), 0, s, a.px, a.L, a.w, a.h, a.nP,
-                       a.M, a.frame0, a.nframes, a.drift, a.count0, a.nFor);
+    const int nfz = a.nframes * a.nP;
+    hipLaunchKernelGGL(k3d_lateral, dim3((a.wM + LAT_X - 1) / LAT_X, (a.hM + LAT_Y - 1) / LAT_Y,
+                                         std::min(nfz, 65535)),
+                       dim3(256), 0, s, a.px, a.L, a.w, a.h, a.nP, a.M, a.frame0, a.nframes,
+                       a.drift, a.count0, a.nFor);
     return hipGetLastError();
 }
 
