@@ -327,22 +327,55 @@ __device__ __forceinline__ float2 unit_or_zero_f(float tx, float ty)
     return make_float2(sx * r, sy * r);
 }
 
-// grid (H * ceil(W / 256), frames): a block takes 256 consecutive pixels of one row, so the
-// pixel's coordinates come from one division per block rather than a 64-bit one per pixel
+// Block: 64 x 16 magnified pixels of one frame (frames over grid.z).  A pixel's unit vector
+// folds the two clamped gradient reads of liveSRRF_noInterpolation.cl:291-292,
+// Gx[m][min(n+1, W-1)] and Gy[min(m+1, H-1)][n], each a Roberts cross of the 2 x 2 pixels
+// ending at its point with edge-clamped neighbours; together they reach columns n-1 .. n+1
+// and rows m-1 .. m+1.  The block stages those pixels (edge-clamped, the tile origin at
+// (X0 - 1, Y0 - 1)) in LDS and evaluates robx on the tile with robx's own coordinates.
+constexpr int FP_X = 64, FP_Y = 16, FP_TW = FP_X + 2, FP_TH = FP_Y + 2;
+
 __global__ void __launch_bounds__(256) k_fht_prep(const float *__restrict__ px,
-                                                  float2 *__restrict__ U, int W, int H)
+                                                  float2 *__restrict__ U, int W, int H, int nF)
 {
-    const int f = blockIdx.y;
-    const int cw = (W + 255) >> 8;
-    const int m = blockIdx.x / cw;
-    const int n = (blockIdx.x - m * cw) * 256 + threadIdx.x;
-    if (n >= W) return;
+    __shared__ float tile[FP_TH * FP_TW];
+    const int X0 = blockIdx.x * FP_X, Y0 = blockIdx.y * FP_Y;
+    const int tid = threadIdx.x, tx = tid & (FP_X - 1), ty = tid >> 6;
     const long wh = (long)W * H;
-    const float *P = px + wh * f;
-    float gx, gy, dummy;
-    robx(P, W, H, min(n + 1, W - 1), m, gx, dummy);
-    robx(P, W, H, n, min(m + 1, H - 1), dummy, gy);
-    U[wh * f + (long)m * W + n] = unit_or_zero_f(gx, gy);
+    // pixel (x, y) of the image at tile[(y - Y0 + 1)][(x - X0 + 1)]
+    auto at = [&](int x, int y) { return tile[(y - Y0 + 1) * FP_TW + (x - X0 + 1)]; };
+#pragma unroll 1
+    for (int f = blockIdx.z; f < nF; f += gridDim.z) {
+        const float *P = px + wh * f;
+        __syncthreads();   // the previous frame's tile is no longer read
+        for (int e = tid; e < FP_TH * FP_TW; e += 256) {
+            const int r = e / FP_TW, c = e - r * FP_TW;
+            const int x = min(max(X0 - 1 + c, 0), W - 1), y = min(max(Y0 - 1 + r, 0), H - 1);
+            tile[e] = P[(long)y * W + x];
+        }
+        __syncthreads();
+        const int n = X0 + tx;
+        if (n >= W) continue;
+#pragma unroll
+        for (int k = 0; k < FP_Y / 4; k++) {
+            const int m = Y0 + ty + 4 * k;
+            if (m >= H) break;
+            float gx, gy;
+            {   // robx at (min(n+1, W-1), m), its x component
+                const int x1 = min(n + 1, W - 1), y1 = m;
+                const int x0 = x1 - 1 < 0 ? 0 : x1 - 1, y0 = y1 - 1 < 0 ? 0 : y1 - 1;
+                const float p01 = at(x1, y0), p10 = at(x0, y1), p11 = at(x1, y1), p00 = at(x0, y0);
+                gx = ((p01 - p10) + p11) - p00;
+            }
+            {   // robx at (n, min(m+1, H-1)), its y component
+                const int x1 = n, y1 = min(m + 1, H - 1);
+                const int x0 = x1 - 1 < 0 ? 0 : x1 - 1, y0 = y1 - 1 < 0 ? 0 : y1 - 1;
+                const float p01 = at(x1, y0), p10 = at(x0, y1), p11 = at(x1, y1), p00 = at(x0, y0);
+                gy = ((-p01 + p10) + p11) - p00;
+            }
+            U[wh * f + (long)m * W + n] = unit_or_zero_f(gx, gy);
+        }
+    }
 }
 
 __global__ void k_fht_grad(const float *__restrict__ px, float *__restrict__ gx,
@@ -361,8 +394,10 @@ __global__ void k_fht_grad(const float *__restrict__ px, float *__restrict__ gx,
 
 hipError_t launch_fht_prep(const float *px, float2 *U, int W, int H, int nF, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fht_prep, dim3((unsigned)H * ((W + 255) / 256), nF), dim3(256), 0, s,
-                       px, U, W, H);
+    if (nF <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fht_prep, dim3((W + FP_X - 1) / FP_X, (H + FP_Y - 1) / FP_Y,
+                                        std::min(nF, 65535)),
+                       dim3(256), 0, s, px, U, W, H, nF);
     return hipGetLastError();
 }
 
