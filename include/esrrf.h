@@ -126,10 +126,11 @@ int esrrf_finish(esrrf_engine *e);
 /* Run on the caller's HIP stream (NULL restores the engine's own stream). */
 int esrrf_set_stream(esrrf_engine *e, void *hip_stream);
 /* esrrf_calculate (BIC mode, no cancel flag) splits a load of host frames into up to
- * `chunks` (1..16) parts of at least 64 frames and copies part c+1 while part c is
- * computed.  No counterpart: the reference queues one write of the whole load ahead of
- * the kernels on its in-order queue (LiveSRRF_CL.java:655-661).  1 = one copy, then the
- * kernels.  Default 4. */
+ * `chunks` (1..16, at most one per 64 frames) parts, each three times the one before (the
+ * first at least 16 frames), and copies part c+1 while part c is computed, so that only the
+ * first, small copy precedes the kernels.  No counterpart: the reference queues one write
+ * of the whole load ahead of the kernels on its in-order queue (LiveSRRF_CL.java:655-661).
+ * 1 = one copy, then the kernels.  Default 4. */
 int esrrf_set_upload_chunks(esrrf_engine *e, int chunks);
 /* Page-locked host memory from the runtime the engine copies with, for frame and result
  * buffers that upload and read back at the link's full rate (no counterpart: the reference

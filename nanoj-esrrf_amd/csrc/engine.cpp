@@ -252,6 +252,29 @@ int run_bic_rgc(esrrf_engine *e, int f0, int c0, int n)
 
 // Smallest upload chunk worth its own RGC launch (frames).
 constexpr int kMinUploadChunk = 64;
+// Chunk sizes (even except the last, summing to n) of a load of n frames uploaded in at
+// most nch chunks.  The kernels start once the first chunk is resident, so it is small; each
+// chunk after it is three times the one before, which keeps its copy (about 2.6 us per 256^2
+// uint16 frame over PCIe) shorter than the kernels of the previous chunk (about 10 us per
+// frame at M = 5), so only the first copy is exposed.  The first chunk is the smallest that
+// lets nch chunks cover the load, at least 16 frames.
+std::vector<int> upload_schedule(int n, int nch)
+{
+    double tot = 0.0, g = 1.0;
+    for (int k = 0; k < nch; k++, g *= 3.0) tot += g;
+    long s = std::max(16L, (long)(n / tot));
+    s += s & 1;
+    std::vector<int> sz;
+    int rem = n;
+    for (int k = 0; rem > 0; k++) {
+        int c = rem;
+        if (k < nch - 1 && s < rem) c = (int)s;
+        sz.push_back(c);
+        rem -= c;
+        s = std::min(3 * s, (long)n);
+    }
+    return sz;
+}
 
 // calculateSRRF for host frames in BIC mode with the upload overlapped: chunk c of the load
 // is copied on copy_stream while the compute stream converts, preps and accumulates chunk
@@ -264,16 +287,16 @@ int calculate_bic_pipelined(esrrf_engine *e, const void *frames, int dtype, int 
     const size_t esz = dtype == ESRRF_U16 ? 2 : 4;
     const long npx = (long)e->W * e->H;
     const int nch = std::min(e->upload_chunks, n / kMinUploadChunk);
-    int csz = (n + nch - 1) / nch;
-    csz += csz & 1;   // even chunks keep the frame-pair kernel on pairs
+    // even chunks keep the frame-pair kernel on pairs
+    const std::vector<int> sched = upload_schedule(n, nch);
     // the staging buffer may still be read by work queued earlier on the compute stream
     HIPCHK(e, hipEventRecord(e->ev_free, e->stream));
     HIPCHK(e, hipStreamWaitEvent(e->copy_stream, e->ev_free, 0));
     e->counters[1] = 0;
     e->loaded = n;
-    int c = 0;
-    for (int f0 = 0; f0 < n; f0 += csz, c++) {
-        const int k = std::min(csz, n - f0);
+    int f0 = 0;
+    for (int c = 0; c < (int)sched.size(); f0 += sched[c], c++) {
+        const int k = sched[c];
         const char *hsrc = (const char *)frames + esz * npx * f0;
         char *dst = (char *)e->d_in + esz * npx * f0;
         HIPCHK(e, hipMemcpyAsync(dst, hsrc, esz * npx * k, hipMemcpyHostToDevice,

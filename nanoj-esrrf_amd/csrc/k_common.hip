@@ -6,6 +6,8 @@
 //   NaN -> 0 on read-back        LiveSRRF_CL.java:789
 //
 // All of these are HBM-bound streaming kernels over the 4 output channels.
+#include <algorithm>
+
 #include "esrrf_internal.h"
 
 namespace esrrf {
@@ -70,10 +72,15 @@ hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch, long ost
 // every macro-pixel (M x M) or macro-voxel (M x M x Mz).  An image is nP planes of
 // (w*M) x (h*M) pixels magnified to nP*Mz planes; a 2-D image is nP = Mz = 1.  The reference
 // sums sequentially in float; this tree-reduces in double, which is at least as accurate.
+// The element index i = (z, y, x) of the entry's macro-pixels is split with 32-bit
+// arithmetic (w * h * nP < 2^31).
 // blockIdx.y: image of a batch (images `ostride` floats apart, maps `mstride` floats apart).
-__global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
-                                                float *__restrict__ mp, int w, int h, int nP,
-                                                int M, int Mz, long ostride, long mstride)
+constexpr int MP_THREADS = 1024;
+
+__global__ void __launch_bounds__(MP_THREADS) k_mp_map(const float *__restrict__ out,
+                                                       float *__restrict__ mp, int w, int h,
+                                                       int nP, int M, int Mz, long ostride,
+                                                       long mstride)
 {
     out += ostride * blockIdx.y;
     mp += mstride * blockIdx.y;
@@ -88,66 +95,66 @@ __global__ void __launch_bounds__(256) k_mp_map(const float *__restrict__ out,
     const long wM = (long)w * M;
     const long whM = wM * h * M;
     const long whdM = whM * nP * Mz;
-    const long count = (long)w * h * nP;
+    const int wh = w * h;
+    const int count = wh * nP;
+    const float *base = out + xMP + wM * yMP + whM * zMP + frame * whdM;
     double acc = 0.0;
-    for (long i = threadIdx.x; i < count; i += blockDim.x) {
-        long z = i / ((long)w * h);
-        long rem = i - z * (long)w * h;
-        long y = rem / w;
-        long x = rem - y * w;
-        long off = xMP + x * M + wM * (yMP + y * M) + whM * (zMP + z * Mz) + frame * whdM;
-        acc += (double)out[off];
+    for (int i = threadIdx.x; i < count; i += MP_THREADS) {
+        const int z = i / wh;
+        const int rem = i - z * wh;
+        const int y = rem / w;
+        const int x = rem - y * w;
+        acc += (double)base[(long)x * M + wM * ((long)y * M) + whM * ((long)z * Mz)];
     }
-    __shared__ double red[256];
+    __shared__ double red[MP_THREADS];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    for (int st = MP_THREADS / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
     if (threadIdx.x == 0) mp[o] = (float)red[0] / (float)count;
 }
 
-__global__ void k_mp_correct(float *__restrict__ out, const float *__restrict__ mp, int w,
-                             int h, int nP, int M, int Mz, long ostride, long mstride)
-{
-    out += ostride * blockIdx.y;
-    mp += mstride * blockIdx.y;
-    const long wM = (long)w * M, hM = (long)h * M;
-    const long whM = wM * hM;
-    const long whdM = whM * nP * Mz;
-    const long n = 3 * whdM;
-    long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long stride = (long)gridDim.x * blockDim.x;
-    const int MM = M * M;
-    for (; o < n; o += stride) {
-        long frame = o / whdM;
-        long rem = o - frame * whdM;
-        long zM = rem / whM;
-        rem -= zM * whM;
-        long yM = rem / wM;
-        long xM = rem - yM * wM;
-        int sx = (int)(xM % M), sy = (int)(yM % M), sz = (int)(zM % Mz);
-        long om = (long)MM * Mz * frame + MM * sz + M * sy + sx;
-        out[o] = out[o] / mp[om];
-    }
-}
-
 hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int Mz,
                          hipStream_t s, int nbatch, long ostride, long mstride)
 {
     int nentries = 3 * M * M * Mz;
-    hipLaunchKernelGGL(k_mp_map, dim3(nentries, nbatch), dim3(256), 0, s, out, mp, w, h, nP, M,
-                       Mz, ostride, mstride);
+    hipLaunchKernelGGL(k_mp_map, dim3(nentries, nbatch), dim3(MP_THREADS), 0, s, out, mp, w, h,
+                       nP, M, Mz, ostride, mstride);
     return hipGetLastError();
+}
+
+// Channels AVG, VAR, TAC2 divided by the MP-map entry of their sub-position (in place).
+// Block: 256 pixels of a row; rows (channel, magnified plane, y) over grid.y, images of a
+// batch over grid.z (32-bit row arithmetic: 3 * nP * Mz * h * M < 2^31).
+__global__ void __launch_bounds__(256) k_mp_correct(float *__restrict__ out,
+                                                    const float *__restrict__ mp, int w, int h,
+                                                    int nP, int M, int Mz, long ostride,
+                                                    long mstride)
+{
+    out += ostride * blockIdx.z;
+    mp += mstride * blockIdx.z;
+    const int wM = w * M, hM = h * M, nZ = nP * Mz, MM = M * M;
+    const int nrows = 3 * nZ * hM;
+    const int xM = blockIdx.x * blockDim.x + threadIdx.x;
+    if (xM >= wM) return;
+    const int sx = xM % M;
+    for (int r = blockIdx.y; r < nrows; r += gridDim.y) {
+        const int cz = r / hM, yM = r - cz * hM;     // cz = channel * nZ + zM
+        const int ch = cz / nZ, zM = cz - ch * nZ;
+        const float d = mp[MM * Mz * ch + MM * (zM % Mz) + M * (yM % M) + sx];
+        const long off = (long)r * wM + xM;
+        out[off] = out[off] / d;
+    }
 }
 
 hipError_t launch_mp_correct(float *out, const float *mp, int w, int h, int nP, int M, int Mz,
                              hipStream_t s, int nbatch, long ostride, long mstride)
 {
-    long n = 3L * w * M * h * M * nP * Mz;
-    hipLaunchKernelGGL(k_mp_correct, dim3(grid_for(n, 256), nbatch), dim3(256), 0, s, out, mp,
-                       w, h, nP, M, Mz, ostride, mstride);
+    const int nrows = 3 * nP * Mz * h * M;
+    hipLaunchKernelGGL(k_mp_correct, dim3((w * M + 255) / 256, std::min(nrows, 65535), nbatch),
+                       dim3(256), 0, s, out, mp, w, h, nP, M, Mz, ostride, mstride);
     return hipGetLastError();
 }
 
