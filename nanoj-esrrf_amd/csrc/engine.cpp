@@ -53,6 +53,7 @@ struct esrrf_engine {
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_free = nullptr;                 // staging buffer free to overwrite
     std::vector<hipEvent_t> ev_chunk;             // chunk c resident
+    hipEvent_t ev_int = nullptr;                  // read-back: INT channel ready to copy
     int upload_chunks = 4;
     Mode mode = MODE_BIC;
     esrrf_params p{};
@@ -143,6 +144,7 @@ void free_all(esrrf_engine *e)
     for (hipEvent_t ev : e->ev_chunk)
         if (ev) (void)hipEventDestroy(ev);
     if (e->ev_free) (void)hipEventDestroy(e->ev_free);
+    if (e->ev_int) (void)hipEventDestroy(e->ev_int);
     if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
 }
@@ -407,7 +409,8 @@ int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtyp
     return ESRRF_OK;
 }
 
-int finalize_impl(esrrf_engine *e, float *dst_dev)
+// VAR, then the MP map and correction (channels 0..2; LiveSRRF_CL.java:757-770)
+int epilogue_impl(esrrf_engine *e)
 {
     HIPCHK(e, launch_var(e->d_out, e->nvox, e->stream));
     if (e->p.do_mp_map_correction && e->mode != MODE_FHT) {
@@ -415,6 +418,13 @@ int finalize_impl(esrrf_engine *e, float *dst_dev)
         HIPCHK(e, launch_mp_correct(e->d_out, e->d_mp, e->w, e->h, e->nP, e->M, e->Mz, e->stream));
         e->has_mp = true;
     }
+    return ESRRF_OK;
+}
+
+int finalize_impl(esrrf_engine *e, float *dst_dev)
+{
+    const int rc = epilogue_impl(e);
+    if (rc) return rc;
     HIPCHK(e, launch_nan_to_zero_copy(e->d_out, dst_dev, 4 * e->nvox, e->stream));
     return ESRRF_OK;
 }
@@ -487,7 +497,8 @@ int esrrf_create(const esrrf_params *p, esrrf_engine **out)
     if (hipStreamCreate(&e->own_stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
     e->stream = e->own_stream;
     if (hipStreamCreate(&e->copy_stream) != hipSuccess) return bad(ESRRF_ERR_HIP);
-    if (hipEventCreateWithFlags(&e->ev_free, hipEventDisableTiming) != hipSuccess)
+    if (hipEventCreateWithFlags(&e->ev_free, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_int, hipEventDisableTiming) != hipSuccess)
         return bad(ESRRF_ERR_HIP);
     e->ev_chunk.assign(kMaxUploadChunks, nullptr);
     for (hipEvent_t &ev : e->ev_chunk)
@@ -644,11 +655,31 @@ int esrrf_read_result(esrrf_engine *e, float *out)
     if (!e || !out) return ESRRF_ERR_INVALID;
     std::lock_guard<std::mutex> lk(e->mu);
     (void)hipSetDevice(e->dev);
-    int rc = finalize_impl(e, e->d_tmp);
+    // INT (channel 3) is final once the RGC launches are done: VAR and the MP correction
+    // change channels 0..2 only (LiveSRRF_CL.java:757-770).  Its NaN-to-0 pass runs first and
+    // its copy to the host goes on the copy stream while the epilogue kernels finish the other
+    // three channels; the values are those of one pass over all four.
+    const long nv = e->nvox;
+    HIPCHK(e, launch_nan_to_zero_copy(e->d_out + 3 * nv, e->d_tmp + 3 * nv, nv, e->stream));
+    HIPCHK(e, hipEventRecord(e->ev_int, e->stream));
+    const int rc = epilogue_impl(e);
     if (rc) return rc;
-    HIPCHK(e, hipMemcpyAsync(out, e->d_tmp, sizeof(float) * 4 * e->nvox, hipMemcpyDeviceToHost,
-                             e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, launch_nan_to_zero_copy(e->d_out, e->d_tmp, 3 * nv, e->stream));
+    // (the epilogue is queued before the first copy: a copy to pageable memory can return
+    // only once it is done; both streams are drained on every path, so no copy into `out`
+    // outlives the call)
+    hipError_t st = hipStreamWaitEvent(e->copy_stream, e->ev_int, 0);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(out + 3 * nv, e->d_tmp + 3 * nv, sizeof(float) * nv,
+                            hipMemcpyDeviceToHost, e->copy_stream);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(out, e->d_tmp, sizeof(float) * 3 * nv, hipMemcpyDeviceToHost,
+                            e->stream);
+    const hipError_t s1 = hipStreamSynchronize(e->copy_stream);
+    const hipError_t s2 = hipStreamSynchronize(e->stream);
+    if (st == hipSuccess) st = s1 != hipSuccess ? s1 : s2;
+    if (st != hipSuccess)
+        return fail(e, ESRRF_ERR_HIP, "result read-back: %s", hipGetErrorString(st));
     return ESRRF_OK;
 }
 

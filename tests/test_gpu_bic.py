@@ -352,6 +352,54 @@ def test_pipelined_host_upload_bit_identical(pkg, n, chunks):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), mode
 
 
+@pytest.mark.parametrize("fht,mp,const", [(0, 1, False), (0, 0, False), (1, 0, False),
+                                          (0, 1, True)])
+def test_host_read_back_matches_device_read_back(pkg, fht, mp, const):
+    """esrrf_read_result copies the INT channel to the host on a second stream while VAR and
+    the MP correction finish the other three; the result equals esrrf_read_result_device's
+    (one NaN-to-0 pass over all four channels after the epilogue) bit for bit, with and
+    without the MP correction, in FHT mode, and for a constant stack whose CGLH is 0/0."""
+    import ctypes
+    from conftest import hip_runtime
+    hip = hip_runtime()
+    C = pkg._capi
+    n, w, h, M = 24, 40, 36, 4
+    st = stack(n, w, h, seed=950 + 2 * fht + mp).astype(np.uint16)
+    if const:
+        st[...] = 300
+    p = C.Params()
+    p.width, p.height, p.magnification, p.fwhm, p.sensitivity = w, h, M, 1.5, 2.0
+    p.n_frames_on_gpu, p.n_frame_for_srrf, p.block_length, p.device = n, n, 20000, 0
+    p.int_weighting, p.do_mp_map_correction, p.do_fht_interpolation = 1, mp, fht
+    nv = 4 * w * M * h * M
+
+    def run(device):
+        eng = pkg.Engine(p)
+        L, hd = eng.L, eng.h
+        eng.check(L.esrrf_reset_frame_position(hd))
+        eng.check(L.esrrf_calculate(hd, st.ctypes.data_as(ctypes.c_void_p), C.ESRRF_U16, n,
+                                    None, None))
+        out = np.zeros(nv, np.float32)
+        if device:
+            buf = ctypes.c_void_p()
+            assert hip.hipMalloc(ctypes.byref(buf), out.nbytes) == 0
+            eng.check(L.esrrf_read_result_device(hd, buf))
+            eng.check(L.esrrf_finish(hd))
+            assert hip.hipMemcpy(out.ctypes.data, buf, out.nbytes, 2) == 0   # device to host
+            hip.hipFree(buf)
+        else:
+            eng.check(L.esrrf_read_result(hd, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        eng.close()
+        return out
+
+    host, dev = run(False), run(True)
+    assert not np.isnan(host).any()
+    assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
+    if const:
+        assert np.all(host[: 3 * nv // 4] == 0.0)
+        assert np.allclose(host[3 * nv // 4:], 300.0, rtol=1e-5, atol=0)
+
+
 def test_upload_chunks_refuses_out_of_range(pkg):
     C = pkg._capi
     p = C.Params()
