@@ -72,10 +72,11 @@ hipError_t launch_var(float *out, long nvox, hipStream_t s, int nbatch, long ost
 // every macro-pixel (M x M) or macro-voxel (M x M x Mz).  An image is nP planes of
 // (w*M) x (h*M) pixels magnified to nP*Mz planes; a 2-D image is nP = Mz = 1.  The reference
 // sums sequentially in float; this tree-reduces in double, which is at least as accurate.
-// The element index i = (z, y, x) of the entry's macro-pixels is split with 32-bit
-// arithmetic (w * h * nP < 2^31).
+// The block is 64 x 16 threads: thread (tx, ty) sums macro-pixel columns tx, tx + 64, ... of
+// macro-pixel rows ty, ty + 16, ... over every plane, so the index arithmetic has one division
+// per row and none per element.
 // blockIdx.y: image of a batch (images `ostride` floats apart, maps `mstride` floats apart).
-constexpr int MP_THREADS = 1024;
+constexpr int MP_TX = 64, MP_TY = 16, MP_THREADS = MP_TX * MP_TY;
 
 __global__ void __launch_bounds__(MP_THREADS) k_mp_map(const float *__restrict__ out,
                                                        float *__restrict__ mp, int w, int h,
@@ -95,33 +96,32 @@ __global__ void __launch_bounds__(MP_THREADS) k_mp_map(const float *__restrict__
     const long wM = (long)w * M;
     const long whM = wM * h * M;
     const long whdM = whM * nP * Mz;
-    const int wh = w * h;
-    const int count = wh * nP;
     const float *base = out + xMP + wM * yMP + whM * zMP + frame * whdM;
+    const int tx = threadIdx.x, ty = threadIdx.y;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < count; i += MP_THREADS) {
-        const int z = i / wh;
-        const int rem = i - z * wh;
-        const int y = rem / w;
-        const int x = rem - y * w;
-        acc += (double)base[(long)x * M + wM * ((long)y * M) + whM * ((long)z * Mz)];
+    for (int row = ty; row < h * nP; row += MP_TY) {
+        const int z = row / h, y = row - z * h;
+        const float *rp = base + whM * ((long)z * Mz) + wM * ((long)y * M);
+#pragma unroll 4
+        for (int x = tx; x < w; x += MP_TX) acc += (double)rp[(long)x * M];
     }
     __shared__ double red[MP_THREADS];
-    red[threadIdx.x] = acc;
+    const int t = ty * MP_TX + tx;
+    red[t] = acc;
     __syncthreads();
     for (int st = MP_THREADS / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (t < st) red[t] += red[t + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) mp[o] = (float)red[0] / (float)count;
+    if (t == 0) mp[o] = (float)red[0] / (float)(w * h * nP);
 }
 
 hipError_t launch_mp_map(const float *out, float *mp, int w, int h, int nP, int M, int Mz,
                          hipStream_t s, int nbatch, long ostride, long mstride)
 {
     int nentries = 3 * M * M * Mz;
-    hipLaunchKernelGGL(k_mp_map, dim3(nentries, nbatch), dim3(MP_THREADS), 0, s, out, mp, w, h,
-                       nP, M, Mz, ostride, mstride);
+    hipLaunchKernelGGL(k_mp_map, dim3(nentries, nbatch), dim3(MP_TX, MP_TY), 0, s, out, mp, w,
+                       h, nP, M, Mz, ostride, mstride);
     return hipGetLastError();
 }
 
