@@ -170,15 +170,16 @@ int esrrf_plan_load(const int *plan6, int r, int l, int *first_index, int *n_fra
 int esrrf_autotune(esrrf_engine *e, int reps, int *nw, int *ppw, int *taps_in_lds, float *ms);
 /* Launch shape the engine's tiled 2-D kernel uses now (no counterpart in the reference):
  * waves per block, sub-pixel phases per wave, tap tables in LDS, and frames per pass
- * (2: two frames share each tap read).  ESRRF_ERR_STATE when the engine runs no tiled
- * kernel (FHT, 3-D, or a geometry outside the tiled kernel's range). */
+ * (2 or 4: two or four frames share each tap read).  ESRRF_ERR_STATE when the engine runs
+ * no tiled kernel (FHT, 3-D, or a geometry outside the tiled kernel's range). */
 int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
                        int *frames_per_pass);
 /* Pin the tiled 2-D kernel's launch shape (no counterpart in the reference), e.g. one the
  * autotuner chose earlier: nw waves per block (1..16, at most 8 sub-pixel phases per wave),
- * tap tables in LDS (1) or through scalar loads (0), one or two frames per pass.
+ * tap tables in LDS (1) or through scalar loads (0), one, two or four frames per pass.
  * ESRRF_ERR_UNSUPPORTED when the kernels cannot run that shape for this geometry (tables or
- * tiles beyond their LDS budget, frame pairs with more than 4 phases per wave);
+ * tiles beyond their LDS budget, two or four frames with more than 4 phases per wave, four
+ * frames with the tables outside LDS or a tile wider than that kernel's);
  * ESRRF_ERR_STATE when the engine runs no tiled kernel. */
 int esrrf_set_launch_shape(esrrf_engine *e, int nw, int taps_in_lds, int frames_per_pass);
 /* Name of the RGC kernel the engine's last calculate launched, e.g. "k_bic_rgc2",

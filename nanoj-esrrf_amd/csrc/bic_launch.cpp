@@ -71,6 +71,18 @@ static bool pair_fits(const BicTables &t, bool tsm = false)
     return bic_pair_lds_bytes(a, tsm) <= 64 * 1024;
 }
 
+// geometry of the four-frame kernel (parity-plane pitch, tile rows, LDS)
+static bool quad_fits(const BicTables &t, int M)
+{
+    BicRgcArgs a{};
+    a.PP = t.geo.PP;
+    a.PH = t.geo.PH;
+    a.TW = t.geo.TW;
+    a.M = M;
+    a.max_class_taps = t.geo.max_class_taps;
+    return bic_quad_fits(a);
+}
+
 BicShape bic_default_shape(const BicTables &t, int M, bool adj)
 {
     const int MM = M * M;
@@ -80,8 +92,12 @@ BicShape bic_default_shape(const BicTables &t, int M, bool adj)
     s.nw = nw;
     s.ppw = (MM + nw - 1) / nw;
     s.taps_in_lds = tlds_fits(t, adj) ? 1 : 0;
-    // frame pairs: one tap read and one address add per two samples
-    if (s.taps_in_lds && !adj && s.ppw <= 4 && pair_fits(t)) s.fp = 2;
+    // four frames per pass where the geometry fits that kernel (one tap read and one address
+    // add per four samples), else frame pairs (per two samples)
+    if (s.taps_in_lds && !adj && s.ppw <= 4 && quad_fits(t, M))
+        s.fp = 4;
+    else if (s.taps_in_lds && !adj && s.ppw <= 4 && pair_fits(t))
+        s.fp = 2;
     return s;
 }
 
@@ -91,6 +107,7 @@ bool bic_shape_supported(const BicTables &t, int M, const BicShape &s, bool adj)
     if (s.nw < 1 || s.nw > 16 || s.ppw != (MM + s.nw - 1) / s.nw || s.ppw > 8) return false;
     if (s.taps_in_lds && !tlds_fits(t, adj)) return false;
     if (s.fp == 2) return !adj && s.ppw <= 4 && pair_fits(t, s.taps_in_lds == 0);
+    if (s.fp == 4) return !adj && s.ppw <= 4 && s.taps_in_lds && quad_fits(t, M);
     return s.fp == 1;
 }
 
@@ -114,6 +131,11 @@ std::vector<BicShape> bic_candidate_shapes(const BicTables &t, int M, bool adj)
             // frame pairs, taps in LDS (tl 1) or through scalar loads (tl 0)
             if (!adj && ppw <= 4 && pair_fits(t, tl == 0)) {
                 s.fp = 2;
+                out.push_back(s);
+            }
+            // four frames per pass (taps in LDS)
+            if (!adj && ppw <= 4 && tl == 1 && quad_fits(t, M)) {
+                s.fp = 4;
                 out.push_back(s);
             }
         }

@@ -16,7 +16,8 @@ namespace esrrf {
 struct BicShape {
     int nw = 1, ppw = 1;
     int taps_in_lds = 0;
-    int fp = 1;           // frames per pass: 2 = frame-pair kernel (taps in LDS, ppw <= 4)
+    int fp = 1;           // frames per pass: 4 = four-frame kernel (mode 0, taps in LDS,
+                          // ppw <= 4, tile fits), 2 = frame-pair kernel (ppw <= 4)
     float ms = 0.0f;      // measured time of the autotuner's sample launch (0: not tuned)
 };
 

@@ -82,7 +82,7 @@ struct BicRgcArgs {
                            //    current pair instead of staging them in registers
     int taps_in_lds;       // 1: stage each block's tap tables in LDS
     int max_class_taps;    // largest tap-table size of one x/y class (LDS sizing)
-    int frames_per_pass;   // 2: frame-pair kernel (modes 0, 1 with taps in LDS), else 1
+    int frames_per_pass;   // 4: four-frame kernel (mode 0), 2: frame-pair kernel, else 1
     // kernel variant: 0 = accumulate the four cumulants (liveSRRF.cl:363-393);
     // 1 = emit the normalised CGLH (before the sensitivity power) and the intensity of
     //     every pixel and frame, phase-major [frame][M*M][h][w], into emitC / emitV;
@@ -149,6 +149,9 @@ hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s,
                           const char **kname = nullptr);
 size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm);
+// four frames per pass (k_bic_rgc4): LDS bytes and whether the geometry fits the kernel
+size_t bic_quad_lds_bytes(const BicRgcArgs &a);
+bool bic_quad_fits(const BicRgcArgs &a);
 // The sweep's emit pass over a union table of nfg <= SWEEP_NFG fwhm values (k_bic_emit_multi):
 // a.nw waves x ppw (1 or 2) phases per block, pg_taps = the largest tap count of one block's
 // phases (LDS sizing), CGLH of fwhm f at a.emitC + f * cstride.

@@ -1129,6 +1129,338 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
 }
 
 // ---------------------------------------------------------------------------------------
+// Four frames per pass (mode 0, tap tables in LDS, parity-plane pitch 24).  The U tile of
+// frames f, f+1 and the tile of frames f+2, f+3 sit QUAD_UPL float4 elements apart, a
+// compile-time distance, so one tap read and one address serve four samples: the second
+// tile's read takes the first's address with an immediate offset.  The raw tiles hold the
+// four frames as float4 (pitch 24: the two rows of a 16-lane pass fall in opposite bank
+// halves).  One set of barriers per four frames.  Taps are walked two at a time (every
+// phase's gated count is rounded up to an even number); each frame's sum takes the taps in
+// order and the cumulants are updated frame by frame, so results equal k_bic_rgc2's bit
+// for bit.
+// ---------------------------------------------------------------------------------------
+constexpr int QUAD_PP = 24;                    // parity-plane pitch the kernel assumes
+constexpr int QUAD_UPL = 4 * QUAD_PP * 14;     // float4 slots of one tile (parity planes of
+                                               // up to 14 rows, TW <= 28)
+constexpr int RAW_P4 = 24;                     // raw-tile pitch (float4 elements)
+
+template <int PPW, bool INTW, int STG>
+__global__ void __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc4(const BicRgcArgs a,
+                                                   const Tap *__restrict__ taps,
+                                                   const Phase *__restrict__ phases,
+                                                   const int *__restrict__ tapk)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4 *lU = reinterpret_cast<float4 *>(smem);                 // [2][QUAD_UPL]
+    float4 *lR = lU + 2 * QUAD_UPL;                                  // [RAW_T][RAW_P4]
+    float4 *lXW = lR + RAW_T * RAW_P4;                               // [M]
+    float4 *lXS = lXW + a.M;                                         // [M][RAW_T][8]
+    int *lXD = reinterpret_cast<int *>(lXS + a.M * XS_PER_SX);      // [M]
+    Tap *lT = static_cast<Tap *>(__builtin_assume_aligned(lXD + XS_MAX_M, 16));
+    int *lPlan = reinterpret_cast<int *>(lT + a.max_class_taps + TAP_SPARE);
+    constexpr int PH_UNROLL = PPW <= 4 ? PPW : 1;
+
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lx = lane & 7, ly = lane >> 3;
+    const int lin = xcd_tile_1d(a.tiles_x * a.tiles_y);
+    const int tX = lin % a.tiles_x, tY = lin / a.tiles_x;
+    const int X0 = tX << 3, Y0 = tY << 3;
+    const int X = X0 + lx, Y = Y0 + ly;
+    const int M = a.M, MM = M * M;
+    const int w = a.w, h = a.h;
+    const Phase *PHS = phases + (axis_class(Y0, a.xs_log2) * a.ncls_x +
+                                   axis_class(X0, a.xs_log2)) * MM;
+    const bool interior = (2 * X0 + a.R2lo >= 1) && (2 * (X0 + 7) + 1 + a.R2hi <= 2 * w - 1) &&
+                          (2 * Y0 + a.R2lo >= 1) && (2 * (Y0 + 7) + 1 + a.R2hi <= 2 * h - 1);
+    const bool inimg = (X < w) && (Y < h);
+    const long whM = (long)a.wM * a.hM;
+
+    float s_avg[PPW], s_e2[PPW], s_tac[PPW], s_int[PPW], s_prv[PPW];
+    float invw[PPW];
+    int xbase[PPW];
+#pragma unroll
+    for (int p = 0; p < PPW; p++) {
+        const int ph = wave + p * a.nw;
+        invw[p] = 0.0f;
+        xbase[p] = -1;
+        s_avg[p] = s_e2[p] = s_tac[p] = s_int[p] = s_prv[p] = 0.0f;
+        if (ph < MM) {
+            const Phase P = PHS[ph];
+            float ws = P.wsum;
+            if (!interior) {
+                ws = 0.0f;
+                for (int t = 0; t < P.ntaps; t++) {
+                    const int k = tapk[P.first_tap + t];
+                    const int kx = 2 * X + (k & 255) - 128;
+                    const int ky = 2 * Y + ((k >> 8) & 255) - 128;
+                    if (kx >= 1 && kx <= 2 * w - 1 && ky >= 1 && ky <= 2 * h - 1)
+                        ws += taps[P.first_tap + t].W;
+                }
+            }
+            invw[p] = 1.0f / ws;
+            const int u0 = X + P.du, v0 = Y + P.dv;
+            if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2)
+                xbase[p] = (P.sx * RAW_T + (v0 - 1 - (Y0 - 2))) * 8 + lx;
+            if (!a.first_init && inimg) {
+                const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
+                s_avg[p] = a.out[pix];
+                s_e2[p] = a.out[pix + whM];
+                s_tac[p] = a.out[pix + 2 * whM];
+                s_int[p] = a.out[pix + 3 * whM];
+                s_prv[p] = a.prev[pix];
+            }
+        }
+    }
+
+    const int wI = 2 * w, hI = 2 * h;
+    const long whI = (long)wI * hI, wh = (long)w * h;
+    const int kxo = 2 * X0 - a.G0, kyo = 2 * Y0 - a.G0;
+    const int planesz = a.PP * a.PH;
+    const int base = ly * a.PP + lx;
+    const int pw = a.TW >> 1;
+    const int nU = 4 * a.PH * pw;
+
+    for (int e = tid; e < nU; e += nthr) {
+        const int pl = e / (a.PH * pw);
+        const int r = (e - pl * a.PH * pw) / pw;
+        const int c = e - pl * a.PH * pw - r * pw;
+        const int gy = 2 * r + (pl >> 1), gx = 2 * c + (pl & 1);
+        lPlan[e] = (pl * planesz + r * a.PP + c) | (gy << 15) | (gx << 23);
+    }
+    const int tcls0 = PHS[0].first_tap;
+    {
+        const int tcls1 = PHS[MM - 1].first_tap + PHS[MM - 1].ntaps;
+        for (int e = tid; e < tcls1 - tcls0 + TAP_SPARE; e += nthr) lT[e] = taps[tcls0 + e];
+    }
+    for (int e = tid; e < M; e += nthr) {
+        const Phase P = PHS[e];
+        lXW[e] = make_float4(P.wx[0], P.wx[1], P.wx[2], P.wx[3]);
+        lXD[e] = P.du + 1;
+    }
+    __syncthreads();
+
+    // x row sums of the bicubic intensity (liveSRRF.cl:361) for the four frames, as in
+    // k_bic_rgc2
+    auto xpass = [&]() {
+        const int nxs = M * XS_PER_SX;
+        for (int e = tid; e < nxs; e += nthr) {
+            const int sx = e / XS_PER_SX;
+            const int rr = e - sx * XS_PER_SX;
+            const float4 wv = lXW[sx];
+            const float4 *src = lR + (rr >> 3) * RAW_P4 + (rr & 7) + lXD[sx];
+            const float4 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
+            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            q.x = fmaf(v0.x, wv.x, q.x);
+            q.y = fmaf(v0.y, wv.x, q.y);
+            q.z = fmaf(v0.z, wv.x, q.z);
+            q.w = fmaf(v0.w, wv.x, q.w);
+            q.x = fmaf(v1.x, wv.y, q.x);
+            q.y = fmaf(v1.y, wv.y, q.y);
+            q.z = fmaf(v1.z, wv.y, q.z);
+            q.w = fmaf(v1.w, wv.y, q.w);
+            q.x = fmaf(v2.x, wv.z, q.x);
+            q.y = fmaf(v2.y, wv.z, q.y);
+            q.z = fmaf(v2.z, wv.z, q.z);
+            q.w = fmaf(v2.w, wv.z, q.w);
+            q.x = fmaf(v3.x, wv.w, q.x);
+            q.y = fmaf(v3.y, wv.w, q.y);
+            q.z = fmaf(v3.z, wv.w, q.z);
+            q.w = fmaf(v3.w, wv.w, q.w);
+            lXS[e] = q;
+        }
+    };
+
+    // frames f .. f+nf-1 (nf <= 4; missing frames read as 0) into the LDS tiles
+    auto fill = [&](int f, int nf) {
+        const float2 *Ua = a.U + whI * f;
+        for (int e = tid; e < nU; e += nthr) {
+            const int pk = lPlan[e];
+            const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
+            const int kx = kxo + gx, ky = kyo + gy;
+            float2 u[4] = {};
+            if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI) {
+                const long g = (long)ky * wI + kx;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (j < nf) u[j] = Ua[g + j * whI];
+            }
+            lU[pk & 0x7fff] = make_float4(u[0].x, u[0].y, u[1].x, u[1].y);
+            lU[QUAD_UPL + (pk & 0x7fff)] = make_float4(u[2].x, u[2].y, u[3].x, u[3].y);
+        }
+        const float *Pa = a.px + wh * f;
+        for (int e = tid; e < RAW_T * RAW_T; e += nthr) {
+            const int r = e / RAW_T, c = e - r * RAW_T;
+            const long g = (long)min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+            float v[4] = {};
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < nf) v[j] = Pa[g + j * wh];
+            lR[r * RAW_P4 + c] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    };
+
+    // register staging of the next four frames (STG > 0; the launch guarantees
+    // nU <= STG * nthr and RAW_T^2 <= nthr)
+    float4 su[STG > 0 ? STG : 1], su2[STG > 0 ? STG : 1];
+    float4 sr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int sgo[STG > 0 ? STG : 1], sdst[STG > 0 ? STG : 1];
+    int rgo = -1;
+#pragma unroll
+    for (int k = 0; k < STG; k++) {
+        const int e = tid + k * nthr;
+        sgo[k] = -1;
+        sdst[k] = -1;
+        if (e < nU) {
+            const int pk = lPlan[e];
+            const int gy = (pk >> 15) & 255, gx = (pk >> 23) & 255;
+            const int kx = kxo + gx, ky = kyo + gy;
+            sdst[k] = pk & 0x7fff;
+            if (gy < a.TH && gx < a.TW && kx >= 0 && kx < wI && ky >= 0 && ky < hI)
+                sgo[k] = ky * wI + kx;
+        }
+    }
+    if (STG && tid < RAW_T * RAW_T) {
+        const int r = tid / RAW_T, c = tid - r * RAW_T;
+        rgo = min(max(Y0 - 2 + r, 0), h - 1) * w + min(max(X0 - 2 + c, 0), w - 1);
+    }
+    auto prefetch = [&](int f, int nf) {
+        const float2 *Ua = a.U + whI * f;
+#pragma unroll
+        for (int k = 0; k < STG; k++) {
+            float2 u[4] = {};
+            if (sgo[k] >= 0) {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (j < nf) u[j] = Ua[sgo[k] + j * whI];
+            }
+            su[k] = make_float4(u[0].x, u[0].y, u[1].x, u[1].y);
+            su2[k] = make_float4(u[2].x, u[2].y, u[3].x, u[3].y);
+        }
+        if (rgo >= 0) {
+            const float *Pa = a.px + wh * f;
+            float v[4] = {};
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < nf) v[j] = Pa[rgo + j * wh];
+            sr = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int k = 0; k < STG; k++)
+            if (sdst[k] >= 0) {
+                lU[sdst[k]] = su[k];
+                lU[QUAD_UPL + sdst[k]] = su2[k];
+            }
+        if (rgo >= 0) {
+            const int r = tid / RAW_T, c = tid - r * RAW_T;
+            lR[r * RAW_P4 + c] = sr;
+        }
+    };
+
+    fill(a.frame0, min(4, a.nframes));
+    __syncthreads();
+    xpass();
+    __syncthreads();
+
+    for (int fi = 0; fi < a.nframes; fi += 4) {
+        const int nf = min(4, a.nframes - fi);
+        const bool first = (fi == 0) && a.first_init;
+        if (STG && fi + 4 < a.nframes) prefetch(a.frame0 + fi + 4, min(4, a.nframes - fi - 4));
+        const int zf = a.zero * fi;
+        const Phase *PHf = PHS + zf;
+
+#pragma unroll PH_UNROLL
+        for (int p = 0; p < PPW; p++) {
+            const int ph = wave + p * a.nw;
+            if (ph < MM) {
+                const Phase P = PHf[ph];
+                float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+                const Tap *Tl = lT + (P.first_tap - tcls0);
+                // (the loop stops at the even gated count: the zero-weight padding beyond it
+                // adds -0 to each sum)
+#pragma unroll 1
+                for (int t = 0; t < P.ntaps2; t += 2) {
+                    const Tap c0 = Tl[t], c1 = Tl[t + 1];
+                    const float4 *p0 = lU + base + c0.off;
+                    const float4 *p1 = lU + base + c1.off;
+                    const float4 a0 = p0[0], b0 = p0[QUAD_UPL];
+                    const float4 a1 = p1[0], b1 = p1[QUAD_UPL];
+                    acc0 = tap_acc(make_float2(a0.x, a0.y), c0, acc0);
+                    acc1 = tap_acc(make_float2(a0.z, a0.w), c0, acc1);
+                    acc2 = tap_acc(make_float2(b0.x, b0.y), c0, acc2);
+                    acc3 = tap_acc(make_float2(b0.z, b0.w), c0, acc3);
+                    acc0 = tap_acc(make_float2(a1.x, a1.y), c1, acc0);
+                    acc1 = tap_acc(make_float2(a1.z, a1.w), c1, acc1);
+                    acc2 = tap_acc(make_float2(b1.x, b1.y), c1, acc2);
+                    acc3 = tap_acc(make_float2(b1.z, b1.w), c1, acc3);
+                }
+                const float C0[4] = {(P.wsum + acc0) * invw[p], (P.wsum + acc1) * invw[p],
+                                     (P.wsum + acc2) * invw[p], (P.wsum + acc3) * invw[p]};
+                float v[4];
+                if (xbase[p] >= 0) {
+                    const float4 *xs = lXS + xbase[p];
+                    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float4 r4 = xs[j * 8];
+                        q.x = fmaf(r4.x, P.wy[j], q.x);
+                        q.y = fmaf(r4.y, P.wy[j], q.y);
+                        q.z = fmaf(r4.z, P.wy[j], q.z);
+                        q.w = fmaf(r4.w, P.wy[j], q.w);
+                    }
+                    v[0] = q.x;
+                    v[1] = q.y;
+                    v[2] = q.z;
+                    v[3] = q.w;
+                } else {
+                    const int xM = (X + zf) * M + P.sx, yM = (Y + zf) * M + P.sy;
+                    const float xf = (float)xM / (float)M - 0.5f;
+                    const float yf = (float)yM / (float)M - 0.5f;
+                    const float *lRf = reinterpret_cast<const float *>(lR);
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        v[j] = interp_tile_s<4>(lRf + j, RAW_P4, X0 - 2, Y0 - 2, w, h, xf, yf);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (j < nf)
+                        cumulate<INTW>(s_avg[p], s_e2[p], s_tac[p], s_int[p], s_prv[p], C0[j],
+                                       v[j], first && j == 0, a);
+            }
+        }
+        if (fi + 4 < a.nframes) {
+            __syncthreads();   // every wave is done with these frames' tiles
+            if (STG)
+                commit();
+            else
+                fill(a.frame0 + fi + 4, min(4, a.nframes - fi - 4));
+            __syncthreads();
+            xpass();
+            __syncthreads();
+        }
+    }
+
+    if (inimg) {
+#pragma unroll
+        for (int p = 0; p < PPW; p++) {
+            const int ph = wave + p * a.nw;
+            if (ph >= MM) continue;
+            const Phase P = PHS[ph];
+            const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
+            a.out[pix] = s_avg[p];
+            a.out[pix + whM] = s_e2[p];
+            a.out[pix + 2 * whM] = s_tac[p];
+            a.out[pix + 3 * whM] = s_int[p];
+            a.prev[pix] = s_prv[p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // k_bic_emit_multi: the parameter sweep's emit pass for up to SWEEP_NFG fwhm values of one
 // magnification at once.  A sample's angular term h = (1 - |sin|)[cos < 0] - 1 depends on
 // the tap's offset and direction only; the fwhm sets the weights and the gate.  The union
@@ -1386,6 +1718,35 @@ size_t bic_pair_lds_bytes(const BicRgcArgs &a, bool tsm)
            sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
 }
 
+size_t bic_quad_lds_bytes(const BicRgcArgs &a)
+{
+    return sizeof(float4) * (2 * (size_t)QUAD_UPL + RAW_T * RAW_P4) +
+           (sizeof(float4) + sizeof(float4) * XS_PER_SX) * (size_t)a.M +
+           sizeof(int) * XS_MAX_M + sizeof(Tap) * ((size_t)a.max_class_taps + TAP_SPARE) +
+           sizeof(int) * (size_t)(4 * a.PH * (a.TW >> 1));
+}
+
+bool bic_quad_fits(const BicRgcArgs &a)
+{
+    return a.PP == QUAD_PP && 4 * a.PP * a.PH <= QUAD_UPL && a.M <= XS_MAX_M &&
+           bic_quad_lds_bytes(a) <= 80 * 1024;
+}
+
+template <int PPW, bool INTW>
+static hipError_t launch_quad(const BicRgcArgs &a, hipStream_t s)
+{
+    // register staging needs every tile element within 2 per thread
+    const int nthr = 64 * a.nw, nU = 4 * a.PH * (a.TW >> 1);
+    const size_t lds = bic_quad_lds_bytes(a);
+    if (!a.pair_direct && nthr >= RAW_T * RAW_T && nU <= 2 * nthr)
+        hipLaunchKernelGGL((k_bic_rgc4<PPW, INTW, 2>), dim3(a.tiles_x * a.tiles_y), dim3(nthr),
+                           lds, s, a, a.taps, a.phases, a.tapk);
+    else
+        hipLaunchKernelGGL((k_bic_rgc4<PPW, INTW, 0>), dim3(a.tiles_x * a.tiles_y), dim3(nthr),
+                           lds, s, a, a.taps, a.phases, a.tapk);
+    return hipGetLastError();
+}
+
 // taps_in_lds 0: the tap tables are read with scalar loads (no LDS copy)
 template <int PPW, int MODE, bool INTW, int STG>
 static hipError_t launch_pair_s(const BicRgcArgs &a, hipStream_t s)
@@ -1464,9 +1825,20 @@ static hipError_t launch_ppw(const BicRgcArgs &a, hipStream_t s)
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s, const char **kname)
 {
     if (a.nframes <= 0) return hipSuccess;
-    const bool pair = a.frames_per_pass == 2 && a.mode != 2 && ppw <= 4 && a.tac_prev_scaled &&
-                      bic_pair_lds_bytes(a, !a.taps_in_lds) <= 64 * 1024;
-    if (kname) *kname = pair ? "k_bic_rgc2" : "k_bic_rgc";
+    const bool quad = a.frames_per_pass == 4 && a.mode == 0 && ppw <= 4 && a.tac_prev_scaled &&
+                      a.taps_in_lds && bic_quad_fits(a);
+    const bool pair = !quad && a.frames_per_pass >= 2 && a.mode != 2 && ppw <= 4 &&
+                      a.tac_prev_scaled && bic_pair_lds_bytes(a, !a.taps_in_lds) <= 64 * 1024;
+    if (kname) *kname = quad ? "k_bic_rgc4" : pair ? "k_bic_rgc2" : "k_bic_rgc";
+    if (quad) {
+        const bool iw = a.int_weighting != 0;
+        switch (ppw) {
+        case 1: return iw ? launch_quad<1, true>(a, s) : launch_quad<1, false>(a, s);
+        case 2: return iw ? launch_quad<2, true>(a, s) : launch_quad<2, false>(a, s);
+        case 3: return iw ? launch_quad<3, true>(a, s) : launch_quad<3, false>(a, s);
+        default: return iw ? launch_quad<4, true>(a, s) : launch_quad<4, false>(a, s);
+        }
+    }
     if (pair) {
         switch (ppw) {
         case 1: return launch_pair<1>(a, s);

@@ -84,19 +84,35 @@ def test_bic_configs_parity(pkg, O, M, fwhm, S, w, h):
 
 @pytest.mark.parametrize("M,w,h,loads", [
     (5, 40, 32, [(0, 5), (5, 9)]), (4, 27, 19, [(0, 7)]), (2, 24, 24, [(0, 1), (1, 4)]),
-    (7, 16, 24, [(0, 6)]),
+    (7, 16, 24, [(0, 6)]), (5, 33, 29, [(0, 2), (2, 13), (13, 16)]), (3, 20, 18, [(0, 11)]),
 ])
 def test_bic_frame_pair_kernel_matches_single_frame(pkg, M, w, h, loads):
-    """The frame-pair kernel (two frames per LDS pass, the default) reproduces the
-    single-frame kernel bit for bit: odd tails, one-frame loads and continuation across
-    loads included."""
+    """The frame-pair kernel (two frames per LDS pass) and the four-frame kernel (the default
+    where the geometry fits it) reproduce the single-frame kernel bit for bit: tails of one
+    to three frames, one-frame loads and continuation across loads included."""
     n = loads[-1][1]
     st = stack(n, w, h, seed=40 + M)
     res = {}
-    for fp in (1, 2):
+    for fp in (1, 2, 4):
         res[fp], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
                                    nFor=n, loads=loads, shape=dict(frames_per_pass=fp))
     assert np.array_equal(res[1], res[2])
+    assert np.array_equal(res[1], res[4])
+
+
+def test_bic_four_frame_kernel_is_the_default(pkg):
+    """Where the geometry fits it (parity-plane pitch 24, tile rows, LDS), the engine runs
+    four frames per pass by default; a wider window falls back to frame pairs."""
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(40, 32, 5, 1.5, 2, 4, 4, 20000, e.allCLdevices[0], True, True, False, None, 0, 1)
+    assert e.launchShape()["frames_per_pass"] == 4
+    e.release()
+    e = pkg.LiveSRRF_CL()
+    e.checkDevices()
+    e.initialise(64, 64, 5, 4.0, 2, 4, 4, 20000, e.allCLdevices[0], True, True, False, None, 0, 1)
+    assert e.launchShape()["frames_per_pass"] in (1, 2)
+    e.release()
 
 
 @pytest.mark.parametrize("M,w,h,loads", [
@@ -116,19 +132,23 @@ def test_bic_frame_pair_taps_from_scalar_loads(pkg, M, w, h, loads):
     assert np.array_equal(res[0], res[1])
 
 
-@pytest.mark.parametrize("M,fwhm,w,h,nw,tl", [(5, 1.5, 40, 32, "8", 1), (4, 2.0, 27, 19, "4", 1),
-                                               (2, 1.0, 24, 24, "4", 1),
-                                               (5, 3.0, 16, 16, "12", 0)])
-def test_bic_frame_pair_register_staging(pkg, monkeypatch, M, fwhm, w, h, nw, tl):
-    """Staging the next frame pair's tiles in registers during the current pair (stored
-    to LDS between the barriers) gives the same bits as filling them after it, odd frame
-    tails and continuation across loads included."""
+@pytest.mark.parametrize("M,fwhm,w,h,nw,tl,fp", [(5, 1.5, 40, 32, "8", 1, 2),
+                                                  (4, 2.0, 27, 19, "4", 1, 2),
+                                                  (2, 1.0, 24, 24, "4", 1, 2),
+                                                  (5, 3.0, 16, 16, "12", 0, 2),
+                                                  (5, 1.5, 40, 32, "8", 1, 4),
+                                                  (5, 1.5, 40, 32, "16", 1, 4),
+                                                  (4, 2.0, 27, 19, "4", 1, 4)])
+def test_bic_frame_pair_register_staging(pkg, monkeypatch, M, fwhm, w, h, nw, tl, fp):
+    """Staging the next frame pair's (or four frames') tiles in registers during the current
+    pass (stored to LDS between the barriers) gives the same bits as filling them after it,
+    odd frame tails and continuation across loads included."""
     st = stack(7, w, h, seed=140 + M)
     res = {}
     for sg in ("0", "1"):
         monkeypatch.setenv("ESRRF_BIC_PAIR_DIRECT", sg)
         res[sg], _, _ = run_engine(pkg, st, M, fwhm, 2, nOnGPU=7, loads=[(0, 5), (5, 7)],
-                                   shape=dict(frames_per_pass=2, nw=int(nw), taps_in_lds=tl))
+                                   shape=dict(frames_per_pass=fp, nw=int(nw), taps_in_lds=tl))
     assert np.array_equal(res["0"], res["1"])
 
 

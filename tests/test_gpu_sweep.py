@@ -179,14 +179,14 @@ def test_sweep_full_grid_autotuned_matches_per_configuration_engine(pkg):
 @pytest.mark.parametrize("fwhm", C3_FWHM)
 def test_every_runnable_launch_shape_is_bit_identical(pkg, M, fwhm):
     """Every launch shape the autotuner may pick (4 to 16 waves, tap tables in LDS or
-    through scalar loads, one or two frames per pass) gives the default shape's bits for
-    this (magnification, fwhm), on a 40x32 stack in two loads."""
+    through scalar loads, one, two or four frames per pass) gives the default shape's bits
+    for this (magnification, fwhm), on a 40x32 stack in two loads."""
     st = stack(24, 40, 32, seed=404 + M)
     ref, shape0 = None, None
     ran = []
     for nw in ([M * M] if M * M <= 4 else []) + [4, 8, 12, 16]:
         for tl in (1, 0):
-            for fp in (1, 2):
+            for fp in (1, 2, 4):
                 e = pkg.LiveSRRF_CL()
                 e.checkDevices()
                 e.initialise(40, 32, M, fwhm, 2.0, 12, 24, 20000, e.allCLdevices[0], True, True,
