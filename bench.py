@@ -118,10 +118,11 @@ def cpu_baseline(stack, nframes):
                       f"holds for the full stack"}
 
 
-def load_traffic():
-    """HBM bytes per RGC launch from the committed rocprofv3 PMC summary (or None): the
-    line `hbm_bytes_per_launch: <int>` of profiles/k_bic_rgc2_256x256x1000_m5_traffic.md."""
-    p = os.path.join(REPO, "profiles", "k_bic_rgc2_256x256x1000_m5_traffic.md")
+def load_traffic(kname):
+    """HBM bytes per launch of RGC kernel `kname` from its committed rocprofv3 PMC summary
+    (or None): the line `hbm_bytes_per_launch: <int>` of
+    profiles/<kname>_256x256x1000_m5_traffic.md."""
+    p = os.path.join(REPO, "profiles", f"{kname}_256x256x1000_m5_traffic.md")
     try:
         with open(p) as f:
             for line in f:
@@ -569,7 +570,7 @@ def main():
                      "kernel": kname,
                      "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None if fht else load_traffic(),
+                     "traffic": None if fht else load_traffic(kname),
                      "kernel_ms": round(rgc_avg, 3), "launches": rgc_n.value,
                      "algorithmic_flop_per_launch": flops,
                      "gated_samples_per_frame": nact.value,
