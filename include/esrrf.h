@@ -182,7 +182,7 @@ int esrrf_launch_shape(esrrf_engine *e, int *nw, int *ppw, int *taps_in_lds,
  * frames with the tables outside LDS or a tile wider than that kernel's);
  * ESRRF_ERR_STATE when the engine runs no tiled kernel. */
 int esrrf_set_launch_shape(esrrf_engine *e, int nw, int taps_in_lds, int frames_per_pass);
-/* Name of the RGC kernel the engine's last calculate launched, e.g. "k_bic_rgc2",
+/* Name of the RGC kernel the engine's last calculate launched, e.g. "k_bic_rgc4",
  * "k_fht_rgc2", "k3d_rgc3" ("" before the first launch).  No counterpart in the reference
  * (its single OpenCL kernel per mode needs none); for profiling and tests. */
 int esrrf_rgc_kernel(esrrf_engine *e, char *buf, int len);
