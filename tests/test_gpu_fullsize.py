@@ -82,8 +82,12 @@ def test_full_size_mp_correction(pkg, production, full_nomp):
 
 
 def test_full_size_kernels_and_loads_bit_identical(pkg, production, full_nomp):
+    # the default shape runs four frames per pass; one and two frames per pass, and the
+    # stack in four loads, give the same bits
     single = engine(pkg, production, mp=False, shape=dict(frames_per_pass=1))[0]
     assert np.array_equal(single, full_nomp)
+    pair = engine(pkg, production, mp=False, shape=dict(frames_per_pass=2))[0]
+    assert np.array_equal(pair, full_nomp)
     four = engine(pkg, production, mp=False, loads=[(0, 250), (250, 500), (500, 750),
                                                     (750, 1000)])[0]
     assert np.array_equal(four, full_nomp)
