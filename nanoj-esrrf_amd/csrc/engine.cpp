@@ -309,7 +309,7 @@ int calculate_bic_pipelined(esrrf_engine *e, const void *frames, int dtype, int 
         {
             ProfScope ps(e, 1);
             HIPCHK(e, launch_bic_prep(e->d_px + npx * f0, e->d_U + 4 * npx * f0, e->w, e->h, k,
-                                      e->stream, e->var.bic_prep_per_point));
+                                      e->stream, e->var.bic_prep));
         }
         int rc = run_bic_rgc(e, f0, e->counters[0], k);
         if (rc != ESRRF_OK) return rc;
@@ -357,7 +357,7 @@ int calculate_impl(esrrf_engine *e, const void *frames, bool on_device, int dtyp
         {
             ProfScope ps(e, 1);
             HIPCHK(e, launch_bic_prep(e->d_px, e->d_U, e->w, e->h, n, e->stream,
-                                      e->var.bic_prep_per_point));
+                                      e->var.bic_prep));
         }
     } else {
         ModeCtx c = {e->stream, e->w, e->h, e->W, e->H, e->nP, e->M, e->wM, e->hM,

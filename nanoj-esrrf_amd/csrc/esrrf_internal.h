@@ -141,8 +141,9 @@ struct Rgc3Tables {
 
 // Host helpers implemented in the .hip files (all launches on `s`).
 hipError_t launch_convert_frames(const void *src, int dtype, float *dst, long n, hipStream_t s);
+// kind 4: k_bic_prep4 (four frames per block, the default); 2: k_bic_prep2; 1: k_bic_prep
 hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s,
-                           bool per_point = false);
+                           int kind = 4);
 hipError_t launch_bic_gint(const float *px, float *gxi, float *gyi, int w, int h, int f,
                            hipStream_t s);
 // *kname (when given) receives the name of the kernel launched.

@@ -290,17 +290,187 @@ __global__ void __launch_bounds__(256) k_bic_gint(const float *__restrict__ px,
     gyi[(long)y * 2 * w + x] = gint_at(lgy, PREP_G, gx0, gy0, w, h, x, y);
 }
 
+// k_bic_prep4: k_bic_prep2 for four frames at once.  Every LDS tile holds the four frames'
+// values as float4, so a point's index arithmetic, gates and interpolation weights are
+// formed once for the four frames; each frame's values come from the same operations as in
+// k_bic_prep2 (blockIdx.z: frames 4z .. 4z+3, those past nF skipped).
+__global__ void __launch_bounds__(256) k_bic_prep4(const float *__restrict__ px,
+                                                   float2 *__restrict__ U, int w, int h, int nF)
+{
+    __shared__ float4 lgx[P2_G * P2_G], lgy[P2_G * P2_G];
+    __shared__ float4 rsx[P2_G * P2_T], rsy[P2_G * P2_T];   // row sums [tile row][sample x]
+    const int f0 = blockIdx.z * 4;
+    const int nf = min(4, nF - f0);
+    const int kx0 = blockIdx.x * P2_T, ky0 = blockIdx.y * P2_T;
+    const int gx0 = (kx0 >> 1) - 2, gy0 = (ky0 >> 1) - 2;
+    const int tid = threadIdx.x;
+    const long wh = (long)w * h;
+    const float *P = px + wh * f0;
+    // 2-point gradient (liveSRRF.cl:199-204) at image (gx0 + c, gy0 + r) of each frame
+    constexpr int GK = (P2_G * P2_G + 255) / 256;
+    float vp[GK][4], vl[GK][4], vu[GK][4];
+    bool ok[GK];
+#pragma unroll
+    for (int k = 0; k < GK; k++) {
+        const int e = tid + k * 256;
+        const int r = e / P2_G, c = e - r * P2_G;
+        const int x1 = gx0 + c, y1 = gy0 + r;
+        ok[k] = e < P2_G * P2_G && x1 >= 0 && x1 < w && y1 >= 0 && y1 < h;
+#pragma unroll
+        for (int j = 0; j < 4; j++) vp[k][j] = vl[k][j] = vu[k][j] = 0.0f;
+        if (ok[k]) {
+            const long op = (long)y1 * w + x1;
+            const long ol = (long)y1 * w + max(x1 - 1, 0);
+            const long ou = (long)max(y1 - 1, 0) * w + x1;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < nf) {
+                    vp[k][j] = P[op + j * wh];
+                    vl[k][j] = P[ol + j * wh];
+                    vu[k][j] = P[ou + j * wh];
+                }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < GK; k++) {
+        const int e = tid + k * 256;
+        if (e < P2_G * P2_G) {
+            lgx[e] = ok[k] ? make_float4(vp[k][0] - vl[k][0], vp[k][1] - vl[k][1],
+                                         vp[k][2] - vl[k][2], vp[k][3] - vl[k][3])
+                           : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            lgy[e] = ok[k] ? make_float4(vp[k][0] - vu[k][0], vp[k][1] - vu[k][1],
+                                         vp[k][2] - vu[k][2], vp[k][3] - vu[k][3])
+                           : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    __syncthreads();
+    // row sums along x: GxInt samples x = kx, GyInt samples x = kx - 1
+    auto rowsum = [](const float4 *t, float w0, float w1, float w2, float w3) {
+        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        p.x = fmaf(t[0].x, w0, p.x);
+        p.y = fmaf(t[0].y, w0, p.y);
+        p.z = fmaf(t[0].z, w0, p.z);
+        p.w = fmaf(t[0].w, w0, p.w);
+        p.x = fmaf(t[1].x, w1, p.x);
+        p.y = fmaf(t[1].y, w1, p.y);
+        p.z = fmaf(t[1].z, w1, p.z);
+        p.w = fmaf(t[1].w, w1, p.w);
+        p.x = fmaf(t[2].x, w2, p.x);
+        p.y = fmaf(t[2].y, w2, p.y);
+        p.z = fmaf(t[2].z, w2, p.z);
+        p.w = fmaf(t[2].w, w2, p.w);
+        p.x = fmaf(t[3].x, w3, p.x);
+        p.y = fmaf(t[3].y, w3, p.y);
+        p.z = fmaf(t[3].z, w3, p.z);
+        p.w = fmaf(t[3].w, w3, p.w);
+        return p;
+    };
+    for (int e = tid; e < P2_G * P2_T; e += 256) {
+        const int r = e / P2_T, q = e - r * P2_T;
+        const int kx = kx0 + q;
+        float w0, w1, w2, w3;
+        {
+            const int X = kx, u0 = X >> 1;
+            cubic_half_weights(X & 1, w0, w1, w2, w3);
+            rsx[e] = rowsum(lgx + r * P2_G + (u0 - 1 - gx0), w0, w1, w2, w3);
+        }
+        {
+            const int X = kx - 1, u0 = X >> 1;
+            cubic_half_weights(X & 1, w0, w1, w2, w3);
+            rsy[e] = rowsum(lgy + r * P2_G + (u0 - 1 - gx0), w0, w1, w2, w3);
+        }
+    }
+    __syncthreads();
+    const int wI = 2 * w, hI = 2 * h;
+    const long whI = (long)wI * hI;
+    float2 *Uf = U + whI * f0;
+    const float *lgxf = reinterpret_cast<const float *>(lgx);
+    const float *lgyf = reinterpret_cast<const float *>(lgy);
+#pragma unroll
+    for (int o = 0; o < (P2_T * P2_T) / 256; o++) {
+        const int e = tid + o * 256;
+        const int q = e & (P2_T - 1), yy = e / P2_T;
+        const int kx = kx0 + q, ky = ky0 + yy;
+        if (kx >= wI || ky >= hI) continue;
+        float tx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ty[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const bool gated = kx >= 1 && ky >= 1;   // gates 0 < vx, 0 < vy of liveSRRF.cl:282,285
+        if (gated) {
+            float w0, w1, w2, w3;
+            {   // GxInt at (kx, ky - 1)
+                const int X = kx, Y = ky - 1, u0 = X >> 1, v0 = Y >> 1;
+                if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+                    cubic_half_weights(Y & 1, w0, w1, w2, w3);
+                    const float4 *c = rsx + (v0 - 1 - gy0) * P2_T + q;
+                    const float4 c0 = c[0], c1 = c[P2_T], c2 = c[2 * P2_T], c3 = c[3 * P2_T];
+                    const float a0[4] = {c0.x, c0.y, c0.z, c0.w}, a1[4] = {c1.x, c1.y, c1.z, c1.w};
+                    const float a2[4] = {c2.x, c2.y, c2.z, c2.w}, a3[4] = {c3.x, c3.y, c3.z, c3.w};
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        float t = 0.0f;
+                        t = fmaf(a0[j], w0, t);
+                        t = fmaf(a1[j], w1, t);
+                        t = fmaf(a2[j], w2, t);
+                        t = fmaf(a3[j], w3, t);
+                        tx[j] = t;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (j < nf)
+                            tx[j] = interp_tile_s<4>(lgxf + j, P2_G, gx0, gy0, w, h,
+                                                     (float)X / 2.0f, (float)Y / 2.0f);
+                }
+            }
+            {   // GyInt at (kx - 1, ky)
+                const int X = kx - 1, Y = ky, u0 = X >> 1, v0 = Y >> 1;
+                if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2) {
+                    cubic_half_weights(Y & 1, w0, w1, w2, w3);
+                    const float4 *c = rsy + (v0 - 1 - gy0) * P2_T + q;
+                    const float4 c0 = c[0], c1 = c[P2_T], c2 = c[2 * P2_T], c3 = c[3 * P2_T];
+                    const float a0[4] = {c0.x, c0.y, c0.z, c0.w}, a1[4] = {c1.x, c1.y, c1.z, c1.w};
+                    const float a2[4] = {c2.x, c2.y, c2.z, c2.w}, a3[4] = {c3.x, c3.y, c3.z, c3.w};
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        float t = 0.0f;
+                        t = fmaf(a0[j], w0, t);
+                        t = fmaf(a1[j], w1, t);
+                        t = fmaf(a2[j], w2, t);
+                        t = fmaf(a3[j], w3, t);
+                        ty[j] = t;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (j < nf)
+                            ty[j] = interp_tile_s<4>(lgyf + j, P2_G, gx0, gy0, w, h,
+                                                     (float)X / 2.0f, (float)Y / 2.0f);
+                }
+            }
+        }
+        const long pix = (long)ky * wI + kx;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (j < nf)
+                Uf[pix + j * whI] = gated ? unit_or_zero(tx[j], ty[j]) : make_float2(0.0f, 0.0f);
+    }
+}
+
 hipError_t launch_bic_prep(const float *px, float2 *U, int w, int h, int nF, hipStream_t s,
-                           bool per_point)
+                           int kind)
 {
     if (nF <= 0) return hipSuccess;
-    if (per_point) {   // one 4x4 stencil per sample point (k_bic_prep)
+    if (kind == 1) {   // one 4x4 stencil per sample point (k_bic_prep)
         dim3 grid((2 * w + PREP_T - 1) / PREP_T, (2 * h + PREP_T - 1) / PREP_T, nF);
         hipLaunchKernelGGL(k_bic_prep, grid, dim3(PREP_T, PREP_T), 0, s, px, U, w, h);
         return hipGetLastError();
     }
-    dim3 grid((2 * w + P2_T - 1) / P2_T, (2 * h + P2_T - 1) / P2_T, nF);
-    hipLaunchKernelGGL(k_bic_prep2, grid, dim3(256), 0, s, px, U, w, h);
+    if (kind == 2) {   // shared row sums, one frame per block
+        dim3 grid((2 * w + P2_T - 1) / P2_T, (2 * h + P2_T - 1) / P2_T, nF);
+        hipLaunchKernelGGL(k_bic_prep2, grid, dim3(256), 0, s, px, U, w, h);
+        return hipGetLastError();
+    }
+    dim3 grid((2 * w + P2_T - 1) / P2_T, (2 * h + P2_T - 1) / P2_T, (nF + 3) / 4);
+    hipLaunchKernelGGL(k_bic_prep4, grid, dim3(256), 0, s, px, U, w, h, nF);
     return hipGetLastError();
 }
 

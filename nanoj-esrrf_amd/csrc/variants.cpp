@@ -18,7 +18,8 @@ int env_int(const char *name, int dflt)
 KernelVariants kernel_variants_from_env()
 {
     KernelVariants k;
-    k.bic_prep_per_point = env_int("ESRRF_BIC_PREP", 0) == 1;
+    k.bic_prep = env_int("ESRRF_BIC_PREP", 4);
+    if (k.bic_prep != 1 && k.bic_prep != 2) k.bic_prep = 4;
     k.bic_stage = env_int("ESRRF_BIC_STAGE", 0) != 0;
     k.bic_pair_direct = env_int("ESRRF_BIC_PAIR_DIRECT", 0) != 0;
     k.fht_scatter = env_int("ESRRF_FHT_SCATTER", 0) != 0;

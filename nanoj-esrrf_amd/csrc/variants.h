@@ -7,7 +7,8 @@
 namespace esrrf {
 
 struct KernelVariants {
-    int bic_prep_per_point = 0;   // ESRRF_BIC_PREP=1: one 4x4 stencil per sample point
+    int bic_prep = 4;             // ESRRF_BIC_PREP=1: one 4x4 stencil per sample point;
+                                  // 2: shared row sums, one frame per block
     int bic_stage = 0;            // ESRRF_BIC_STAGE=1: single-frame kernel stages in registers
     int bic_pair_direct = 0;      // ESRRF_BIC_PAIR_DIRECT=1: pair kernel fills after compute
     int fht_scatter = 0;          // ESRRF_FHT_SCATTER=1: pixel-by-pixel last FHT pass

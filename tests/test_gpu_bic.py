@@ -304,20 +304,23 @@ def test_bic_constant_stack(pkg):
 
 @pytest.mark.parametrize("w,h,M,loads", [
     (64, 64, 5, [(0, 6)]), (45, 37, 3, [(0, 2), (2, 5)]), (7, 9, 2, [(0, 4)]),
-    (100, 33, 4, [(0, 3)]),
+    (100, 33, 4, [(0, 3)]), (40, 36, 5, [(0, 1), (1, 8), (8, 11)]),
 ])
 def test_bic_prep_shared_row_sums_match_per_point_stencils(pkg, monkeypatch, w, h, M, loads):
-    """k_bic_prep2 (row sums shared per block) and k_bic_prep (a 4x4 stencil per sample point)
-    feed the RGC the same unit gradients: the results agree bit for bit (tiles that overhang
-    the image, border points on the bilinear-extrapolation branch, tiny frames)."""
+    """k_bic_prep4 (four frames per block, the default), k_bic_prep2 (row sums shared per
+    block) and k_bic_prep (a 4x4 stencil per sample point) feed the RGC the same unit
+    gradients: the results agree bit for bit (tiles that overhang the image, border points on
+    the bilinear-extrapolation branch, tiny frames, loads of 1 to 3 frames past a multiple
+    of four)."""
     n = loads[-1][1]
     st = stack(n, w, h, seed=300 + w)
     res = {}
-    for v in ("1", "2"):
+    for v in ("1", "2", "4"):
         monkeypatch.setenv("ESRRF_BIC_PREP", v)
         res[v], _, _ = run_engine(pkg, st, M, 1.5, 2, nOnGPU=max(b - a for a, b in loads),
                                   loads=loads)
     assert np.array_equal(res["1"], res["2"])
+    assert np.array_equal(res["1"], res["4"])
 
 
 @pytest.mark.parametrize("n,chunks", [(130, 1), (130, 2), (301, 4), (301, 16), (257, 3)])
