@@ -21,14 +21,24 @@ __device__ __forceinline__ float apply_sens(float C, float sens, int sens_int)
 // One frame of the temporal accumulation (liveSRRF.cl:363-393) from the normalised CGLH C0
 // and the interpolated intensity v: t = v C^S (C^S without intensity weighting); the first
 // frame of an SR image initialises; Prev holds t/(n-1) (the BIC TAC2 quirk, :374,377).
+// A frame's accumulation terms from the normalised CGLH C0 and the intensity v: t = v C^S and
+// e2 = v C^S v C^S (C^S and C^S C^S without intensity weighting) (liveSRRF.cl:357,365-366,379-380).
+template <bool INTW>
+__device__ __forceinline__ void bic_terms(float C0, float v, float sens, int sens_int, float &t,
+                                          float &e2)
+{
+    const float C = apply_sens(C0, sens, sens_int);
+    t = INTW ? __fmul_rn(v, C) : C;
+    e2 = INTW ? __fmul_rn(__fmul_rn(t, v), C) : __fmul_rn(C, C);
+}
+
 template <bool INTW>
 __device__ __forceinline__ void cumulate_bic(float &avg, float &e2s, float &tac, float &in,
                                              float &prv, float C0, float v, bool first,
                                              float sens, int sens_int, float rn, float rn1)
 {
-    const float C = apply_sens(C0, sens, sens_int);
-    const float t = INTW ? __fmul_rn(v, C) : C;
-    const float e2 = INTW ? __fmul_rn(__fmul_rn(t, v), C) : __fmul_rn(C, C);
+    float t, e2;
+    bic_terms<INTW>(C0, v, sens, sens_int, t, e2);
     if (first) {
         avg = __fmul_rn(t, rn);
         e2s = __fmul_rn(e2, rn);

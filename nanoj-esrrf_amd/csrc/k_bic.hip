@@ -1307,14 +1307,15 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc2(const BicRgcArgs a,
 // halves).  One set of barriers per four frames.  Taps are walked two at a time (every
 // phase's gated count is rounded up to an even number); each frame's sum takes the taps in
 // order and the cumulants are updated frame by frame, so results equal k_bic_rgc2's bit
-// for bit.
+// for bit.  MODE 1 emits each frame's normalised CGLH and intensity instead (the emit order
+// of k_bic_rgc2's MODE 1).
 // ---------------------------------------------------------------------------------------
 constexpr int QUAD_PP = 24;                    // parity-plane pitch the kernel assumes
 constexpr int QUAD_UPL = 4 * QUAD_PP * 14;     // float4 slots of one tile (parity planes of
                                                // up to 14 rows, TW <= 28)
 constexpr int RAW_P4 = 24;                     // raw-tile pitch (float4 elements)
 
-template <int PPW, bool INTW, int STG>
+template <int PPW, bool INTW, int STG, int MODE>
 __global__ void __launch_bounds__(1024)
 __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc4(const BicRgcArgs a,
                                                    const Tap *__restrict__ taps,
@@ -1374,7 +1375,7 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc4(const BicRgcArgs a,
             const int u0 = X + P.du, v0 = Y + P.dv;
             if (u0 > 0 && u0 < w - 2 && v0 > 0 && v0 < h - 2)
                 xbase[p] = (P.sx * RAW_T + (v0 - 1 - (Y0 - 2))) * 8 + lx;
-            if (!a.first_init && inimg) {
+            if (MODE == 0 && !a.first_init && inimg) {
                 const long pix = (long)(Y * M + P.sy) * a.wM + (X * M + P.sx);
                 s_avg[p] = a.out[pix];
                 s_e2[p] = a.out[pix + whM];
@@ -1595,6 +1596,18 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc4(const BicRgcArgs a,
                     for (int j = 0; j < 4; j++)
                         v[j] = interp_tile_s<4>(lRf + j, RAW_P4, X0 - 2, Y0 - 2, w, h, xf, yf);
                 }
+                if (MODE == 1) {
+                    if (inimg) {
+                        const long e = (((long)(a.frame0 + fi) * MM + ph) * h + Y) * w + X;
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (j < nf) {
+                                a.emitC[e + j * (long)MM * wh] = C0[j];
+                                a.emitV[e + j * (long)MM * wh] = v[j];
+                            }
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if (j < nf)
@@ -1614,7 +1627,7 @@ __attribute__((amdgpu_waves_per_eu(4))) k_bic_rgc4(const BicRgcArgs a,
         }
     }
 
-    if (inimg) {
+    if (MODE == 0 && inimg) {
 #pragma unroll
         for (int p = 0; p < PPW; p++) {
             const int ph = wave + p * a.nw;
@@ -1902,19 +1915,26 @@ bool bic_quad_fits(const BicRgcArgs &a)
            bic_quad_lds_bytes(a) <= 80 * 1024;
 }
 
-template <int PPW, bool INTW>
+template <int PPW, bool INTW, int MODE>
 static hipError_t launch_quad(const BicRgcArgs &a, hipStream_t s)
 {
     // register staging needs every tile element within 2 per thread
     const int nthr = 64 * a.nw, nU = 4 * a.PH * (a.TW >> 1);
     const size_t lds = bic_quad_lds_bytes(a);
     if (!a.pair_direct && nthr >= RAW_T * RAW_T && nU <= 2 * nthr)
-        hipLaunchKernelGGL((k_bic_rgc4<PPW, INTW, 2>), dim3(a.tiles_x * a.tiles_y), dim3(nthr),
-                           lds, s, a, a.taps, a.phases, a.tapk);
+        hipLaunchKernelGGL((k_bic_rgc4<PPW, INTW, 2, MODE>), dim3(a.tiles_x * a.tiles_y),
+                           dim3(nthr), lds, s, a, a.taps, a.phases, a.tapk);
     else
-        hipLaunchKernelGGL((k_bic_rgc4<PPW, INTW, 0>), dim3(a.tiles_x * a.tiles_y), dim3(nthr),
-                           lds, s, a, a.taps, a.phases, a.tapk);
+        hipLaunchKernelGGL((k_bic_rgc4<PPW, INTW, 0, MODE>), dim3(a.tiles_x * a.tiles_y),
+                           dim3(nthr), lds, s, a, a.taps, a.phases, a.tapk);
     return hipGetLastError();
+}
+
+template <int PPW>
+static hipError_t launch_quad_m(const BicRgcArgs &a, hipStream_t s)
+{
+    if (a.mode == 1) return launch_quad<PPW, false, 1>(a, s);
+    return a.int_weighting ? launch_quad<PPW, true, 0>(a, s) : launch_quad<PPW, false, 0>(a, s);
 }
 
 // taps_in_lds 0: the tap tables are read with scalar loads (no LDS copy)
@@ -1995,18 +2015,17 @@ static hipError_t launch_ppw(const BicRgcArgs &a, hipStream_t s)
 hipError_t launch_bic_rgc(const BicRgcArgs &a, int ppw, hipStream_t s, const char **kname)
 {
     if (a.nframes <= 0) return hipSuccess;
-    const bool quad = a.frames_per_pass == 4 && a.mode == 0 && ppw <= 4 && a.tac_prev_scaled &&
+    const bool quad = a.frames_per_pass == 4 && a.mode != 2 && ppw <= 4 && a.tac_prev_scaled &&
                       a.taps_in_lds && bic_quad_fits(a);
     const bool pair = !quad && a.frames_per_pass >= 2 && a.mode != 2 && ppw <= 4 &&
                       a.tac_prev_scaled && bic_pair_lds_bytes(a, !a.taps_in_lds) <= 64 * 1024;
     if (kname) *kname = quad ? "k_bic_rgc4" : pair ? "k_bic_rgc2" : "k_bic_rgc";
     if (quad) {
-        const bool iw = a.int_weighting != 0;
         switch (ppw) {
-        case 1: return iw ? launch_quad<1, true>(a, s) : launch_quad<1, false>(a, s);
-        case 2: return iw ? launch_quad<2, true>(a, s) : launch_quad<2, false>(a, s);
-        case 3: return iw ? launch_quad<3, true>(a, s) : launch_quad<3, false>(a, s);
-        default: return iw ? launch_quad<4, true>(a, s) : launch_quad<4, false>(a, s);
+        case 1: return launch_quad_m<1>(a, s);
+        case 2: return launch_quad_m<2>(a, s);
+        case 3: return launch_quad_m<3>(a, s);
+        default: return launch_quad_m<4>(a, s);
         }
     }
     if (pair) {

@@ -52,9 +52,9 @@ struct esrrf_rolling {
     float *d_px = nullptr;
     float2 *d_U = nullptr;
     float *d_C = nullptr, *d_V = nullptr, *d_dummy = nullptr;
-    float *d_state = nullptr; // [nslots][5][hM][wM]
+    float *d_state = nullptr; // [nslots][5][M*M][h][w]; [4][hM][wM] once complete
     float *d_mp = nullptr;
-    float *d_tmp = nullptr;   // read-out staging [4][hM][wM]
+    float *d_tmp = nullptr;   // image-order and read-out staging [4][hM][wM]
 };
 
 namespace {
@@ -238,6 +238,9 @@ int step_chunk(esrrf_rolling *r)
         if (win * gap + nF <= c1) done.push_back(win);
     for (int win : done) {
         float *out = slot_of(r, win);
+        LCHK(r, launch_roll_to_image(out, r->d_tmp, r->w, r->h, r->M, st));
+        LCHK(r, hipMemcpyAsync(out, r->d_tmp, sizeof(float) * 4 * whM, hipMemcpyDeviceToDevice,
+                               st));
         LCHK(r, launch_var(out, whM, st));
         if (r->mp) {
             LCHK(r, launch_mp_map(out, r->d_mp, r->w, r->h, 1, r->M, 1, st));

@@ -4,12 +4,13 @@
 
 namespace esrrf {
 
-constexpr int ROLL_GROUP = 16;    // windows per launch: one wave each
+constexpr int ROLL_GROUP = 16;    // windows per launch, held in registers per pixel
 constexpr int ROLL_MAX_WIN = 64;  // windows one chunk may touch (launched in groups)
 
 // The launch covers the consecutive windows r0 .. r0 + nwin - 1; window r starts at frame
 // r gap of the stack, the chunk at frame c0, and window r keeps its cumulants in slot
-// r mod nslots of `state` ([nslots][5][hM][wM]: AVG, E2, TAC2, INT, Prev).
+// r mod nslots of `state` ([nslots][5][M*M][h][w]: AVG, E2, TAC2, INT, Prev, in the emit
+// order of the frame values) until it is complete.
 struct RollAccArgs {
     const float *C, *V;           // emitted CGLH / intensity of the chunk [nframes][M*M][h][w]
     float *state;
@@ -25,5 +26,7 @@ struct RollAccArgs {
 };
 
 hipError_t launch_roll_acc(const RollAccArgs &a, hipStream_t s);
+// channels 0..3 of a complete window's slot to the image order [4][hM][wM] of dst
+hipError_t launch_roll_to_image(const float *src, float *dst, int w, int h, int M, hipStream_t s);
 
 }  // namespace esrrf

@@ -45,9 +45,15 @@ def per_window(pkg, st, M, fwhm, S, nF, gap, intw=True, mp=True):
     (45, 18, 16, 4, 2.0, 3, 8, 13, 6, False, False),    # frame gap > window: skipped frames
     (30, 16, 16, 5, 1.5, 2, 30, 0, 256, True, True),    # one window over the whole stack
     (64, 16, 20, 2, 1.5, 2, 20, 1, 9, True, True),      # a window at every frame
+    (48, 20, 18, 3, 1.5, 2, 16, 4, 10, True, False),    # float frames with negative values
+    (40, 16, 16, 2, 1.0, 1.5, 10, 3, 8, True, True),    # a non-integer sensitivity (powf)
 ])
 def test_rolling_matches_engine_per_window(pkg, n, w, h, M, fwhm, S, nF, gap, chunk, intw, mp):
     st = stack(n, w, h, seed=n + w)
+    if n == 48:
+        # intensities below zero: v C^S is -0 where the CGLH is 0, including at the first
+        # frame of windows, whose sums then start from a negative zero
+        st = st.astype(np.float32) - np.float32(st.mean())
     exp = per_window(pkg, st, M, fwhm, S, nF, gap, intw, mp)
     r = pkg.Rolling(w, h, M, fwhm, S, nF, gap, intw, mp, chunk_frames=chunk)
     nsr = r.begin(st)
