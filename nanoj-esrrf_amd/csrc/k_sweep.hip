@@ -29,6 +29,45 @@ __device__ __forceinline__ float sweep_sens(float C, float sens, int sens_int)
     }
 }
 
+// C^S of a thread's SG sensitivities for one frame.  Integer sensitivities 1..4 (POW false):
+// the four powers apply_sens forms (C, C^2, C^3 = C^2 C, C^4 = C^2 C^2 of C, negative or NaN
+// -> 0, liveSRRF.cl:358-359) are formed once and each sensitivity takes its power's bits
+// through masks set once per thread (all ones for its power, zero for the others): no branch
+// in the frame loop, the same values as sweep_sens.
+template <int SG, bool POW>
+struct SensPow {
+    unsigned m[SG][4];
+    float sens[SG];
+    int sint[SG];
+    __device__ __forceinline__ void init(const float *s, const int *si, int si0, int nS)
+    {
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            const int k = min(si0 + g, nS - 1);
+            sens[g] = s[k];
+            sint[g] = si[k];
+#pragma unroll
+            for (int e = 0; e < 4; e++) m[g][e] = sint[g] == e + 1 ? 0xffffffffu : 0u;
+        }
+    }
+    __device__ __forceinline__ void apply(float C, float (&Cs)[SG]) const
+    {
+        if (POW) {
+#pragma unroll
+            for (int g = 0; g < SG; g++) Cs[g] = sweep_sens<true>(C, sens[g], sint[g]);
+            return;
+        }
+        const float c1 = C >= 0.0f ? C : 0.0f;
+        const float c2 = c1 * c1;
+        const unsigned b1 = __float_as_uint(c1), b2 = __float_as_uint(c2),
+                       b3 = __float_as_uint(c2 * c1), b4 = __float_as_uint(c2 * c2);
+#pragma unroll
+        for (int g = 0; g < SG; g++)
+            Cs[g] = __uint_as_float((b1 & m[g][0]) | (b2 & m[g][1]) | (b3 & m[g][2]) |
+                                    (b4 & m[g][3]));
+    }
+};
+
 // grid.x: pixels in emit order (phase, Y, X); grid.y: groups of SG sensitivities.  A thread
 // reads each frame's CGLH and intensity of its pixel once and updates the cumulants of the
 // NQ frame counts of each of its SG sensitivities (the frames are read once per group, not
@@ -42,14 +81,8 @@ __global__ void __launch_bounds__(256) k_sweep_acc(const SweepAccArgs a)
     const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n) return;
     const int si0 = blockIdx.y * SG;
-    float sens[SG];
-    int sint[SG];
-#pragma unroll
-    for (int g = 0; g < SG; g++) {
-        const int si = min(si0 + g, a.nS - 1);
-        sens[g] = a.sens[si];
-        sint[g] = a.sens_int[si];
-    }
+    SensPow<SG, POW> sp;
+    sp.init(a.sens, a.sens_int, si0, a.nS);
     const int ph = (int)(o / wh);
     const long r = o - (long)ph * wh;
     const int Y = (int)(r / a.w), X = (int)(r - (long)Y * a.w);
@@ -70,33 +103,57 @@ __global__ void __launch_bounds__(256) k_sweep_acc(const SweepAccArgs a)
 #pragma unroll
     for (int g = 0; g < SG; g++) tp[g] = 0.0f;
     const bool intw = a.int_weighting != 0;
-#pragma unroll 4
-    for (int f = 0; f < a.nframes; f++) {
-        const float C = a.C[(long)f * n + o];
-        const float v = a.V[(long)f * n + o];
+    if (a.nframes <= 0) return;
+    {   // frame 0 initialises every configuration (each holds at least one frame)
+        const float C = a.C[o];
+        const float v = a.V[o];
 #pragma unroll
-        for (int q = 0; q < NQ; q++)
-            if (q < a.nq && f < a.nf[q]) s3[q] = f == 0 ? v * a.rn[q] : s3[q] + v * a.rn[q];
+        for (int q = 0; q < NQ; q++) s3[q] = v * a.rn[q];
+        float Csg[SG];
+        sp.apply(C, Csg);
 #pragma unroll
         for (int g = 0; g < SG; g++) {
-            const float Cs = sweep_sens<POW>(C, sens[g], sint[g]);
+            const float Cs = Csg[g];
             const float t = intw ? v * Cs : Cs;
             const float e2 = intw ? (t * v) * Cs : Cs * Cs;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                if (q < a.nq && f < a.nf[q]) {
-                    const float rn = a.rn[q], rn1 = a.rn1[q];
-                    if (f == 0) {
-                        s0[g][q] = t * rn;
-                        s1[g][q] = e2 * rn;
-                        s2[g][q] = 0.0f;
-                    } else {
-                        s0[g][q] = s0[g][q] + t * rn;
-                        s1[g][q] = s1[g][q] + e2 * rn;
-                        // BIC: Prev = t/(n-1) (liveSRRF.cl:374,377)
-                        s2[g][q] = s2[g][q] + (t * (tp[g] * rn1)) * rn1;
-                    }
-                }
+                s0[g][q] = t * a.rn[q];
+                s1[g][q] = e2 * a.rn[q];
+                s2[g][q] = 0.0f;
+            }
+            tp[g] = t;
+        }
+    }
+    // frames 1 .. nf[q] - 1 of configuration q.  Frame 0 outside the loop and 1/n, 1/(n-1)
+    // set to 0 (a scalar select) for a configuration past its last frame keep the loop body
+    // free of branches: such a configuration adds products t * 0 = +-0, which leave its sums'
+    // values unchanged.
+#pragma unroll 4
+    for (int f = 1; f < a.nframes; f++) {
+        const float C = a.C[(long)f * n + o];
+        const float v = a.V[(long)f * n + o];
+        float rnq[NQ], rn1q[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const bool on = q < a.nq && f < a.nf[q];
+            rnq[q] = on ? a.rn[q] : 0.0f;
+            rn1q[q] = on ? a.rn1[q] : 0.0f;
+            s3[q] = s3[q] + v * rnq[q];
+        }
+        float Csg[SG];
+        sp.apply(C, Csg);
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            const float Cs = Csg[g];
+            const float t = intw ? v * Cs : Cs;
+            const float e2 = intw ? (t * v) * Cs : Cs * Cs;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                s0[g][q] = s0[g][q] + t * rnq[q];
+                s1[g][q] = s1[g][q] + e2 * rnq[q];
+                // BIC: Prev = t/(n-1) (liveSRRF.cl:374,377)
+                s2[g][q] = s2[g][q] + (t * (tp[g] * rn1q[q])) * rn1q[q];
             }
             tp[g] = t;
         }
@@ -129,14 +186,8 @@ __global__ void __launch_bounds__(256) k_sweep_acc_fht(const SweepAccArgs a)
     const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n) return;
     const int si0 = blockIdx.y * SG;
-    float sens[SG];
-    int sint[SG];
-#pragma unroll
-    for (int g = 0; g < SG; g++) {
-        const int si = min(si0 + g, a.nS - 1);
-        sens[g] = a.sens[si];
-        sint[g] = a.sens_int[si];
-    }
+    SensPow<SG, POW> sp;
+    sp.init(a.sens, a.sens_int, si0, a.nS);
     float s0[SG][NQ], s1[SG][NQ], s2[SG][NQ], s3[NQ], tp[SG];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
@@ -155,9 +206,11 @@ __global__ void __launch_bounds__(256) k_sweep_acc_fht(const SweepAccArgs a)
         for (int q = 0; q < NQ; q++)
             if (q < a.nq && f < a.nf[q])
                 s3[q] = f == 0 ? __fmul_rn(v, a.rn[q]) : fmaf(v, a.rn[q], s3[q]);
+        float Csg[SG];
+        sp.apply(C, Csg);
 #pragma unroll
         for (int g = 0; g < SG; g++) {
-            const float Cs = sweep_sens<POW>(C, sens[g], sint[g]);
+            const float Cs = Csg[g];
             const float t = intw ? __fmul_rn(v, Cs) : Cs;
             const float e2 = intw ? __fmul_rn(__fmul_rn(t, v), Cs) : __fmul_rn(Cs, Cs);
 #pragma unroll
